@@ -1,0 +1,183 @@
+"""NP-MD throughput of the per-step path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128]
+
+Workload: BaTiO3 bulk 128^3 supercell (2,097,152 cells), the bulk
+phase-transition parameter set, Nose-Poincare MD ('md') at 300 K, dt = 2 fs,
+acoustic displacement optimised each step, FP64 plain mode.  Initial u is
+uniform in [-0.05, 0.05] A per component from a fixed seed, p = 0.  State and
+tables are resident in HBM before the timed region; K steps run back to back
+on the library's stream and are bracketed by device synchronisation.
+
+One JSON line is printed.  ``roofline`` is the dominant kernel's algorithmic
+HBM bytes per launch over its average launch time, taken with HIP events on
+the library's stream in a second pass of the same K steps right after the
+timed one (events bracket every launch there, so that pass is not the one
+``value`` comes from).  ``cpu_baseline`` times the CPU FP64 oracle (the
+restatement of the reference in C, OpenMP over all host threads) on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes(name: str, N: int, Kh: int, nf: int, materialize_frac: float) -> float:
+    """Algorithmic HBM bytes of one launch (compulsory reads + writes), FP64."""
+    if name == "xfwd_drift_r2c":      # read u, pk, code; write u, nf half spectra
+        return 3 * 8 * N * 3 + N + nf * 16 * Kh
+    if name in ("y_forward", "y_inverse"):
+        return 2 * nf * 16 * Kh
+    if name == "z_dipole_dipole":     # 3 spectra in/out + 6-word L(k) table
+        return 2 * 3 * 16 * Kh + 6 * 8 * Kh
+    if name == "z_strain":            # 6 spectra in/out
+        return 2 * (nf - 3) * 16 * Kh
+    if name == "xinv_c2r_local_kick":  # read nf spectra, u, pk, code; write pk (+p, F when materialised)
+        return nf * 16 * Kh + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
+    return 0.0
+
+
+def step_bytes(N: int, Kh: int) -> float:
+    """Compulsory bytes of a 3-sweep fused md step (18 N w_s + 78 Kh w_f, FP64)."""
+    return 18 * N * 8 + 78 * Kh * 8
+
+
+def cpu_baseline(p, u0, steps: int) -> dict:
+    import oracle
+    oracle.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = oracle.Oracle(p)
+    o.field("dipoR")[...] = u0
+    e = o.md_prologue()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.nose_poincare(e)
+    dt = time.perf_counter() - t0
+    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} NP steps of the same {p.Lx}^3 workload with the C FP64 oracle "
+                      f"(OpenMP, {threads} threads), after its untimed table set-up"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
+              "(replica ensembles shard with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from feram_amd.params import batio3_zvr
+    from feram_amd.system import System
+
+    L = args.size
+    p = batio3_zvr(L, method="md", kelvin=300.0)
+    rng = np.random.default_rng(20241203)
+    u0 = rng.uniform(-0.05, 0.05, size=(L, L, L, 3))
+
+    print(f"[bench] creating {L}^3 context", file=sys.stderr, flush=True)
+    g = System(p)
+    g.put("dipoR", u0)
+    g.md_prologue()
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        g.nose_poincare(args.warmup)
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        have_torch = False
+
+    def sync():
+        g.synchronize()
+        if have_torch:
+            torch.cuda.synchronize()
+
+    sync()
+    t0 = time.perf_counter()
+    e = g.nose_poincare(args.steps)
+    sync()
+    dt = time.perf_counter() - t0
+    steps_per_s = args.steps / dt
+    print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel timing pass (events around every launch)
+    g.profile(True)
+    g.nose_poincare(args.steps)
+    sync()
+    stats = g.kernel_stats()
+    g.profile(False)
+    N = p.N
+    Kh = (L // 2 + 1) * L * L
+    nf = 9
+    per = {}
+    for name, (ms, cnt) in stats.items():
+        avg_ms = ms / cnt
+        b = kernel_bytes(name, N, Kh, nf, 1.0 / args.steps)
+        per[name] = {"avg_us": avg_ms * 1e3, "launches": cnt,
+                     "GBs": (b / (avg_ms * 1e-3) / 1e9) if b else None, "bytes": b}
+    step_kernel_ms = sum(ms for ms, cnt in stats.values()) / args.steps
+    dom = max((k for k in per if per[k]["bytes"]), key=lambda k: per[k]["avg_us"] * per[k]["launches"])
+    achieved = per[dom]["GBs"]
+    traffic = None
+    tr_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tr_path):
+        with open(tr_path) as f:
+            traffic = json.load(f).get(f"{L}:{dom}")
+
+    cpu = None
+    if not args.no_cpu:
+        print(f"[bench] cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(p, u0, args.cpu_steps)
+
+    sb = step_bytes(N, Kh)
+    out = {
+        "metric": "NP-MD steps/s at 128^3 (FP64 plain mode)",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"BaTiO3 bulk {L}x{L}x{L} NP-MD ('md'), 300 K, dt=2 fs, optimised "
+                               "acoustic strain, FP64 plain, u0 ~ U[-0.05,0.05] A seeded",
+                   "L": L, "cells": N, "method": "md", "precision": "fp64",
+                   "cell_updates_per_s": steps_per_s * N,
+                   "total_energy_eV_per_uc": e["total"] / N},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": traffic, "bytes_per_launch": per[dom]["bytes"],
+                     "avg_launch_us": per[dom]["avg_us"]},
+        "step_roofline": {"algorithmic_bytes_per_step": sb,
+                          "achieved_GBs": sb * steps_per_s / 1e9,
+                          "frac": sb * steps_per_s / 1e9 / HBM_PEAK_GBS,
+                          "kernel_sum_ms_per_step": step_kernel_ms},
+        "kernels": per,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

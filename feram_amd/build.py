@@ -1,0 +1,62 @@
+"""Build the gfx950 shared library libferam_hip.so in-tree with hipcc.
+
+    python -m feram_amd.build            # compile (always recompiles)
+
+Flags: -O3 for gfx950 only; packed FP32 instructions are disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops); host code keeps IEEE
+semantics with -ffp-contract=off on the host side.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libferam_hip.so")
+SOURCES = [os.path.join(CSRC, "feram_hip.hip")]
+HEADERS = [os.path.join(CSRC, "fhip_fft.hpp"), os.path.join(CSRC, "fhip_kernels.hpp"),
+           os.path.join(ROOT, "include", "feram_hip.h")]
+ARCH = os.environ.get("FHIP_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def command(out: str = LIB) -> list:
+    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Xarch_host", "-ffp-contract=off",
+            "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+            "-I", os.path.join(ROOT, "include"), "-o", out] + SOURCES
+
+
+def build(verbose: bool = True) -> str:
+    tmp = LIB + ".tmp"
+    cmd = command(tmp)
+    t0 = time.time()
+    if verbose:
+        print("[feram_amd.build]", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+    os.replace(tmp, LIB)
+    if verbose:
+        warn = [l for l in res.stderr.splitlines() if "warning" in l and "packed-fp32" not in l]
+        for l in warn[:20]:
+            print(l)
+        print(f"[feram_amd.build] built {LIB} in {time.time() - t0:.1f} s", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,1038 @@
+// feram_hip.hip -- context, set-up and step orchestration behind the C ABI of
+// include/feram_hip.h.  One HIP stream per context; every kernel of a step is
+// launched on it back to back, and the host only synchronises when it reads
+// energies or fields.
+#include "../../include/feram_hip.h"
+#include "fhip_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fhip;
+
+namespace {
+
+constexpr double M_PI_D = 3.14159265358979323846;
+constexpr double HARTREE_BOHR = 14.399644;      // define.h:32
+constexpr double KELVIN_EV = 11604.45;          // define.h:33
+constexpr double ELEMENTARY_CHARGE = 1.60217653;
+
+#define FHIP_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+bool size_supported(int n)
+{
+    switch (n) {
+#define FHIP_CASE(n) case n:
+        FHIP_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+        return true;
+    default:
+        return false;
+    }
+}
+
+int tile_rows(int n)   // must agree with tile_of<n>::value
+{
+    int p = 1;
+    while (2 * p * n <= 1024) p *= 2;
+    return p;
+}
+
+enum KernelId {
+    K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
+    K_KE, K_SCALE, K_SETUP, K_NKERNEL
+};
+const char *kKernelNames[K_NKERNEL] = {
+    "xfwd_drift_r2c", "y_forward", "z_dipole_dipole", "z_strain", "homo_strain", "y_inverse",
+    "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup"};
+
+}  // namespace
+
+struct fhip_ctx {
+    fhip_params p{};
+    int Lx = 0, Ly = 0, Lz = 0, Kx = 0, N = 0, Kh = 0, nslots = 0;
+    int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0, maxblocks = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Consts c{};
+    ScalarConsts sk{};
+    Fields f{};
+    Spectra<double> sp{};
+    Scalars *dsc = nullptr;
+    Scalars *hsc = nullptr;           // pinned mirror
+    double *partials = nullptr;
+    double *Ltab = nullptr;
+    cpx<double> *modk = nullptr, *acouK = nullptr;
+    cpx<double> *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
+    unsigned char *code = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    bool acouK_valid = false;
+    // measurement
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int id; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    double kms[K_NKERNEL] = {0};
+    int64_t kcount[K_NKERNEL] = {0};
+};
+
+namespace {
+
+int fail(fhip_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(ctx, FHIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename P>
+int dalloc(fhip_ctx *ctx, P **ptr, size_t count)
+{
+    void *v = nullptr;
+    size_t nb = count * sizeof(P);
+    if (nb == 0) nb = 16;
+    hipError_t e = hipMalloc(&v, nb);
+    if (e != hipSuccess) return fail(ctx, FHIP_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(v, 0, nb, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, FHIP_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    ctx->allocs.push_back(v);
+    ctx->bytes += (int64_t)nb;
+    *ptr = (P *)v;
+    return FHIP_OK;
+}
+
+hipEvent_t take_event(fhip_ctx *ctx)
+{
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch helper: brackets the launch with events when measuring.
+template <typename F>
+int launch(fhip_ctx *ctx, int id, F &&fn)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (a) hipEventRecord(a, ctx->stream);
+    }
+    fn();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(ctx, FHIP_ERR_HIP, std::string("launch ") + kKernelNames[id] + ": " + hipGetErrorString(e));
+    if (ctx->prof && a && b) {
+        hipEventRecord(b, ctx->stream);
+        ctx->recs.push_back({id, a, b});
+    }
+    return FHIP_OK;
+}
+
+int collect_events(fhip_ctx *ctx)
+{
+    if (ctx->recs.empty()) return FHIP_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto &r : ctx->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ctx->kms[r.id] += ms;
+            ctx->kcount[r.id] += 1;
+        }
+        ctx->ev_pool.push_back(r.a);
+        ctx->ev_pool.push_back(r.b);
+    }
+    ctx->recs.clear();
+    return FHIP_OK;
+}
+
+#define RC(expr)                       \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != FHIP_OK) return rc_; \
+    } while (0)
+
+// ---- size dispatch ---------------------------------------------------------
+int run_xfwd(fhip_ctx *ctx, const XfwdArgs &a)
+{
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    Spectra<double> sp = ctx->sp;
+    const Scalars *sc = ctx->dsc;
+    const cpx<double> *tw = ctx->twx;
+    const int nb = ctx->xblocks;
+    hipStream_t s = ctx->stream;
+    switch (ctx->Lx) {
+#define FHIP_CASE(n)                                                                         \
+    case n:                                                                                  \
+        return launch(ctx, K_XFWD, [&] { k_xfwd<double, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+        FHIP_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx not supported");
+}
+
+int run_xinv(fhip_ctx *ctx, const XinvArgs &a)
+{
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    Spectra<double> sp = ctx->sp;
+    const Scalars *sc = ctx->dsc;
+    const cpx<double> *tw = ctx->twx;
+    const int nb = ctx->xblocks;
+    hipStream_t s = ctx->stream;
+    switch (ctx->Lx) {
+#define FHIP_CASE(n)                                                                         \
+    case n:                                                                                  \
+        return launch(ctx, K_XINV, [&] { k_xinv<double, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+        FHIP_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx not supported");
+}
+
+int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf)
+{
+    Consts c = ctx->c;
+    Spectra<double> sp = ctx->sp;
+    const cpx<double> *tw = ctx->twy;
+    dim3 grid(ctx->yblocks, nf);
+    hipStream_t s = ctx->stream;
+    const int id = sign < 0 ? K_YFWD : K_YINV;
+    switch (ctx->Ly) {
+#define FHIP_CASE(n)                                                                                   \
+    case n:                                                                                            \
+        if (sign < 0)                                                                                  \
+            return launch(ctx, id, [&] { k_ypass<double, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }); \
+        return launch(ctx, id, [&] { k_ypass<double, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); });
+        FHIP_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return fail(ctx, FHIP_ERR_UNSUPPORTED, "Ly not supported");
+}
+
+template <int G>
+int run_zpass_g(fhip_ctx *ctx, const ZArgsD &a)
+{
+    Consts c = ctx->c;
+    Spectra<double> sp = ctx->sp;
+    Scalars *sc = ctx->dsc;
+    const cpx<double> *tw = ctx->twz;
+    const int nb = ctx->zblocks;
+    hipStream_t s = ctx->stream;
+    const int id = (G == ZG_DD) ? K_ZDD : (G == ZG_PLAIN ? K_SETUP : K_ZSTRAIN);
+    switch (ctx->Lz) {
+#define FHIP_CASE(n)                                                                           \
+    case n:                                                                                    \
+        return launch(ctx, id, [&] { k_zpass<double, n, G><<<nb, NT, 0, s>>>(c, sp, a, sc, tw); });
+        FHIP_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lz not supported");
+}
+
+ZArgsD zargs(fhip_ctx *ctx, bool want_acouK)
+{
+    ZArgsD a{};
+    a.Ltab = ctx->Ltab;
+    a.csx = ctx->csx;
+    a.csy = ctx->csy;
+    a.csz = ctx->csz;
+    a.modk = ctx->modk;
+    a.acouK = want_acouK ? ctx->acouK : nullptr;
+    a.partials = ctx->partials;
+    a.maxblocks = ctx->maxblocks;
+    a.nplain = 0;
+    return a;
+}
+
+// ---- one force evaluation (mirror_force_mirror) inside a step ---------------
+int nforward(const fhip_ctx *ctx) { return ctx->c.acoustic ? 12 : 9; }
+
+enum StepKind { SK_FORCE = 0, SK_NP = 1, SK_LF = 2, SK_VS = 3 };
+
+int force_pipeline(fhip_ctx *ctx, int kind, bool materialize)
+{
+    const int nf = nforward(ctx);
+    XfwdArgs xa{};
+    xa.mode = (kind == SK_NP) ? XF_NP : (kind == SK_LF ? XF_LF : (kind == SK_VS ? XF_VS : XF_FORCE));
+    xa.nf = nf;
+    RC(run_xfwd(ctx, xa));
+    RC(run_ypass(ctx, -1, 0, nf));
+    ZArgsD za = zargs(ctx, materialize && !ctx->c.acoustic);
+    RC(run_zpass_g<ZG_DD>(ctx, za));
+    if (ctx->c.acoustic) RC(run_zpass_g<ZG_ACOU>(ctx, za));
+    else RC(run_zpass_g<ZG_STRAIN>(ctx, za));
+    if (materialize && !ctx->c.acoustic) ctx->acouK_valid = true;
+    {
+        ScalarConsts k = ctx->sk;
+        Scalars *sc = ctx->dsc;
+        double *part = ctx->partials;
+        int mb = ctx->maxblocks, zb = ctx->zblocks;
+        hipStream_t s = ctx->stream;
+        RC(launch(ctx, K_HOMO, [&] { k_homo<<<1, NT, 0, s>>>(k, sc, part, mb, zb); }));
+    }
+    RC(run_ypass(ctx, +1, 0, nf));
+    XinvArgs ia{};
+    ia.mode = (kind == SK_NP) ? XI_NP : (kind == SK_LF ? XI_LF : (kind == SK_VS ? XI_VS : XI_FORCE));
+    ia.nf = nf;
+    ia.materialize = materialize ? 1 : 0;
+    ia.partials = ctx->partials;
+    ia.maxblocks = ctx->maxblocks;
+    RC(run_xinv(ctx, ia));
+    {
+        ScalarConsts k = ctx->sk;
+        Scalars *sc = ctx->dsc;
+        double *part = ctx->partials;
+        int mb = ctx->maxblocks, xb = ctx->xblocks;
+        int mode = (kind == SK_NP) ? FIN_NP : (kind == SK_LF ? FIN_LF : (kind == SK_VS ? FIN_VS : FIN_FORCE));
+        int ac = ctx->c.acoustic;
+        hipStream_t s = ctx->stream;
+        RC(launch(ctx, K_FINISH, [&] { k_finish<<<1, NT, 0, s>>>(k, sc, part, mb, xb, mode, ac); }));
+    }
+    return FHIP_OK;
+}
+
+int pull_scalars(fhip_ctx *ctx)
+{
+    HIPCHK(hipMemcpyAsync(ctx->hsc, ctx->dsc, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return collect_events(ctx);
+}
+
+int push_scalars(fhip_ctx *ctx)
+{
+    HIPCHK(hipMemcpyAsync(ctx->dsc, ctx->hsc, sizeof(Scalars), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FHIP_OK;
+}
+
+void energies_out(const fhip_ctx *ctx, fhip_energy *e)
+{
+    if (!e) return;
+    const double *en = ctx->hsc->en;
+    e->acou_kinetic = en[E_ACOU_KIN];
+    e->dipo_kinetic = en[E_DIPO_KIN];
+    e->short_range = en[E_SR];
+    e->long_range = en[E_LR];
+    e->dipole_E_field = en[E_DE];
+    e->unharmonic = en[E_UNH];
+    e->homo_strain = en[E_HS];
+    e->homo_coupling = en[E_HC];
+    e->inho_strain = en[E_IS];
+    e->inho_coupling = en[E_IC];
+    e->inho_modulation = en[E_IM];
+    e->total = en[E_TOTAL];
+    e->H_Nose_Poincare = en[E_HNP];
+}
+
+int launch_kick(fhip_ctx *ctx, bool kick)
+{
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    const Scalars *sc = ctx->dsc;
+    double *part = ctx->partials;
+    int mb = ctx->maxblocks, kb = ctx->kblocks;
+    hipStream_t s = ctx->stream;
+    if (kick) return launch(ctx, K_KICK, [&] { k_kick<1><<<kb, NT, 0, s>>>(c, f, sc, part, mb); });
+    return launch(ctx, K_KE, [&] { k_kick<0><<<kb, NT, 0, s>>>(c, f, sc, part, mb); });
+}
+
+// ---- host-side layout conversion -------------------------------------------
+size_t padded_index(const fhip_ctx *ctx, int comp, int ncomp, int x, int y, int z)
+{
+    const size_t dx = 2 * (size_t)ctx->Kx, dy = (size_t)ctx->Ly + ctx->p.padding_y;
+    return (size_t)comp + ncomp * ((size_t)x + dx * ((size_t)y + dy * (size_t)z));
+}
+
+double **field_ptrs(fhip_ctx *ctx, int field)
+{
+    switch (field) {
+    case FHIP_DIPO_R: return ctx->f.u;
+    case FHIP_DIPO_P: return ctx->f.p;
+    case FHIP_DVDDI: return ctx->f.F;
+    case FHIP_ACOU_R: return ctx->f.w;
+    case FHIP_ACOU_P: return ctx->f.pw;
+    case FHIP_DVDAC: return ctx->f.dVdw;
+    case FHIP_E_LOCAL: return ctx->f.eloc;
+    default: return nullptr;
+    }
+}
+
+void fill_consts(fhip_ctx *ctx)
+{
+    const fhip_params &p = ctx->p;
+    Consts &c = ctx->c;
+    c.Lx = p.Lx; c.Ly = p.Ly; c.Lz = p.Lz;
+    c.Kx = ctx->Kx; c.N = ctx->N; c.Kh = ctx->Kh;
+    c.acoustic = p.mass_acou[0] > 0.0;
+    c.efield = !p.E_field_always_zero;
+    c.has_eloc = 0;
+    c.modulation = 0;
+    c.n_sum_p = ctx->N;
+    c.Ninv = 1.0 / ctx->N;
+    c.dt = p.dt;
+    c.dt_2 = p.dt / 2;
+    c.a0 = p.a0;
+    c.Zstar = p.Z_star[0];
+    c.Z2 = p.Z_star[0] * p.Z_star[0];
+    for (int s = 0; s < 3; ++s) {
+        c.kappa2[s] = p.P_kappa2[s];
+        c.fa2g[s] = 4 * p.P_alpha[s] + 2 * p.P_gamma[s];
+        c.tg[s] = 2 * p.P_gamma[s];
+        c.sk[s] = 6 * p.P_k1[s] + 2 * p.P_k2[s];
+        c.tk2[s] = 2 * p.P_k2[s];
+        c.fk[s] = 4 * p.P_k2[s] - 2 * p.P_k3[s];
+        c.k4x8[s] = 8 * p.P_k4[s];
+        c.alpha[s] = p.P_alpha[s];
+        c.gamma[s] = p.P_gamma[s];
+        c.k1[s] = p.P_k1[s];
+        c.k2[s] = p.P_k2[s];
+        c.k3[s] = p.P_k3[s];
+        c.k4[s] = p.P_k4[s];
+        c.minv[s] = 1.0 / p.mass_dipo[s];
+        c.mwinv[s] = 1.0 / p.mass_acou[s];
+    }
+    c.B11 = p.B11; c.B12 = p.B12; c.B44 = p.B44;
+    c.B1xx = p.B1xx; c.B1yy = p.B1yy; c.B4yz = p.B4yz;
+    c.mod_c = p.modulation_constant;
+    for (int i = 0; i < 7; ++i) c.j[i] = p.j[i];
+    // homogeneous_coupling_2 (elastic.F:55-71)
+    for (int i = 0; i < 36; ++i) c.hc2[i] = 0.0;
+    for (int i = 0; i < 3; ++i) c.hc2[i + 6 * i] = p.B1xx / 2;
+    for (int i = 3; i < 6; ++i) c.hc2[i + 6 * i] = p.B4yz;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) c.hc2[i + 6 * j] = p.B1yy / 2;
+
+    ScalarConsts &k = ctx->sk;
+    k.geometry = p.bulk_or_film;
+    k.method = p.method;
+    k.modulation_energy = p.modulation_constant != 0.0;
+    k.n_sum_p = ctx->N;
+    k.N = ctx->N;
+    k.Ninv = 1.0 / ctx->N;
+    k.dt = p.dt;
+    k.dt_2 = p.dt / 2;
+    k.Q = p.Q_Nose;
+    k.T_eV = p.kelvin / KELVIN_EV;
+    k.n_freedom = (double)(3 * ctx->N);
+    k.three_N = (double)(3 * ctx->N);
+    k.target_ke = p.kelvin * 1.5 * ctx->N / KELVIN_EV;
+    k.epi_strain = p.epi_strain;
+    const double V = p.a0 * p.a0 * p.a0;
+    for (int i = 0; i < 3; ++i) k.pNa03[i] = p.GPa[i] * 0.01 / ELEMENTARY_CHARGE * ctx->N * V;
+    // homogeneous and its Cholesky factor (elastic.F:26-52)
+    double *H = k.H, *G = k.G;
+    for (int i = 0; i < 36; ++i) { H[i] = 0.0; G[i] = 0.0; }
+    for (int i = 0; i < 3; ++i) H[i + 6 * i] = p.B11 * ctx->N;
+    for (int i = 3; i < 6; ++i) H[i + 6 * i] = p.B44 * ctx->N;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) H[i + 6 * j] = p.B12 * ctx->N;
+    for (int j = 0; j < 6; ++j) {
+        double s = 0.0;
+        for (int l = 0; l < j; ++l) s += G[j + 6 * l] * G[j + 6 * l];
+        G[j + 6 * j] = std::sqrt(H[j + 6 * j] - s);
+        k.Ginv[j] = 1.0 / G[j + 6 * j];
+        for (int i = j + 1; i < 6; ++i) {
+            double s2 = 0.0;
+            for (int l = 0; l < j; ++l) s2 += G[i + 6 * l] * G[j + 6 * l];
+            G[i + 6 * j] = (H[i + 6 * j] - s2) * k.Ginv[j];
+        }
+    }
+    for (int i = 0; i < 36; ++i) k.hc2[i] = c.hc2[i];
+}
+
+int upload_table(fhip_ctx *ctx, cpx<double> **dst, int n, bool twiddle)
+{
+    std::vector<cpx<double>> h(n);
+    for (int j = 0; j < n; ++j) {
+        const double a = 2.0 * M_PI_D * (double)j / (double)n;
+        h[j] = twiddle ? cpx<double>{std::cos(a), -std::sin(a)} : cpx<double>{std::cos(a), std::sin(a)};
+    }
+    RC(dalloc(ctx, dst, n));
+    HIPCHK(hipMemcpyAsync(*dst, h.data(), sizeof(cpx<double>) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FHIP_OK;
+}
+
+// Forward 3-D r2c of nf real device fields into spectral slots 0..nf-1.
+int forward_generic(fhip_ctx *ctx, const double *const *fields, int nf)
+{
+    XfwdArgs xa{};
+    xa.mode = XF_GENERIC;
+    xa.nf = nf;
+    for (int i = 0; i < nf; ++i) xa.gen[i] = fields[i];
+    RC(run_xfwd(ctx, xa));
+    RC(run_ypass(ctx, -1, 0, nf));
+    ZArgsD za = zargs(ctx, false);
+    za.nplain = nf;
+    return run_zpass_g<ZG_PLAIN>(ctx, za);
+}
+
+int build_long_range(fhip_ctx *ctx)
+{
+    const fhip_params &p = ctx->p;
+    double *tmp = nullptr;
+    void *raw = nullptr;
+    HIPCHK(hipMalloc(&raw, sizeof(double) * 6 * (size_t)ctx->N));
+    tmp = (double *)raw;
+    EwaldArgs e{};
+    e.max_x = 24 / p.Lx; e.max_y = 24 / p.Ly; e.max_z = 24 / p.Lz;
+    e.kappa = p.kappa > 0.0 ? p.kappa : std::sqrt(M_PI_D) / p.a0 / std::pow((double)ctx->N, 1.0 / 6);
+    e.a0 = p.a0;
+    e.Lx_a0 = p.Lx * p.a0; e.Ly_a0 = p.Ly * p.a0; e.Lz_a0 = p.Lz * p.a0;
+    e.four_pi_over_v = 4 * M_PI_D / (p.a0 * p.a0 * p.a0);
+    e.two_pi_over_a = 2 * M_PI_D / p.a0;
+    e.four_kappa2 = 4 * (e.kappa * e.kappa);
+    e.scale = HARTREE_BOHR / p.epsilon_inf;
+    Consts c = ctx->c;
+    hipStream_t s = ctx->stream;
+    int rc = launch(ctx, K_SETUP, [&] { k_ewald_real<<<ctx->kblocks, NT, 0, s>>>(c, e, tmp); });
+    if (rc == FHIP_OK) {
+        const double *flds[6];
+        for (int m = 0; m < 6; ++m) flds[m] = tmp + (size_t)m * ctx->N;
+        rc = forward_generic(ctx, flds, 6);
+    }
+    if (rc == FHIP_OK) {
+        Spectra<double> sp = ctx->sp;
+        double *L = ctx->Ltab;
+        const int kb = (ctx->Kh + NT - 1) / NT;
+        rc = launch(ctx, K_SETUP, [&] { k_ewald_recip<<<kb, NT, 0, s>>>(c, e, sp, L); });
+    }
+    hipError_t e2 = hipStreamSynchronize(ctx->stream);
+    hipFree(raw);
+    if (rc != FHIP_OK) return rc;
+    if (e2 != hipSuccess) return fail(ctx, FHIP_ERR_HIP, std::string("ewald set-up: ") + hipGetErrorString(e2));
+    return FHIP_OK;
+}
+
+int check_params(fhip_ctx *ctx, const fhip_params *p)
+{
+    if (!p) return fail(ctx, FHIP_ERR_ARG, "null params");
+    if (p->Lx <= 0 || p->Ly <= 0 || p->Lz <= 0 || p->padding_y < 0)
+        return fail(ctx, FHIP_ERR_ARG, "bad supercell size");
+    if (p->Lx % 2 != 0) return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx must be even");
+    if (!size_supported(p->Lx) || !size_supported(p->Ly) || !size_supported(p->Lz))
+        return fail(ctx, FHIP_ERR_UNSUPPORTED,
+                    "supercell edges must be one of 4 6 8 12 16 24 32 48 64 96 128 192 256");
+    if (p->method < 0 || p->method > 2) return fail(ctx, FHIP_ERR_ARG, "method must be md, vs or lf");
+    if (p->bulk_or_film == FHIP_FILM || p->bulk_or_film == FHIP_EPIT)
+        return fail(ctx, FHIP_ERR_UNSUPPORTED, "film and epit geometries (mirror) are not supported");
+    if (p->bulk_or_film < 0 || p->bulk_or_film > 3) return fail(ctx, FHIP_ERR_ARG, "bad geometry");
+    if (p->precision != FHIP_FP64) return fail(ctx, FHIP_ERR_UNSUPPORTED, "only FP64 is built");
+    if (p->method == FHIP_MD && p->mass_acou[0] > 0.0)
+        return fail(ctx, FHIP_ERR_ARG, "method 'md' cannot be used with acoustic_mass_amu (param_module.F:238-242)");
+    for (int s = 0; s < 3; ++s)
+        if (!(p->mass_dipo[s] > 0.0)) return fail(ctx, FHIP_ERR_ARG, "mass_dipo must be positive");
+    if (!(p->epsilon_inf > 0.0) || !(p->a0 > 0.0) || !(p->dt > 0.0))
+        return fail(ctx, FHIP_ERR_ARG, "a0, dt and epsilon_inf must be positive");
+    return FHIP_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char *fhip_last_error(const fhip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t fhip_device_bytes(const fhip_ctx *ctx) { return ctx ? ctx->bytes : 0; }
+
+void fhip_destroy(fhip_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (void *v : ctx->allocs) hipFree(v);
+    if (ctx->hsc) hipHostFree(ctx->hsc);
+    for (auto &r : ctx->recs) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->ev_pool) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int fhip_create(const fhip_params *p, fhip_ctx **out)
+{
+    if (!out) return FHIP_ERR_ARG;
+    *out = nullptr;
+    fhip_ctx *ctx = new fhip_ctx();
+    int rc = check_params(ctx, p);
+    if (rc != FHIP_OK) {
+        // keep the context so the caller can read the message
+        *out = ctx;
+        return rc;
+    }
+    *out = ctx;
+    ctx->p = *p;
+    ctx->Lx = p->Lx; ctx->Ly = p->Ly; ctx->Lz = p->Lz;
+    ctx->Kx = p->Lx / 2 + 1;
+    ctx->N = p->Lx * p->Ly * p->Lz;
+    ctx->Kh = ctx->Kx * p->Ly * p->Lz;
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    fill_consts(ctx);
+    ctx->nslots = ctx->c.acoustic ? 12 : 9;
+    ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
+    ctx->yblocks = (ctx->Kx * ctx->Lz + tile_rows(ctx->Ly) - 1) / tile_rows(ctx->Ly);
+    ctx->zblocks = (ctx->Kx * ctx->Ly + tile_rows(ctx->Lz) - 1) / tile_rows(ctx->Lz);
+    ctx->kblocks = (ctx->N + NT - 1) / NT;
+    ctx->maxblocks = std::max(std::max(ctx->xblocks, ctx->zblocks), ctx->kblocks);
+
+    const size_t N = ctx->N, Kh = ctx->Kh;
+    double *blk = nullptr;
+    // u, p, pk, F (3 each); w, p_w, dV/dw only with acoustic MD
+    const int nreal = ctx->c.acoustic ? 21 : 12;
+    RC(dalloc(ctx, &blk, nreal * N));
+    for (int q = 0; q < 3; ++q) {
+        ctx->f.u[q] = blk + (0 + q) * N;
+        ctx->f.p[q] = blk + (3 + q) * N;
+        ctx->f.pk[q] = blk + (6 + q) * N;
+        ctx->f.F[q] = blk + (9 + q) * N;
+        if (ctx->c.acoustic) {
+            ctx->f.w[q] = blk + (12 + q) * N;
+            ctx->f.pw[q] = blk + (15 + q) * N;
+            ctx->f.dVdw[q] = blk + (18 + q) * N;
+        }
+    }
+    RC(dalloc(ctx, &ctx->code, N));
+    {
+        std::vector<unsigned char> h(N, (unsigned char)8);
+        HIPCHK(hipMemcpyAsync(ctx->code, h.data(), N, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->f.code = ctx->code;
+    cpx<double> *spec = nullptr;
+    RC(dalloc(ctx, &spec, (size_t)ctx->nslots * Kh));
+    for (int s = 0; s < MAX_SLOTS; ++s) ctx->sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * Kh : nullptr;
+    RC(dalloc(ctx, &ctx->Ltab, 6 * Kh));
+    RC(dalloc(ctx, &ctx->acouK, 3 * Kh));
+    RC(dalloc(ctx, &ctx->partials, (size_t)P_NSLOT * ctx->maxblocks));
+    RC(dalloc(ctx, &ctx->dsc, 1));
+    {
+        void *h = nullptr;
+        HIPCHK(hipHostMalloc(&h, sizeof(Scalars), 0));
+        ctx->hsc = (Scalars *)h;
+        std::memset(ctx->hsc, 0, sizeof(Scalars));
+        ctx->hsc->s = 1.0;
+        ctx->hsc->scale_p = 1.0;
+        ctx->hsc->scale_pw = 1.0;
+        for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = p->external_E_field[q];
+        RC(push_scalars(ctx));
+    }
+    RC(upload_table(ctx, &ctx->twx, ctx->Lx, true));
+    RC(upload_table(ctx, &ctx->twy, ctx->Ly, true));
+    RC(upload_table(ctx, &ctx->twz, ctx->Lz, true));
+    RC(upload_table(ctx, &ctx->csx, ctx->Lx, false));
+    RC(upload_table(ctx, &ctx->csy, ctx->Ly, false));
+    RC(upload_table(ctx, &ctx->csz, ctx->Lz, false));
+    RC(build_long_range(ctx));
+    if (p->modulation_constant != 0.0) {
+        // the reference stops when modulation_constant is set without a .modulation
+        // file (elastic.F:226-233); the table is filled by fhip_set_modulation.
+        RC(dalloc(ctx, &ctx->modk, Kh));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FHIP_OK;
+}
+
+int fhip_set_long_range(fhip_ctx *ctx, const double *lr)
+{
+    if (!ctx || !lr) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    const int Kx = ctx->Kx, Ly = ctx->Ly, Lz = ctx->Lz;
+    const size_t dy = (size_t)Ly + ctx->p.padding_y;
+    std::vector<double> h(6 * (size_t)ctx->Kh);
+    const int ia[6] = {0, 1, 2, 1, 2, 0}, ib[6] = {0, 1, 2, 2, 0, 1};
+    for (int z = 0; z < Lz; ++z)
+        for (int y = 0; y < Ly; ++y)
+            for (int x = 0; x < Kx; ++x) {
+                const size_t k = x + (size_t)Kx * (y + (size_t)Ly * z);
+                const size_t base = 9 * (x + (size_t)Kx * (y + dy * z));
+                for (int m = 0; m < 6; ++m) h[(size_t)m * ctx->Kh + k] = lr[base + ia[m] + 3 * ib[m]];
+            }
+    HIPCHK(hipMemcpyAsync(ctx->Ltab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FHIP_OK;
+}
+
+int fhip_get_long_range(fhip_ctx *ctx, double *lr)
+{
+    if (!ctx || !lr) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    const int Kx = ctx->Kx, Ly = ctx->Ly, Lz = ctx->Lz;
+    const size_t dy = (size_t)Ly + ctx->p.padding_y;
+    std::vector<double> h(6 * (size_t)ctx->Kh);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->Ltab, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int ia[6] = {0, 1, 2, 1, 2, 0}, ib[6] = {0, 1, 2, 2, 0, 1};
+    for (int z = 0; z < Lz; ++z)
+        for (int y = 0; y < Ly; ++y)
+            for (int x = 0; x < Kx; ++x) {
+                const size_t k = x + (size_t)Kx * (y + (size_t)Ly * z);
+                const size_t base = 9 * (x + (size_t)Kx * (y + dy * z));
+                for (int m = 0; m < 6; ++m) {
+                    const double v = h[(size_t)m * ctx->Kh + k];
+                    lr[base + ia[m] + 3 * ib[m]] = v;
+                    lr[base + ib[m] + 3 * ia[m]] = v;
+                }
+            }
+    return FHIP_OK;
+}
+
+int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
+{
+    if (!ctx || !mod) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    if (ctx->p.modulation_constant == 0.0)
+        return fail(ctx, FHIP_ERR_ARG, "There is a .modulation file, but modulation_constant is zero (elastic.F:218-222)");
+    std::vector<double> h(ctx->N);
+    for (int z = 0; z < ctx->Lz; ++z)
+        for (int y = 0; y < ctx->Ly; ++y)
+            for (int x = 0; x < ctx->Lx; ++x)
+                h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] = mod[padded_index(ctx, 0, 1, x, y, z)];
+    double *d = ctx->f.F[0];      // scratch: overwritten by the next force evaluation
+    HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    const double *flds[1] = {d};
+    RC(forward_generic(ctx, flds, 1));
+    HIPCHK(hipMemcpyAsync(ctx->modk, ctx->sp.s[0], sizeof(cpx<double>) * ctx->Kh, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * ctx->N, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->c.modulation = 1;
+    return collect_events(ctx);
+}
+
+int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, const double *minv)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    const fhip_params &p = ctx->p;
+    std::vector<unsigned char> h(ctx->N);
+    int nsum = 0;
+    for (int z = 0; z < ctx->Lz; ++z)
+        for (int y = 0; y < ctx->Ly; ++y)
+            for (int x = 0; x < ctx->Lx; ++x) {
+                const size_t hi = padded_index(ctx, 0, 1, x, y, z);
+                int s = relaxor ? relaxor[hi] - 1 : 0;
+                if (s < 0 || s > 2) return fail(ctx, FHIP_ERR_ARG, "relaxor species must be 1..3");
+                if (p.Z_star[s] != p.Z_star[0])
+                    return fail(ctx, FHIP_ERR_UNSUPPORTED, "species with different Z_star are not supported");
+                const bool counted = sum_p ? sum_p[hi] != 0 : true;
+                bool frozen = false;
+                if (minv) {
+                    const double m = minv[hi];
+                    if (m == 0.0) frozen = true;
+                    else if (m != 1.0 / p.mass_dipo[s])
+                        return fail(ctx, FHIP_ERR_UNSUPPORTED, "mass_dipo_inv must be 1/mass_dipo(species) or 0");
+                }
+                h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] =
+                    (unsigned char)(s | (frozen ? 4 : 0) | (counted ? 8 : 0));
+                nsum += counted ? 1 : 0;
+            }
+    if (nsum == 0) return fail(ctx, FHIP_ERR_ARG, "no counted cell (sum_p all false)");
+    HIPCHK(hipMemcpyAsync(ctx->code, h.data(), ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->c.n_sum_p = nsum;
+    ctx->sk.n_sum_p = nsum;
+    return FHIP_OK;
+}
+
+static int put_impl(fhip_ctx *ctx, int field, const double *host, bool soa)
+{
+    if (!ctx || !host) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    if (field == FHIP_DVDDI || field == FHIP_DVDAC || field == FHIP_ACOU_R_OPT)
+        return fail(ctx, FHIP_ERR_ARG, "field is read only");
+    if (field == FHIP_E_LOCAL && !ctx->f.eloc[0]) {
+        double *blk = nullptr;
+        RC(dalloc(ctx, &blk, 3 * (size_t)ctx->N));
+        for (int q = 0; q < 3; ++q) ctx->f.eloc[q] = blk + q * (size_t)ctx->N;
+    }
+    double **d = field_ptrs(ctx, field);
+    if (!d || !d[0]) return fail(ctx, FHIP_ERR_ARG, "field not present in this configuration");
+    const size_t N = ctx->N;
+    std::vector<double> h(3 * N);
+    if (soa) {
+        std::memcpy(h.data(), host, sizeof(double) * 3 * N);
+    } else {
+        for (int z = 0; z < ctx->Lz; ++z)
+            for (int y = 0; y < ctx->Ly; ++y)
+                for (int x = 0; x < ctx->Lx; ++x)
+                    for (int q = 0; q < 3; ++q)
+                        h[q * N + x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] = host[padded_index(ctx, q, 3, x, y, z)];
+    }
+    for (int q = 0; q < 3; ++q)
+        HIPCHK(hipMemcpyAsync(d[q], h.data() + q * N, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (field == FHIP_E_LOCAL) ctx->c.has_eloc = 1;
+    return FHIP_OK;
+}
+
+static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h);
+
+static int get_impl(fhip_ctx *ctx, int field, double *host, bool soa)
+{
+    if (!ctx || !host) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    const size_t N = ctx->N;
+    std::vector<double> h(3 * N);
+    if (field == FHIP_ACOU_R_OPT) {
+        RC(reconstruct_w(ctx, h));
+    } else {
+        double **d = field_ptrs(ctx, field);
+        if (!d || !d[0]) return fail(ctx, FHIP_ERR_ARG, "field not present in this configuration");
+        for (int q = 0; q < 3; ++q)
+            HIPCHK(hipMemcpyAsync(h.data() + q * N, d[q], sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if (soa) {
+        std::memcpy(host, h.data(), sizeof(double) * 3 * N);
+    } else {
+        for (int z = 0; z < ctx->Lz; ++z)
+            for (int y = 0; y < ctx->Ly; ++y)
+                for (int x = 0; x < ctx->Lx; ++x)
+                    for (int q = 0; q < 3; ++q)
+                        host[padded_index(ctx, q, 3, x, y, z)] = h[q * N + x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)];
+    }
+    return FHIP_OK;
+}
+
+int fhip_put(fhip_ctx *ctx, int field, const double *host) { return put_impl(ctx, field, host, false); }
+int fhip_get(fhip_ctx *ctx, int field, double *host) { return get_impl(ctx, field, host, false); }
+int fhip_put_soa(fhip_ctx *ctx, int field, const double *host) { return put_impl(ctx, field, host, true); }
+int fhip_get_soa(fhip_ctx *ctx, int field, double *host) { return get_impl(ctx, field, host, true); }
+
+int fhip_get_nose(fhip_ctx *ctx, double *s, double *pi, double *H0)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    RC(pull_scalars(ctx));
+    if (s) *s = ctx->hsc->s;
+    if (pi) *pi = ctx->hsc->pi;
+    if (H0) *H0 = ctx->hsc->H0;
+    return FHIP_OK;
+}
+
+int fhip_set_nose(fhip_ctx *ctx, double s, double pi, double H0)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (!(s > 0.0)) return fail(ctx, FHIP_ERR_ARG, "s_Nose must be positive");
+    RC(pull_scalars(ctx));
+    ctx->hsc->s = s;
+    ctx->hsc->pi = pi;
+    ctx->hsc->H0 = H0;
+    return push_scalars(ctx);
+}
+
+int fhip_get_strain(fhip_ctx *ctx, double strain[6])
+{
+    if (!ctx || !strain) return FHIP_ERR_ARG;
+    RC(pull_scalars(ctx));
+    for (int i = 0; i < 6; ++i) strain[i] = ctx->hsc->strain[i];
+    return FHIP_OK;
+}
+
+int fhip_set_E_field_add(fhip_ctx *ctx, const double E[3])
+{
+    if (!ctx || !E) return FHIP_ERR_ARG;
+    RC(pull_scalars(ctx));
+    for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = E[q];
+    return push_scalars(ctx);
+}
+
+int fhip_force(fhip_ctx *ctx, fhip_energy *e)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    RC(force_pipeline(ctx, SK_FORCE, true));
+    RC(pull_scalars(ctx));
+    energies_out(ctx, e);
+    return FHIP_OK;
+}
+
+int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    RC(force_pipeline(ctx, SK_FORCE, true));
+    RC(launch_kick(ctx, false));
+    {
+        ScalarConsts k = ctx->sk;
+        Scalars *sc = ctx->dsc;
+        double *part = ctx->partials;
+        int mb = ctx->maxblocks, kb = ctx->kblocks;
+        hipStream_t s = ctx->stream;
+        RC(launch(ctx, K_KE, [&] { k_ke_report<<<1, NT, 0, s>>>(k, sc, part, mb, kb); }));
+    }
+    RC(pull_scalars(ctx));
+    // initialize-dipoR.F:64-68 and molecular-dynamics.F:44-55, in FP64 on the host
+    Scalars &h = *ctx->hsc;
+    h.s = 1.0;
+    h.pi = 0.0;
+    double *en = h.en;
+    en[E_HNP] = 0.0;
+    const double V = en[E_SR] + en[E_LR] + en[E_DE] + en[E_UNH] + en[E_HS] + en[E_HC] + en[E_IS] + en[E_IC] + en[E_IM];
+    en[E_TOTAL] = en[E_DIPO_KIN] / (h.s * h.s) + en[E_ACOU_KIN] + V;
+    auto np_h = [&]() {
+        const double H = en[E_DIPO_KIN] / (h.s * h.s) + V + h.pi * h.pi / 2 / ctx->sk.Q
+                       + ctx->sk.three_N * ctx->sk.T_eV * std::log(h.s) - h.H0;
+        return H * h.s;
+    };
+    h.H0 = 0.0;
+    en[E_HNP] = np_h();
+    h.H0 = en[E_HNP] / h.s;
+    en[E_HNP] = np_h();
+    RC(push_scalars(ctx));
+    energies_out(ctx, e);
+    return FHIP_OK;
+}
+
+int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (nsteps < 1) return fail(ctx, FHIP_ERR_ARG, "nsteps must be >= 1");
+    if (ctx->p.method != FHIP_MD) return fail(ctx, FHIP_ERR_STATE, "context was created for another method");
+    RC(launch_kick(ctx, true));
+    for (int i = 0; i < nsteps; ++i) {
+        {
+            ScalarConsts k = ctx->sk;
+            Scalars *sc = ctx->dsc;
+            double *part = ctx->partials;
+            int mb = ctx->maxblocks, kb = ctx->kblocks, fold_ke = (i == 0);
+            hipStream_t s = ctx->stream;
+            RC(launch(ctx, K_NPBEGIN, [&] { k_np_begin<<<1, NT, 0, s>>>(k, sc, part, mb, kb, fold_ke); }));
+        }
+        RC(force_pipeline(ctx, SK_NP, i == nsteps - 1));
+    }
+    RC(pull_scalars(ctx));
+    energies_out(ctx, e);
+    return FHIP_OK;
+}
+
+int fhip_leapfrog(fhip_ctx *ctx, int i_step, int nsteps, fhip_energy *e)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (nsteps < 1) return fail(ctx, FHIP_ERR_ARG, "nsteps must be >= 1");
+    if (ctx->p.method != FHIP_LF) return fail(ctx, FHIP_ERR_STATE, "context was created for another method");
+    if (i_step == 0)
+        return fail(ctx, FHIP_ERR_UNSUPPORTED,
+                    "the i_step == 0 half kick is not reachable from molecular_dynamics (molecular-dynamics.F:64)");
+    for (int i = 0; i < nsteps; ++i) RC(force_pipeline(ctx, SK_LF, i == nsteps - 1));
+    RC(pull_scalars(ctx));
+    energies_out(ctx, e);
+    return FHIP_OK;
+}
+
+int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (nsteps < 1) return fail(ctx, FHIP_ERR_ARG, "nsteps must be >= 1");
+    if (ctx->p.method != FHIP_VS) return fail(ctx, FHIP_ERR_STATE, "context was created for another method");
+    for (int i = 0; i < nsteps; ++i) RC(force_pipeline(ctx, SK_VS, i == nsteps - 1));
+    {
+        Consts c = ctx->c;
+        Fields f = ctx->f;
+        Scalars *sc = ctx->dsc;
+        int kb = ctx->kblocks;
+        hipStream_t s = ctx->stream;
+        RC(launch(ctx, K_SCALE, [&] { k_apply_scale<<<kb, NT, 0, s>>>(c, f, sc); }));
+        RC(launch(ctx, K_SCALE, [&] { k_reset_scale<<<1, 1, 0, s>>>(sc); }));
+    }
+    RC(pull_scalars(ctx));
+    energies_out(ctx, e);
+    return FHIP_OK;
+}
+
+int fhip_profile(fhip_ctx *ctx, int enable)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    RC(collect_events(ctx));
+    ctx->prof = enable != 0;
+    for (int i = 0; i < K_NKERNEL; ++i) {
+        ctx->kms[i] = 0.0;
+        ctx->kcount[i] = 0;
+    }
+    return FHIP_OK;
+}
+
+int fhip_kernel_stats(fhip_ctx *ctx, int n, const char **names, double *total_ms, int64_t *launches)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    RC(collect_events(ctx));
+    for (int i = 0; i < K_NKERNEL && i < n; ++i) {
+        if (names) names[i] = kKernelNames[i];
+        if (total_ms) total_ms[i] = ctx->kms[i];
+        if (launches) launches[i] = ctx->kcount[i];
+    }
+    return K_NKERNEL;
+}
+
+int fhip_synchronize(fhip_ctx *ctx)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return collect_events(ctx);
+}
+
+}  // extern "C"
+
+// w reconstructed from the optimised w(k): acouR = c2r(-i acouK)/N
+// (coord_module.F:457-469).
+__global__ void k_prep_w(Consts c, Spectra<double> sp, const cpx<double> *acouK)
+{
+    const size_t kk = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (kk >= (size_t)c.Kh) return;
+    for (int q = 0; q < 3; ++q) {
+        const cpx<double> a = acouK[(size_t)q * c.Kh + kk];
+        sp.s[q][kk] = cpx<double>{a.y, -a.x};
+    }
+}
+
+static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
+{
+    if (ctx->c.acoustic) return fail(ctx, FHIP_ERR_ARG, "w is dynamical here: read FHIP_ACOU_R");
+    if (!ctx->acouK_valid) return fail(ctx, FHIP_ERR_STATE, "no force evaluation yet");
+    const size_t N = ctx->N;
+    double *tmp = nullptr;
+    void *raw = nullptr;
+    HIPCHK(hipMalloc(&raw, sizeof(double) * 3 * N));
+    tmp = (double *)raw;
+    Consts c = ctx->c;
+    Spectra<double> sp = ctx->sp;
+    const cpx<double> *aK = ctx->acouK;
+    hipStream_t s = ctx->stream;
+    const int kb = (ctx->Kh + NT - 1) / NT;
+    int rc = launch(ctx, K_SETUP, [&] { k_prep_w<<<kb, NT, 0, s>>>(c, sp, aK); });
+    if (rc == FHIP_OK) rc = run_ypass(ctx, +1, 0, 3);
+    if (rc == FHIP_OK) {
+        XinvArgs ia{};
+        ia.mode = XI_GENERIC;
+        ia.nf = 3;
+        ia.partials = ctx->partials;
+        ia.maxblocks = ctx->maxblocks;
+        for (int q = 0; q < 3; ++q) ia.gen[q] = tmp + q * N;
+        rc = run_xinv(ctx, ia);
+    }
+    if (rc == FHIP_OK) {
+        hipError_t e = hipMemcpyAsync(h.data(), tmp, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(ctx, FHIP_ERR_HIP, std::string("reconstruct_w: ") + hipGetErrorString(e));
+    }
+    hipFree(raw);
+    if (rc != FHIP_OK) return rc;
+    for (size_t i = 0; i < 3 * N; ++i) h[i] = h[i] * ctx->c.Ninv;
+    // the spectral slots were used as scratch: the next force evaluation refills them
+    return collect_events(ctx);
+}

@@ -1,0 +1,156 @@
+// fhip_fft.hpp -- LDS-staged Stockham FFT building blocks for gfx950.
+//
+// A workgroup holds a tile of TB lines of length N in LDS, element t of line b
+// at buf[t*TB + b] (lines interleaved, so that TB consecutive lines that are
+// consecutive in HBM load and store coalesced).  Each pass of radix R does the
+// N/R*TB butterflies of the tile: every thread loads its butterflies into
+// registers, the workgroup synchronises, and the results are written back to
+// their self-sorting (Stockham) positions.  Radices 4, 2, 3 and 5 cover the
+// supercell sizes feram runs (powers of two times 3 or 5).
+//
+// Conventions follow FFTW's: forward is e^{-i}, inverse is e^{+i}, neither is
+// normalised.  Twiddles come from a per-size table tw[j] = e^{-2 pi i j / N}
+// held in HBM (read through the L1/L2 caches).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fhip {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// (SIGN * i) * z
+template <int SIGN, typename T>
+__device__ __forceinline__ cpx<T> mul_si(cpx<T> z)
+{
+    return SIGN > 0 ? cpx<T>{-z.y, z.x} : cpx<T>{z.y, -z.x};
+}
+
+// In-register DFT of length R with kernel e^{SIGN 2 pi i pq / R}.
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft(cpx<T>* v)
+{
+    if constexpr (R == 2) {
+        cpx<T> a = v[0];
+        v[0] = cadd(a, v[1]);
+        v[1] = csub(a, v[1]);
+    } else if constexpr (R == 4) {
+        cpx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        cpx<T> s13 = cadd(v[1], v[3]), d13 = mul_si<SIGN>(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        const T h = T(0.5), s3 = T(0.86602540378443864676);
+        cpx<T> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        cpx<T> m = {v[0].x - h * t1.x, v[0].y - h * t1.y};
+        cpx<T> r = mul_si<SIGN>(cpx<T>{s3 * t2.x, s3 * t2.y});
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    } else if constexpr (R == 5) {
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);
+        cpx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+        cpx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+        cpx<T> m1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
+        cpx<T> m2 = {v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y};
+        cpx<T> r1 = mul_si<SIGN>(cpx<T>{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y});
+        cpx<T> r2 = mul_si<SIGN>(cpx<T>{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y});
+        v[0] = cadd(v[0], cadd(a1, a2));
+        v[1] = cadd(m1, r1);
+        v[4] = csub(m1, r1);
+        v[2] = cadd(m2, r2);
+        v[3] = csub(m2, r2);
+    }
+}
+
+template <int N>
+struct radix_of {
+    template <int NS>
+    static constexpr int at()
+    {
+        constexpr int rem = N / NS;
+        return (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : (rem % 5 == 0) ? 5 : 0;
+    }
+};
+
+template <typename T, int N, int TB, int NT, int R, int NS, int SIGN>
+__device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
+{
+    constexpr int M = N / R;
+    constexpr int NB = M * TB;
+    constexpr int BPT = (NB + NT - 1) / NT;
+    cpx<T> v[BPT][R];
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int bf = tid + i * NT;
+        if (NB % NT == 0 || bf < NB) {
+            const int j = bf / TB, b = bf - j * TB;
+            const int k = j % NS;
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[i][q] = buf[(j + q * M) * TB + b];
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int q = 1; q < R; ++q) {
+                    cpx<T> w = tw[k * q * (N / (NS * R))];
+                    if (SIGN > 0) w.y = -w.y;
+                    v[i][q] = cmul(v[i][q], w);
+                }
+            }
+            dft<R, SIGN>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+        const int bf = tid + i * NT;
+        if (NB % NT == 0 || bf < NB) {
+            const int j = bf / TB, b = bf - j * TB;
+            const int k = j % NS;
+            const int idxD = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[(idxD + q * NS) * TB + b] = v[i][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Full in-LDS transform of TB interleaved lines of length N.  Callers
+// synchronise before (data in LDS) and may read the result right after.
+template <typename T, int N, int TB, int NT, int SIGN, int NS = 1>
+__device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
+{
+    if constexpr (NS < N) {
+        constexpr int R = radix_of<N>::template at<NS>();
+        static_assert(R > 1, "FFT length must factor into 2, 3 and 5");
+        stockham_pass<T, N, TB, NT, R, NS, SIGN>(buf, tw, tid);
+        fft_lds<T, N, TB, NT, SIGN, NS * R>(buf, tw, tid);
+    }
+}
+
+// Largest power of two p with p*N <= cap (lines per tile).
+template <int N, int CAP = 1024>
+struct tile_of {
+    static constexpr int tb()
+    {
+        int p = 1;
+        while (2 * p * N <= CAP) p *= 2;
+        return p;
+    }
+    static constexpr int value = tb();
+};
+
+}  // namespace fhip

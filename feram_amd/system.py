@@ -1,0 +1,218 @@
+"""Host-side mirror of the reference's per-step interface on the HIP library.
+
+``System`` plays the role of feram's ``Coord_type`` plus its step routines:
+its methods are the subroutines the reference's drivers call
+(``mirror_force_mirror``, ``nose_poincare``, ``leapfrog``,
+``velocity_scaling``; see include/feram_hip.h for the file:line each one
+replaces) and return the ``Energy_type`` values as a dict.  State lives on the
+GPU between calls; ``get``/``put`` move fields in the reference's component
+order as numpy arrays of shape (Lz, Ly, Lx, 3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .params import GEOMETRIES, METHODS, Param
+
+
+class FhipError(RuntimeError):
+    pass
+
+
+def params_struct(p: Param, precision: int = 0) -> _lib.FhipParams:
+    s = _lib.FhipParams()
+    s.Lx, s.Ly, s.Lz, s.padding_y = p.Lx, p.Ly, p.Lz, p.padding_y
+    s.method = METHODS[p.method]
+    s.bulk_or_film = GEOMETRIES[p.bulk_or_film]
+    s.precision = precision
+    s.E_field_always_zero = 1 if p.E_field_always_zero else 0
+    for name in ("dt", "kelvin", "Q_Nose", "a0", "epi_strain", "B11", "B12", "B44", "B1xx",
+                 "B1yy", "B4yz", "modulation_constant", "epsilon_inf", "kappa"):
+        setattr(s, name, float(getattr(p, name)))
+    for name in ("GPa", "mass_dipo", "mass_acou", "P_kappa2", "P_alpha", "P_gamma", "P_k1",
+                 "P_k2", "P_k3", "P_k4", "j", "Z_star", "external_E_field"):
+        arr = getattr(s, name)
+        for i, v in enumerate(getattr(p, name)):
+            arr[i] = float(v)
+    return s
+
+
+class System:
+    """One device context (``fhip_ctx``) for a supercell."""
+
+    def __init__(self, p: Param, *, precision: int = 0):
+        self.p = p
+        self._L = _lib.load()
+        self._s = params_struct(p, precision)
+        ctx = C.c_void_p()
+        rc = self._L.fhip_create(C.byref(self._s), C.byref(ctx))
+        self.ctx = ctx
+        if rc != 0:
+            msg = self._L.fhip_last_error(ctx).decode() if ctx.value else ""
+            self.close()
+            raise FhipError(f"fhip_create: {_lib.ERRORS.get(rc, rc)}: {msg}")
+        self.N = p.Lx * p.Ly * p.Lz
+
+    # ---- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self._L.fhip_destroy(self.ctx)
+        self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise FhipError(f"{what}: {_lib.ERRORS.get(rc, rc)}: {self._L.fhip_last_error(self.ctx).decode()}")
+        return rc
+
+    # ---- fields ------------------------------------------------------------
+    def put(self, name: str, arr) -> None:
+        p = self.p
+        a = np.asarray(arr, dtype=np.float64).reshape(p.Lz, p.Ly, p.Lx, 3)
+        soa = np.ascontiguousarray(np.moveaxis(a, -1, 0))
+        self._check(self._L.fhip_put_soa(self.ctx, _lib.FIELDS[name], soa.ctypes.data), f"put {name}")
+
+    def get(self, name: str) -> np.ndarray:
+        p = self.p
+        soa = np.empty((3, p.Lz, p.Ly, p.Lx))
+        self._check(self._L.fhip_get_soa(self.ctx, _lib.FIELDS[name], soa.ctypes.data), f"get {name}")
+        return np.ascontiguousarray(np.moveaxis(soa, 0, -1))
+
+    def put_padded(self, name: str, host: np.ndarray) -> None:
+        """Host array in the reference's padded layout (coord_module.F:134-141)."""
+        h = np.ascontiguousarray(host, dtype=np.float64)
+        self._check(self._L.fhip_put(self.ctx, _lib.FIELDS[name], h.ctypes.data), f"put {name}")
+
+    def get_padded(self, name: str, host: np.ndarray) -> np.ndarray:
+        assert host.flags.c_contiguous and host.dtype == np.float64
+        self._check(self._L.fhip_get(self.ctx, _lib.FIELDS[name], host.ctypes.data), f"get {name}")
+        return host
+
+    def padded_shape(self, ncomp: int = 3):
+        """Shape of a padded host field in C order (z, y+pad, 2(Lx/2+1), ncomp)."""
+        p = self.p
+        return (p.Lz, p.Ly + p.padding_y, 2 * (p.Lx // 2 + 1), ncomp)
+
+    def set_long_range(self, table: np.ndarray) -> None:
+        """c%long_range(3,3,0:Lx/2,0:Ly+pad-1,0:Lz-1) given in C order (z, y+pad, kx, b, a)."""
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        self._check(self._L.fhip_set_long_range(self.ctx, t.ctypes.data), "set_long_range")
+
+    def get_long_range(self) -> np.ndarray:
+        """The device table as c%long_range in C order (z, y+pad, kx, b, a)."""
+        p = self.p
+        t = np.zeros((p.Lz, p.Ly + p.padding_y, p.Lx // 2 + 1, 3, 3))
+        self._check(self._L.fhip_get_long_range(self.ctx, t.ctypes.data), "get_long_range")
+        return t
+
+    def set_modulation(self, modulation_R: np.ndarray) -> None:
+        """modulation_R of shape (Lz, Ly, Lx) (elastic.F:187-200)."""
+        p = self.p
+        h = np.zeros((p.Lz, p.Ly + p.padding_y, 2 * (p.Lx // 2 + 1)))
+        h[:, :p.Ly, :p.Lx] = modulation_R
+        self._check(self._L.fhip_set_modulation(self.ctx, h.ctypes.data), "set_modulation")
+
+    def set_cells(self, relaxor=None, sum_p=None, mass_dipo_inv=None) -> None:
+        """Per-cell species (1..3), sum_p mask and 1/m, each of shape (Lz, Ly, Lx)."""
+        p = self.p
+        shp = (p.Lz, p.Ly + p.padding_y, 2 * (p.Lx // 2 + 1))
+        keep = []
+
+        def pad(a, dtype):
+            if a is None:
+                return None
+            h = np.zeros(shp, dtype=dtype)
+            h[:, :p.Ly, :p.Lx] = a
+            keep.append(h)
+            return h.ctypes.data
+
+        rc = self._L.fhip_set_cells(self.ctx, pad(relaxor, np.int32), pad(sum_p, np.int32),
+                                    pad(mass_dipo_inv, np.float64))
+        self._check(rc, "set_cells")
+
+    # ---- scalars -----------------------------------------------------------
+    @property
+    def nose(self):
+        s, pi, h0 = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._L.fhip_get_nose(self.ctx, C.byref(s), C.byref(pi), C.byref(h0)), "get_nose")
+        return s.value, pi.value, h0.value
+
+    def set_nose(self, s, pi, H0):
+        self._check(self._L.fhip_set_nose(self.ctx, float(s), float(pi), float(H0)), "set_nose")
+
+    @property
+    def strain(self) -> np.ndarray:
+        out = (C.c_double * 6)()
+        self._check(self._L.fhip_get_strain(self.ctx, out), "get_strain")
+        return np.array(out[:])
+
+    def set_E_field_add(self, E) -> None:
+        e = (C.c_double * 3)(*[float(v) for v in E])
+        self._check(self._L.fhip_set_E_field_add(self.ctx, e), "set_E_field_add")
+
+    # ---- the step ------------------------------------------------------------
+    def mirror_force_mirror(self) -> dict:
+        e = _lib.FhipEnergy()
+        self._check(self._L.fhip_force(self.ctx, C.byref(e)), "force")
+        return e.as_dict()
+
+    def md_prologue(self) -> dict:
+        e = _lib.FhipEnergy()
+        self._check(self._L.fhip_md_prologue(self.ctx, C.byref(e)), "md_prologue")
+        return e.as_dict()
+
+    def nose_poincare(self, nsteps: int = 1) -> dict:
+        e = _lib.FhipEnergy()
+        self._check(self._L.fhip_nose_poincare(self.ctx, int(nsteps), C.byref(e)), "nose_poincare")
+        return e.as_dict()
+
+    def leapfrog(self, i_step: int = 1, nsteps: int = 1) -> dict:
+        e = _lib.FhipEnergy()
+        self._check(self._L.fhip_leapfrog(self.ctx, int(i_step), int(nsteps), C.byref(e)), "leapfrog")
+        return e.as_dict()
+
+    def velocity_scaling(self, nsteps: int = 1) -> dict:
+        e = _lib.FhipEnergy()
+        self._check(self._L.fhip_velocity_scaling(self.ctx, int(nsteps), C.byref(e)), "velocity_scaling")
+        return e.as_dict()
+
+    def step(self, nsteps: int = 1) -> dict:
+        m = self.p.method
+        if m == "md":
+            return self.nose_poincare(nsteps)
+        if m == "lf":
+            return self.leapfrog(1, nsteps)
+        return self.velocity_scaling(nsteps)
+
+    # ---- measurement ---------------------------------------------------------
+    def profile(self, enable: bool = True) -> None:
+        self._check(self._L.fhip_profile(self.ctx, 1 if enable else 0), "profile")
+
+    def kernel_stats(self) -> dict:
+        n = 32
+        names = (C.c_char_p * n)()
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        k = self._check(self._L.fhip_kernel_stats(self.ctx, n, names, ms, cnt), "kernel_stats")
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k) if cnt[i] > 0}
+
+    def synchronize(self) -> None:
+        self._check(self._L.fhip_synchronize(self.ctx), "synchronize")
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self._L.fhip_device_bytes(self.ctx))

@@ -1,0 +1,150 @@
+/*
+ * feram_hip.h -- C ABI of the MI355X (gfx950) per-step force-and-integrate
+ * path of feram.
+ *
+ * The entry points replace, at link time, the Fortran subroutines that the
+ * reference's time-loop drivers call once per MD step.  Paths below are
+ * relative to the reference's src/ directory.
+ *
+ *   fhip_force            <- mirror_force_mirror(p,c,e)   mirror_force_mirror.F:6-89
+ *                            (also called from initialize_dipoR, initialize-dipoR.F:62)
+ *   fhip_nose_poincare    <- nose_poincare(p,c,e)         nose-poincare.F:10-78,
+ *                            called at molecular-dynamics.F:113-116
+ *   fhip_leapfrog         <- leapfrog(p,c,e,i_step)       leapfrog.F:9-157,
+ *                            called at molecular-dynamics.F:119-120
+ *   fhip_velocity_scaling <- velocity_scaling(p,c,e)      velocity_scaling.F:9-103,
+ *                            called at molecular-dynamics.F:117-118, hysteresis_loop.F:69
+ *   fhip_md_prologue      <- initialize-dipoR.F:59-69 followed by the H0 set-up of
+ *                            molecular-dynamics.F:44-55
+ *
+ * Host arrays keep the reference's padded layout (coord_module.F:91-93,134-141):
+ * 3-component fields are (3, 0:2(Lx/2+1)-1, 0:Ly+padding_y-1, 0:Lz-1) real*8
+ * with the component fastest; per-cell scalars drop the leading 3; the
+ * long-range table is long_range(3,3, 0:Lx/2, 0:Ly+padding_y-1, 0:Lz-1).
+ * Padding entries are ignored on input and left untouched on output.
+ *
+ * Every function returns FHIP_OK (0) or a negative FHIP_ERR_* code;
+ * fhip_last_error() gives the message.  One context per process and device;
+ * calls on one context must not overlap (the reference calls from the master
+ * thread, outside OpenMP regions).
+ */
+#ifndef FERAM_HIP_H
+#define FERAM_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FHIP_OK = 0,
+    FHIP_ERR_ARG = -1,          /* bad argument or size */
+    FHIP_ERR_UNSUPPORTED = -2,  /* configuration outside what this build runs */
+    FHIP_ERR_HIP = -3,          /* HIP runtime error (message has the detail) */
+    FHIP_ERR_ALLOC = -4,        /* device memory allocation failed */
+    FHIP_ERR_STATE = -5         /* call out of order */
+};
+
+/* Methods and geometries (param_module.F:36, :43). */
+enum { FHIP_MD = 0, FHIP_VS = 1, FHIP_LF = 2 };
+enum { FHIP_BULK = 0, FHIP_FILM = 1, FHIP_EPIT = 2, FHIP_E110 = 3 };
+/* Arithmetic of spectra and local forces; thermostat and energy sums are FP64 always. */
+enum { FHIP_FP64 = 0, FHIP_FP32 = 1 };
+
+/* The parameters of Param_type (param_module.F:8-111) the step reads. Masses
+ * are in eV A^-2 ps^2 (amu / 9648.53377163899); mass_acou <= 0 means the
+ * acoustic displacement is optimised each step (param_module.F:135). */
+typedef struct fhip_params {
+    int32_t Lx, Ly, Lz, padding_y;
+    int32_t method, bulk_or_film, precision, E_field_always_zero;
+    double dt, kelvin, Q_Nose, a0, epi_strain;
+    double GPa[3];
+    double mass_dipo[3], mass_acou[3];
+    double P_kappa2[3], P_alpha[3], P_gamma[3], P_k1[3], P_k2[3], P_k3[3], P_k4[3];
+    double j[7];
+    double B11, B12, B44, B1xx, B1yy, B4yz;
+    double modulation_constant;
+    double Z_star[3];
+    double epsilon_inf;
+    double kappa;                 /* <= 0: sqrt(pi)/a0/N^(1/6) (param_module.F:462-464) */
+    double external_E_field[3];
+} fhip_params;
+
+/* Energy_type (energy_module.F:7-23), same member order, supercell totals in eV. */
+typedef struct fhip_energy {
+    double acou_kinetic, dipo_kinetic, short_range, long_range, dipole_E_field,
+           unharmonic, homo_strain, homo_coupling, inho_strain, inho_coupling,
+           inho_modulation, total, H_Nose_Poincare;
+} fhip_energy;
+
+/* 3-component fields of Coord_type (coord_module.F:14-64). */
+enum {
+    FHIP_DIPO_R = 0,   /* c%dipoR   u                                   */
+    FHIP_DIPO_P = 1,   /* c%dipoP   p                                   */
+    FHIP_DVDDI  = 2,   /* c%dVddi   dV/du (read only)                   */
+    FHIP_ACOU_R = 3,   /* c%acouR   w                                   */
+    FHIP_ACOU_P = 4,   /* c%acouP   p_w                                 */
+    FHIP_DVDAC  = 5,   /* dV/dw in the dVdac_ptr buffer (read only)      */
+    FHIP_E_LOCAL = 6,  /* c%E_field_local; the uniform part is set with fhip_set_E_field_add */
+    FHIP_ACOU_R_OPT = 7 /* w reconstructed from the optimised w(k) as write_system does
+                           (coord_module.F:457-469) (read only)           */
+};
+
+typedef struct fhip_ctx fhip_ctx;
+
+/* alloc_Coord + dipole_dipole + elastic (coord_module.F:67-432,
+ * dipole-dipole.F:6-73, elastic.F:7-237): device buffers, FFT twiddles, the
+ * Ewald long-range table (built on the device) and the elastic constants.
+ * State starts at u = p = w = p_w = 0, s = 1, pi = 0, H0 = 0. */
+int fhip_create(const fhip_params *p, fhip_ctx **ctx);
+void fhip_destroy(fhip_ctx *ctx);
+const char *fhip_last_error(const fhip_ctx *ctx);
+
+/* Replace the long-range table with the host's c%long_range (already scaled
+ * by 14.399644/epsilon_inf, dipole-dipole-long-range.F:146-150). */
+int fhip_set_long_range(fhip_ctx *ctx, const double *long_range);
+/* Copy the device table out as c%long_range (symmetric: both triangles are
+ * written), e.g. for the dispersion output of dipole-dipole.F:24-70. */
+int fhip_get_long_range(fhip_ctx *ctx, double *long_range);
+/* c%modulation_R (elastic.F:187-200); requires modulation_constant != 0. */
+int fhip_set_modulation(fhip_ctx *ctx, const double *modulation_R);
+/* Per-cell data set by read_relaxor / read_defects_u (read_relaxor.F:24-33,
+ * read_defects_u.F:34-52): relaxor species 1..3 (NULL: all 1), sum_p as 0/1
+ * (NULL: all counted), mass_dipo_inv (NULL: 1/mass_dipo(species)); a cell's
+ * mass_dipo_inv must be 1/mass_dipo(species) or 0 (frozen defect). */
+int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p,
+                   const double *mass_dipo_inv);
+
+int fhip_put(fhip_ctx *ctx, int field, const double *host);
+int fhip_get(fhip_ctx *ctx, int field, double *host);
+/* Unpadded structure-of-arrays variants: host[a*N + x + Lx*(y + Ly*z)]. */
+int fhip_put_soa(fhip_ctx *ctx, int field, const double *host);
+int fhip_get_soa(fhip_ctx *ctx, int field, double *host);
+
+/* Thermostat state c%s_Nose, c%pi_Nose, c%H0 (coord_module.F:53). */
+int fhip_get_nose(fhip_ctx *ctx, double *s, double *pi, double *H0);
+int fhip_set_nose(fhip_ctx *ctx, double s, double pi, double H0);
+/* c%strain(6) of the last force evaluation. */
+int fhip_get_strain(fhip_ctx *ctx, double strain[6]);
+/* Uniform part of E(R) = E_local(R) + E_add (molecular-dynamics.F:66-76). */
+int fhip_set_E_field_add(fhip_ctx *ctx, const double E_add[3]);
+
+int fhip_force(fhip_ctx *ctx, fhip_energy *e);
+int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e);
+/* nsteps consecutive steps (>= 1); e receives the energies after the last one. */
+int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e);
+int fhip_leapfrog(fhip_ctx *ctx, int i_step, int nsteps, fhip_energy *e);
+int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e);
+
+/* Measurement: when enabled, HIP events bracket every kernel launch on the
+ * context's stream and accumulate per-kernel time.  fhip_kernel_stats fills
+ * up to n entries (name, total ms, launches) and returns the entry count. */
+int fhip_profile(fhip_ctx *ctx, int enable);
+int fhip_kernel_stats(fhip_ctx *ctx, int n, const char **names, double *total_ms, int64_t *launches);
+int fhip_synchronize(fhip_ctx *ctx);
+/* Bytes of device memory the context holds. */
+int64_t fhip_device_bytes(const fhip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""HIP path against the FP64 oracle on identical inputs (needs an MI355X).
+
+Tolerances, FP64 plain mode:
+  * forces dV/du and dV/dw: normwise  max|dF| / max|F|          <= 1e-12
+  * each energy term:       |dE| <= 1e-12 * sum of |terms|  (+ 1e-12 relative)
+  * tables L(k):            max|dL| / max|L|                   <= 1e-12
+  * trajectories (20 steps): u, p normwise                     <= 1e-10
+The FFTs, the order of the Parseval and real-space sums and FMA contraction
+differ between the two sides, so agreement is to rounding, not bitwise.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, param_from_dict
+from feram_amd.params import batio3_zvr
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-12
+ENERGY_TOL = 1e-12
+TRAJ_TOL = 1e-10
+
+ENERGY_TERMS = ("short_range", "long_range", "dipole_E_field", "unharmonic", "homo_strain",
+                "homo_coupling", "inho_strain", "inho_coupling", "inho_modulation")
+
+
+def _system(p):
+    from feram_amd.system import System
+    print(f"[gpu] System L=({p.Lx},{p.Ly},{p.Lz}) method={p.method}", flush=True)
+    return System(p)
+
+
+def _pair(oracle_mod, p, modulation=None):
+    o = oracle_mod.Oracle(p, modulation_R=modulation)
+    g = _system(p)
+    if modulation is not None:
+        g.set_modulation(modulation)
+    return o, g
+
+
+def _random(p, seed, amp=0.06, mean=0.08):
+    rng = np.random.default_rng(seed)
+    return mean + amp * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+
+
+def _normwise(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _check_energies(eg, eo, tol=ENERGY_TOL):
+    scale = sum(abs(eo[t]) for t in ENERGY_TERMS)
+    for t in ENERGY_TERMS:
+        assert abs(eg[t] - eo[t]) <= tol * scale + tol * abs(eo[t]), (t, eg[t], eo[t])
+
+
+@pytest.mark.parametrize("L", [8, 12, 16, (8, 12, 16), (16, 6, 24)])
+def test_long_range_table(oracle_mod, L):
+    p = batio3_zvr(L)
+    o, g = _pair(oracle_mod, p)
+    lo = o.field("long_range")                          # (z, y, kx, b, a)
+    lg = g.get_long_range()[:, :p.Ly]
+    assert _normwise(lg, lo) <= FORCE_TOL
+
+
+@pytest.mark.parametrize("L", [8, 12, 16, (8, 12, 16), (16, 6, 24), 24])
+def test_force_and_energies(oracle_mod, L):
+    p = batio3_zvr(L)
+    o, g = _pair(oracle_mod, p)
+    u = _random(p, 7)
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FORCE_TOL
+    _check_energies(eg, eo)
+    assert np.allclose(g.strain, o.strain, rtol=1e-12, atol=1e-16)
+
+
+def test_force_with_fields_cells_and_pressure(oracle_mod):
+    p = batio3_zvr((12, 8, 16))
+    p.GPa = [-3.0, 1.0, 2.0]
+    p.external_E_field = [0.001, -0.002, 0.0005]
+    p.E_field_always_zero = False
+    p.P_kappa2 = [5.502, 4.9, 6.1]
+    p.P_alpha = [110.4, 100.0, 120.0]
+    p.P_k1 = [0.0, 3.0, -2.0]
+    p.P_k2 = [0.0, 1.0, 2.0]
+    p.P_k3 = [0.0, -1.0, 4.0]
+    p.P_k4 = [0.0, 5.0, 7.0]
+    p.mass_dipo = [p.mass_dipo[0], 1.2 * p.mass_dipo[0], 0.8 * p.mass_dipo[0]]
+    o, g = _pair(oracle_mod, p)
+    rng = np.random.default_rng(3)
+    relax = rng.integers(1, 4, size=(p.Lz, p.Ly, p.Lx)).astype(np.int32)
+    o.relaxor[...] = relax
+    for s in range(3):
+        o.field("mass_dipo_inv")[relax == s + 1] = 1.0 / p.mass_dipo[s]
+    g.set_cells(relaxor=relax)
+    eloc = 0.002 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+    o.field("E_field")[...] = eloc + np.array(p.external_E_field)
+    g.put("E_field_local", eloc)
+    u = _random(p, 8)
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FORCE_TOL
+    _check_energies(eg, eo)
+
+
+def test_acoustic_force(oracle_mod):
+    p = batio3_zvr((8, 12, 8), method="lf", acoustic_mass_amu=46.44)
+    o, g = _pair(oracle_mod, p)
+    u = _random(p, 9)
+    w = 0.02 * np.random.default_rng(10).standard_normal(u.shape)
+    for name, v in (("dipoR", u), ("acouR", w)):
+        o.field(name)[...] = v
+        g.put(name, v)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FORCE_TOL
+    assert _normwise(g.get("dVdacR"), o.field("dVdacR")) <= FORCE_TOL
+    _check_energies(eg, eo)
+
+
+@pytest.mark.parametrize("method", ["md", "lf", "vs"])
+def test_trajectory(oracle_mod, method):
+    p = batio3_zvr(12, method=method, kelvin=300.0)
+    o, g = _pair(oracle_mod, p)
+    u = _random(p, 21)
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    eg = g.md_prologue()
+    _check_energies(eg, e_o.as_dict())
+    if method == "md":
+        assert g.nose == pytest.approx(tuple(o.nose), rel=1e-12, abs=1e-12)
+    for chunk in (1, 4, 15):
+        for i in range(chunk):
+            o.step(e_o)
+        eg = g.step(chunk)
+        eo = e_o.as_dict()
+        assert _normwise(g.get("dipoR"), o.field("dipoR")) <= TRAJ_TOL
+        assert _normwise(g.get("dipoP"), o.field("dipoP")) <= TRAJ_TOL
+        assert _normwise(g.get("dVddi"), o.field("dVddi")) <= TRAJ_TOL
+        _check_energies(eg, eo, tol=TRAJ_TOL)
+        assert eg["dipo_kinetic"] == pytest.approx(eo["dipo_kinetic"], rel=TRAJ_TOL)
+        if method == "md":
+            s_g, pi_g, _ = g.nose
+            assert s_g == pytest.approx(o.nose[0], rel=TRAJ_TOL)
+            assert pi_g == pytest.approx(o.nose[1], rel=1e-8, abs=1e-10)
+            assert eg["H_Nose_Poincare"] == pytest.approx(eo["H_Nose_Poincare"], abs=1e-8 * p.N)
+
+
+@pytest.mark.parametrize("method", ["lf", "vs"])
+def test_acoustic_trajectory(oracle_mod, method):
+    p = batio3_zvr((8, 8, 12), method=method, kelvin=250.0, acoustic_mass_amu=46.44)
+    o, g = _pair(oracle_mod, p)
+    u = _random(p, 31)
+    w = 0.02 * np.random.default_rng(32).standard_normal(u.shape)
+    for name, v in (("dipoR", u), ("acouR", w)):
+        o.field(name)[...] = v
+        g.put(name, v)
+    e_o = o.md_prologue()
+    g.md_prologue()
+    for chunk in (1, 6):
+        for _ in range(chunk):
+            o.step(e_o)
+        eg = g.step(chunk)
+        for name in ("dipoR", "dipoP", "acouR", "acouP"):
+            assert _normwise(g.get(name), o.field(name)) <= TRAJ_TOL, name
+        assert eg["acou_kinetic"] == pytest.approx(e_o.acou_kinetic, rel=TRAJ_TOL)
+
+
+def test_modulation_golden():
+    g_ = load_golden()["modulation_check"]
+    p = param_from_dict(g_["params"])
+    mod = np.zeros((p.Lz, p.Ly, p.Lx))
+    for x, y, z, a in g_["modulation"]:
+        mod[z, y, x] = a
+    g = _system(p)
+    g.set_modulation(mod)
+    ex = g_["expect"]
+    e = g.md_prologue()
+    assert "%.12f" % (e["inho_strain"] / p.N) == ex["inho_strain_per_uc_f18_12"]
+    assert "%.12f" % (e["inho_modulation"] / p.N) == ex["inho_modulation_per_uc_f18_12"]
+    e = g.leapfrog(1, 1)
+    w = g.get("acouR_optimised").reshape(-1, 3)
+    from test_oracle_goldens import f_e
+    pos = sum(any(f_e(v, 6, 2).rjust(14) == ex["acouR_e14_6_positive"] for v in c) for c in w)
+    neg = sum(any(f_e(v, 6, 2).rjust(14) == ex["acouR_e14_6_negative"] for v in c) for c in w)
+    assert (pos, neg) == (ex["acouR_positive_lines"], ex["acouR_negative_lines"])
+    assert "%.12f" % (e["inho_strain"] / p.N) == ex["inho_strain_per_uc_f18_12"]
+
+
+def test_padded_layout_roundtrip():
+    for pad in (1, 3):
+        p = batio3_zvr((8, 12, 6))
+        p.padding_y = pad
+        g = _system(p)
+        shp = g.padded_shape()
+        host = np.random.default_rng(pad).standard_normal(shp)
+        g.put_padded("dipoR", host)
+        back = np.full(shp, 7.0)
+        g.get_padded("dipoR", back)
+        real = (slice(None), slice(0, p.Ly), slice(0, p.Lx))
+        assert np.array_equal(back[real], host[real])
+        mask = np.ones(shp, bool)
+        mask[real] = False
+        assert np.all(back[mask] == 7.0)          # padding untouched
+        u = g.get("dipoR")
+        assert np.array_equal(u, host[real])
+
+
+def test_bitwise_repeatable():
+    p = batio3_zvr(16)
+    runs = []
+    for _ in range(2):
+        g = _system(p)
+        g.put("dipoR", _random(p, 41))
+        g.md_prologue()
+        e = g.nose_poincare(5)
+        runs.append((g.get("dipoR"), g.get("dipoP"), e["total"], g.nose))
+        g.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2] and runs[0][3] == runs[1][3]
+
+
+def test_rejects_unsupported():
+    from feram_amd.system import FhipError
+    p = batio3_zvr((10, 8, 8))
+    with pytest.raises(FhipError, match="supercell edges"):
+        _system(p)
+    p = batio3_zvr(8)
+    p.bulk_or_film = "film"
+    with pytest.raises(FhipError, match="film"):
+        _system(p)
