@@ -77,7 +77,21 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 launches) to calibrate "
+                         "the FETCH_SIZE / WRITE_SIZE counters, then exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch_read": n * 8, "copy_bytes_per_launch_write": n * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
               "(replica ensembles shard with no exchange, see DESIGN.md)", file=sys.stderr)
@@ -88,6 +102,10 @@ def main():
 
     L = args.size
     p = batio3_zvr(L, method="md", kelvin=300.0)
+    # eq. 38's root needs CB dt/Q < 1 with CB ~ 3N kT dt/2, so Q_Nose grows with N:
+    # the reference's 64^3 md benchmark uses Q_Nose = 1.0
+    # (18example-benchmark/md64x64x64.feram:6); scale it with N.
+    p.Q_Nose = args.q_nose if args.q_nose > 0 else 1.0 * p.N / 64 ** 3
     rng = np.random.default_rng(20241203)
     u0 = rng.uniform(-0.05, 0.05, size=(L, L, L, 3))
 
@@ -116,6 +134,9 @@ def main():
     dt = time.perf_counter() - t0
     steps_per_s = args.steps / dt
     print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+    if not all(np.isfinite(v) for v in e.values()):
+        print(f"[bench] non-finite energies after the timed steps: {e}", file=sys.stderr)
+        sys.exit(3)
 
     # per-kernel timing pass (events around every launch)
     g.profile(True)
@@ -162,7 +183,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"BaTiO3 bulk {L}x{L}x{L} NP-MD ('md'), 300 K, dt=2 fs, optimised "
                                "acoustic strain, FP64 plain, u0 ~ U[-0.05,0.05] A seeded",
-                   "L": L, "cells": N, "method": "md", "precision": "fp64",
+                   "L": L, "cells": N, "method": "md", "precision": "fp64", "Q_Nose": p.Q_Nose,
                    "cell_updates_per_s": steps_per_s * N,
                    "total_energy_eV_per_uc": e["total"] / N},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

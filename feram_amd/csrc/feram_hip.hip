@@ -484,6 +484,7 @@ int forward_generic(fhip_ctx *ctx, const double *const *fields, int nf)
     RC(run_ypass(ctx, -1, 0, nf));
     ZArgsD za = zargs(ctx, false);
     za.nplain = nf;
+    za.plain_sign = -1;
     return run_zpass_g<ZG_PLAIN>(ctx, za);
 }
 
@@ -1015,6 +1016,12 @@ static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
     hipStream_t s = ctx->stream;
     const int kb = (ctx->Kh + NT - 1) / NT;
     int rc = launch(ctx, K_SETUP, [&] { k_prep_w<<<kb, NT, 0, s>>>(c, sp, aK); });
+    if (rc == FHIP_OK) {
+        ZArgsD za = zargs(ctx, false);
+        za.nplain = 3;
+        za.plain_sign = +1;
+        rc = run_zpass_g<ZG_PLAIN>(ctx, za);
+    }
     if (rc == FHIP_OK) rc = run_ypass(ctx, +1, 0, 3);
     if (rc == FHIP_OK) {
         XinvArgs ia{};

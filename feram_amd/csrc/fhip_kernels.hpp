@@ -272,6 +272,7 @@ struct ZArgsD {
     double* partials;                // [P_NSLOT][maxblocks]
     int maxblocks;
     int nplain;                      // ZG_PLAIN: number of slots
+    int plain_sign;                  // ZG_PLAIN: -1 forward, +1 inverse
 };
 
 // k vector of the half-spectrum point (elastic.F:83-89)
@@ -322,7 +323,8 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
                 if (l0 + b < nlines) buf[e] = s[(size_t)t * plane + l0 + b];
             }
             __syncthreads();
-            fft_lds<T, LZ, TB, NT, -1>(buf, twz, tid);
+            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1>(buf, twz, tid);
+            else fft_lds<T, LZ, TB, NT, +1>(buf, twz, tid);
             for (int e = tid; e < LZ * TB; e += NT) {
                 const int t = e / TB, b = e - t * TB;
                 if (l0 + b < nlines) s[(size_t)t * plane + l0 + b] = buf[e];

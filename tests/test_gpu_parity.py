@@ -235,3 +235,80 @@ def test_rejects_unsupported():
     p.bulk_or_film = "film"
     with pytest.raises(FhipError, match="film"):
         _system(p)
+
+
+@pytest.mark.parametrize("L", [32, 48])
+def test_force_larger_sizes(oracle_mod, L):
+    p = batio3_zvr(L)
+    o, g = _pair(oracle_mod, p)
+    u = _random(p, 50 + L)
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FORCE_TOL
+    _check_energies(eg, eo)
+
+
+def test_np_64_cubed(oracle_mod):
+    p = batio3_zvr(64, kelvin=300.0)
+    p.Q_Nose = 1.0                      # 18example-benchmark/md64x64x64.feram:6
+    o, g = _pair(oracle_mod, p)
+    u = np.random.default_rng(64).uniform(-0.05, 0.05, size=(64, 64, 64, 3))
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    g.md_prologue()
+    for _ in range(4):
+        o.nose_poincare(e_o)
+    eg = g.nose_poincare(4)
+    assert _normwise(g.get("dipoR"), o.field("dipoR")) <= TRAJ_TOL
+    assert _normwise(g.get("dipoP"), o.field("dipoP")) <= TRAJ_TOL
+    _check_energies(eg, e_o.as_dict(), tol=TRAJ_TOL)
+    assert g.nose[0] == pytest.approx(o.nose[0], rel=TRAJ_TOL)
+
+
+def test_acoustic_vs_48_cubed(oracle_mod):
+    p = batio3_zvr(48, method="vs", kelvin=300.0, acoustic_mass_amu=46.44)
+    o, g = _pair(oracle_mod, p)
+    u = np.random.default_rng(48).uniform(-0.05, 0.05, size=(48, 48, 48, 3))
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    g.md_prologue()
+    for _ in range(3):
+        o.velocity_scaling(e_o)
+    eg = g.velocity_scaling(3)
+    for name in ("dipoR", "dipoP", "acouR", "acouP"):
+        assert _normwise(g.get(name), o.field(name)) <= TRAJ_TOL, name
+    _check_energies(eg, e_o.as_dict(), tol=TRAJ_TOL)
+
+
+def test_128_cubed_force_is_gradient():
+    """Size-independent check at the benchmark size: central differences of the
+    reported potential energy reproduce dV/du (no oracle at this size)."""
+    p = batio3_zvr(128)
+    g = _system(p)
+    u = np.random.default_rng(128).uniform(-0.05, 0.05, size=(128, 128, 128, 3)) + 0.05
+    g.put("dipoR", u)
+    g.mirror_force_mirror()
+    F = g.get("dVddi")
+    assert np.all(np.isfinite(F))
+
+    def V():
+        e = g.mirror_force_mirror()
+        return sum(e[t] for t in ENERGY_TERMS)
+
+    h = 1e-4
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        idx = (rng.integers(128), rng.integers(128), rng.integers(128), rng.integers(3))
+        up, um = u.copy(), u.copy()
+        up[idx] += h
+        um[idx] -= h
+        g.put("dipoR", up)
+        vp = V()
+        g.put("dipoR", um)
+        vm = V()
+        fd = (vp - vm) / (2 * h)
+        assert fd == pytest.approx(F[idx], rel=1e-4, abs=1e-6 * np.max(np.abs(F)))
