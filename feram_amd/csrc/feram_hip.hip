@@ -53,7 +53,8 @@ const char *kKernelNames[K_NKERNEL] = {
 
 struct fhip_ctx {
     fhip_params p{};
-    int Lx = 0, Ly = 0, Lz = 0, Kx = 0, N = 0, Kh = 0, nslots = 0;
+    int Lx = 0, Ly = 0, Lz = 0, Kx = 0, Mx = 0, N = 0, Kh = 0, nslots = 0;
+    size_t Mh = 0;
     int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0, maxblocks = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -188,7 +189,7 @@ int run_xfwd(fhip_ctx *ctx, const XfwdArgs &a)
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx not supported");
 }
 
-int run_xinv(fhip_ctx *ctx, const XinvArgs &a)
+int run_xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
 {
     Consts c = ctx->c;
     Fields f = ctx->f;
@@ -200,7 +201,11 @@ int run_xinv(fhip_ctx *ctx, const XinvArgs &a)
     switch (ctx->Lx) {
 #define FHIP_CASE(n)                                                                         \
     case n:                                                                                  \
-        return launch(ctx, K_XINV, [&] { k_xinv<double, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+        if (nf == 12)                                                                        \
+            return launch(ctx, K_XINV, [&] { k_xinv<double, n, 12><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
+        if (nf == 9)                                                                         \
+            return launch(ctx, K_XINV, [&] { k_xinv<double, n, 9><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
+        return launch(ctx, K_XINV, [&] { k_xinv<double, n, 3><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -228,7 +233,7 @@ int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf)
 }
 
 template <int G>
-int run_zpass_g(fhip_ctx *ctx, const ZArgsD &a)
+int run_zpass_g(fhip_ctx *ctx, const ZArgs<double> &a)
 {
     Consts c = ctx->c;
     Spectra<double> sp = ctx->sp;
@@ -247,9 +252,9 @@ int run_zpass_g(fhip_ctx *ctx, const ZArgsD &a)
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lz not supported");
 }
 
-ZArgsD zargs(fhip_ctx *ctx, bool want_acouK)
+ZArgs<double> zargs(fhip_ctx *ctx, bool want_acouK)
 {
-    ZArgsD a{};
+    ZArgs<double> a{};
     a.Ltab = ctx->Ltab;
     a.csx = ctx->csx;
     a.csy = ctx->csy;
@@ -267,7 +272,7 @@ int nforward(const fhip_ctx *ctx) { return ctx->c.acoustic ? 12 : 9; }
 
 enum StepKind { SK_FORCE = 0, SK_NP = 1, SK_LF = 2, SK_VS = 3 };
 
-int force_pipeline(fhip_ctx *ctx, int kind, bool materialize)
+int force_pipeline(fhip_ctx *ctx, int kind, bool materialize, int next_step = 0)
 {
     const int nf = nforward(ctx);
     XfwdArgs xa{};
@@ -275,7 +280,7 @@ int force_pipeline(fhip_ctx *ctx, int kind, bool materialize)
     xa.nf = nf;
     RC(run_xfwd(ctx, xa));
     RC(run_ypass(ctx, -1, 0, nf));
-    ZArgsD za = zargs(ctx, materialize && !ctx->c.acoustic);
+    ZArgs<double> za = zargs(ctx, materialize && !ctx->c.acoustic);
     RC(run_zpass_g<ZG_DD>(ctx, za));
     if (ctx->c.acoustic) RC(run_zpass_g<ZG_ACOU>(ctx, za));
     else RC(run_zpass_g<ZG_STRAIN>(ctx, za));
@@ -291,11 +296,10 @@ int force_pipeline(fhip_ctx *ctx, int kind, bool materialize)
     RC(run_ypass(ctx, +1, 0, nf));
     XinvArgs ia{};
     ia.mode = (kind == SK_NP) ? XI_NP : (kind == SK_LF ? XI_LF : (kind == SK_VS ? XI_VS : XI_FORCE));
-    ia.nf = nf;
     ia.materialize = materialize ? 1 : 0;
     ia.partials = ctx->partials;
     ia.maxblocks = ctx->maxblocks;
-    RC(run_xinv(ctx, ia));
+    RC(run_xinv(ctx, ia, nf));
     {
         ScalarConsts k = ctx->sk;
         Scalars *sc = ctx->dsc;
@@ -304,7 +308,7 @@ int force_pipeline(fhip_ctx *ctx, int kind, bool materialize)
         int mode = (kind == SK_NP) ? FIN_NP : (kind == SK_LF ? FIN_LF : (kind == SK_VS ? FIN_VS : FIN_FORCE));
         int ac = ctx->c.acoustic;
         hipStream_t s = ctx->stream;
-        RC(launch(ctx, K_FINISH, [&] { k_finish<<<1, NT, 0, s>>>(k, sc, part, mb, xb, mode, ac); }));
+        RC(launch(ctx, K_FINISH, [&] { k_finish<<<1, NT, 0, s>>>(k, sc, part, mb, xb, mode, ac, next_step); }));
     }
     return FHIP_OK;
 }
@@ -380,12 +384,16 @@ void fill_consts(fhip_ctx *ctx)
     const fhip_params &p = ctx->p;
     Consts &c = ctx->c;
     c.Lx = p.Lx; c.Ly = p.Ly; c.Lz = p.Lz;
-    c.Kx = ctx->Kx; c.N = ctx->N; c.Kh = ctx->Kh;
+    c.Mx = ctx->Mx; c.Mh = ctx->Mh; c.N = ctx->N; c.Kh = ctx->Kh;
+    {
+        const int ty = tile_rows(ctx->Ly), tz = tile_rows(ctx->Lz);
+        c.ylines_main = (ctx->Mx * ctx->Lz + ty - 1) / ty * ty;
+        c.zlines_main = (ctx->Mx * ctx->Ly + tz - 1) / tz * tz;
+    }
     c.acoustic = p.mass_acou[0] > 0.0;
     c.efield = !p.E_field_always_zero;
     c.has_eloc = 0;
     c.modulation = 0;
-    c.n_sum_p = ctx->N;
     c.Ninv = 1.0 / ctx->N;
     c.dt = p.dt;
     c.dt_2 = p.dt / 2;
@@ -482,7 +490,7 @@ int forward_generic(fhip_ctx *ctx, const double *const *fields, int nf)
     for (int i = 0; i < nf; ++i) xa.gen[i] = fields[i];
     RC(run_xfwd(ctx, xa));
     RC(run_ypass(ctx, -1, 0, nf));
-    ZArgsD za = zargs(ctx, false);
+    ZArgs<double> za = zargs(ctx, false);
     za.nplain = nf;
     za.plain_sign = -1;
     return run_zpass_g<ZG_PLAIN>(ctx, za);
@@ -516,7 +524,7 @@ int build_long_range(fhip_ctx *ctx)
         Spectra<double> sp = ctx->sp;
         double *L = ctx->Ltab;
         const int kb = (ctx->Kh + NT - 1) / NT;
-        rc = launch(ctx, K_SETUP, [&] { k_ewald_recip<<<kb, NT, 0, s>>>(c, e, sp, L); });
+        rc = launch(ctx, K_SETUP, [&] { k_ewald_recip<double><<<kb, NT, 0, s>>>(c, e, sp, L, nullptr); });
     }
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
     hipFree(raw);
@@ -589,14 +597,16 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     ctx->p = *p;
     ctx->Lx = p->Lx; ctx->Ly = p->Ly; ctx->Lz = p->Lz;
     ctx->Kx = p->Lx / 2 + 1;
+    ctx->Mx = p->Lx / 2;
+    ctx->Mh = (size_t)ctx->Mx * p->Ly * p->Lz;
     ctx->N = p->Lx * p->Ly * p->Lz;
     ctx->Kh = ctx->Kx * p->Ly * p->Lz;
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     fill_consts(ctx);
     ctx->nslots = ctx->c.acoustic ? 12 : 9;
     ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
-    ctx->yblocks = (ctx->Kx * ctx->Lz + tile_rows(ctx->Ly) - 1) / tile_rows(ctx->Ly);
-    ctx->zblocks = (ctx->Kx * ctx->Ly + tile_rows(ctx->Lz) - 1) / tile_rows(ctx->Lz);
+    ctx->yblocks = (ctx->c.ylines_main + ctx->Lz + tile_rows(ctx->Ly) - 1) / tile_rows(ctx->Ly);
+    ctx->zblocks = (ctx->c.zlines_main + ctx->Ly + tile_rows(ctx->Lz) - 1) / tile_rows(ctx->Lz);
     ctx->kblocks = (ctx->N + NT - 1) / NT;
     ctx->maxblocks = std::max(std::max(ctx->xblocks, ctx->zblocks), ctx->kblocks);
 
@@ -667,9 +677,9 @@ int fhip_set_long_range(fhip_ctx *ctx, const double *lr)
     for (int z = 0; z < Lz; ++z)
         for (int y = 0; y < Ly; ++y)
             for (int x = 0; x < Kx; ++x) {
-                const size_t k = x + (size_t)Kx * (y + (size_t)Ly * z);
+                const size_t k = kindex(ctx->Mx, Ly, ctx->Mh, x, y, z);
                 const size_t base = 9 * (x + (size_t)Kx * (y + dy * z));
-                for (int m = 0; m < 6; ++m) h[(size_t)m * ctx->Kh + k] = lr[base + ia[m] + 3 * ib[m]];
+                for (int m = 0; m < 6; ++m) h[k * 6 + m] = lr[base + ia[m] + 3 * ib[m]];
             }
     HIPCHK(hipMemcpyAsync(ctx->Ltab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -688,10 +698,10 @@ int fhip_get_long_range(fhip_ctx *ctx, double *lr)
     for (int z = 0; z < Lz; ++z)
         for (int y = 0; y < Ly; ++y)
             for (int x = 0; x < Kx; ++x) {
-                const size_t k = x + (size_t)Kx * (y + (size_t)Ly * z);
+                const size_t k = kindex(ctx->Mx, Ly, ctx->Mh, x, y, z);
                 const size_t base = 9 * (x + (size_t)Kx * (y + dy * z));
                 for (int m = 0; m < 6; ++m) {
-                    const double v = h[(size_t)m * ctx->Kh + k];
+                    const double v = h[k * 6 + m];
                     lr[base + ia[m] + 3 * ib[m]] = v;
                     lr[base + ib[m] + 3 * ia[m]] = v;
                 }
@@ -750,7 +760,7 @@ int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, 
     if (nsum == 0) return fail(ctx, FHIP_ERR_ARG, "no counted cell (sum_p all false)");
     HIPCHK(hipMemcpyAsync(ctx->code, h.data(), ctx->N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->c.n_sum_p = nsum;
+
     ctx->sk.n_sum_p = nsum;
     return FHIP_OK;
 }
@@ -905,17 +915,20 @@ int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e)
     if (!ctx) return FHIP_ERR_ARG;
     if (nsteps < 1) return fail(ctx, FHIP_ERR_ARG, "nsteps must be >= 1");
     if (ctx->p.method != FHIP_MD) return fail(ctx, FHIP_ERR_STATE, "context was created for another method");
+    // first half kick and eq. 38 of the first step; later steps get theirs from
+    // the fused kick in xinv and the scalar tail of k_finish
     RC(launch_kick(ctx, true));
+    {
+        ScalarConsts k = ctx->sk;
+        Scalars *sc = ctx->dsc;
+        double *part = ctx->partials;
+        int mb = ctx->maxblocks, kb = ctx->kblocks;
+        hipStream_t s = ctx->stream;
+        RC(launch(ctx, K_NPBEGIN, [&] { k_np_begin<<<1, NT, 0, s>>>(k, sc, part, mb, kb); }));
+    }
     for (int i = 0; i < nsteps; ++i) {
-        {
-            ScalarConsts k = ctx->sk;
-            Scalars *sc = ctx->dsc;
-            double *part = ctx->partials;
-            int mb = ctx->maxblocks, kb = ctx->kblocks, fold_ke = (i == 0);
-            hipStream_t s = ctx->stream;
-            RC(launch(ctx, K_NPBEGIN, [&] { k_np_begin<<<1, NT, 0, s>>>(k, sc, part, mb, kb, fold_ke); }));
-        }
-        RC(force_pipeline(ctx, SK_NP, i == nsteps - 1));
+        const bool last = (i == nsteps - 1);
+        RC(force_pipeline(ctx, SK_NP, last, last ? 0 : 1));
     }
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
@@ -991,16 +1004,6 @@ int fhip_synchronize(fhip_ctx *ctx)
 
 // w reconstructed from the optimised w(k): acouR = c2r(-i acouK)/N
 // (coord_module.F:457-469).
-__global__ void k_prep_w(Consts c, Spectra<double> sp, const cpx<double> *acouK)
-{
-    const size_t kk = (size_t)blockIdx.x * NT + threadIdx.x;
-    if (kk >= (size_t)c.Kh) return;
-    for (int q = 0; q < 3; ++q) {
-        const cpx<double> a = acouK[(size_t)q * c.Kh + kk];
-        sp.s[q][kk] = cpx<double>{a.y, -a.x};
-    }
-}
-
 static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
 {
     if (ctx->c.acoustic) return fail(ctx, FHIP_ERR_ARG, "w is dynamical here: read FHIP_ACOU_R");
@@ -1015,9 +1018,9 @@ static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
     const cpx<double> *aK = ctx->acouK;
     hipStream_t s = ctx->stream;
     const int kb = (ctx->Kh + NT - 1) / NT;
-    int rc = launch(ctx, K_SETUP, [&] { k_prep_w<<<kb, NT, 0, s>>>(c, sp, aK); });
+    int rc = launch(ctx, K_SETUP, [&] { k_prep_w<double><<<kb, NT, 0, s>>>(c, sp, aK); });
     if (rc == FHIP_OK) {
-        ZArgsD za = zargs(ctx, false);
+        ZArgs<double> za = zargs(ctx, false);
         za.nplain = 3;
         za.plain_sign = +1;
         rc = run_zpass_g<ZG_PLAIN>(ctx, za);
@@ -1026,11 +1029,10 @@ static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
     if (rc == FHIP_OK) {
         XinvArgs ia{};
         ia.mode = XI_GENERIC;
-        ia.nf = 3;
         ia.partials = ctx->partials;
         ia.maxblocks = ctx->maxblocks;
         for (int q = 0; q < 3; ++q) ia.gen[q] = tmp + q * N;
-        rc = run_xinv(ctx, ia);
+        rc = run_xinv(ctx, ia, 3);
     }
     if (rc == FHIP_OK) {
         hipError_t e = hipMemcpyAsync(h.data(), tmp, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s);

@@ -87,7 +87,7 @@ struct radix_of {
     }
 };
 
-template <typename T, int N, int TB, int NT, int R, int NS, int SIGN>
+template <typename T, int N, int TB, int NT, int R, int NS, int SIGN, int ST>
 __device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
 {
     constexpr int M = N / R;
@@ -101,7 +101,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restr
             const int j = bf / TB, b = bf - j * TB;
             const int k = j % NS;
 #pragma unroll
-            for (int q = 0; q < R; ++q) v[i][q] = buf[(j + q * M) * TB + b];
+            for (int q = 0; q < R; ++q) v[i][q] = buf[(j + q * M) * ST + b];
             if constexpr (NS > 1) {
 #pragma unroll
                 for (int q = 1; q < R; ++q) {
@@ -122,22 +122,24 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restr
             const int k = j % NS;
             const int idxD = (j / NS) * NS * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) buf[(idxD + q * NS) * TB + b] = v[i][q];
+            for (int q = 0; q < R; ++q) buf[(idxD + q * NS) * ST + b] = v[i][q];
         }
     }
     __syncthreads();
 }
 
-// Full in-LDS transform of TB interleaved lines of length N.  Callers
-// synchronise before (data in LDS) and may read the result right after.
-template <typename T, int N, int TB, int NT, int SIGN, int NS = 1>
+// Full in-LDS transform of TB interleaved lines of length N, element t of
+// line b at buf[t*ST + b] (ST >= TB; a stride of TB+1 spreads the row-wise
+// accesses of the x passes over all LDS banks).  Callers synchronise before
+// (data in LDS) and may read the result right after.
+template <typename T, int N, int TB, int NT, int SIGN, int ST = TB, int NS = 1>
 __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
 {
     if constexpr (NS < N) {
         constexpr int R = radix_of<N>::template at<NS>();
         static_assert(R > 1, "FFT length must factor into 2, 3 and 5");
-        stockham_pass<T, N, TB, NT, R, NS, SIGN>(buf, tw, tid);
-        fft_lds<T, N, TB, NT, SIGN, NS * R>(buf, tw, tid);
+        stockham_pass<T, N, TB, NT, R, NS, SIGN, ST>(buf, tw, tid);
+        fft_lds<T, N, TB, NT, SIGN, ST, NS * R>(buf, tw, tid);
     }
 }
 

@@ -1,8 +1,12 @@
 // fhip_kernels.hpp -- the per-step kernels of the force-and-integrate path.
 //
 // Device layout (structure of arrays, unpadded, x fastest):
-//   real fields   f[c],        c = x + Lx*(y + Ly*z)            (one array per component)
-//   half spectra  s[k],        k = kx + Kx*(ky + Ly*kz), Kx = Lx/2+1   (cpx<T>)
+//   real fields   f[c],  c = x + Lx*(y + Ly*z)                 (one array per component)
+//   half spectra  Mx = Lx/2 columns kx = 0..Mx-1 in rows of Mx points,
+//                 s[(kz*Ly + ky)*Mx + kx], followed by the Nyquist column
+//                 kx = Mx at s[Mh + kz*Ly + ky], Mh = Mx*Ly*Lz.  Rows of
+//                 Mx points keep every tile of the y and z passes aligned to
+//                 whole cache lines.
 // Spectral slots of one step:
 //   0..2  u_x,u_y,u_z        -> (S + Z*^2 L) u   (dipole-dipole force before 1/N)
 //   3..8  uu_xx..uu_xy       -> eta(k) = k (x) w(k)  (inhomogeneous strain)
@@ -24,12 +28,14 @@ enum PartialSlot {
 
 // Constants shared by all kernels (by value in the kernel arguments).
 struct Consts {
-    int Lx, Ly, Lz, Kx, N, Kh;
+    int Lx, Ly, Lz, Mx, N, Kh;
+    size_t Mh;           // Mx*Ly*Lz: start of the Nyquist column
+    int ylines_main;     // Mx*Lz rounded up to whole y tiles
+    int zlines_main;     // Mx*Ly rounded up to whole z tiles
     int acoustic;        // mass_acou > 0: w is dynamical
     int efield;          // !E_field_always_zero
     int has_eloc;        // per-cell local field present
     int modulation;      // modulation_constant != 0 with a .modulation field
-    int n_sum_p;
     double Ninv, dt, dt_2, a0, Z2, Zstar;
     double kappa2[3], fa2g[3], tg[3], sk[3], tk2[3], fk[3], k4x8[3];
     double alpha[3], gamma[3], k1[3], k2[3], k3[3], k4[3];
@@ -73,6 +79,12 @@ struct Spectra {
     cpx<T>* s[MAX_SLOTS];
 };
 
+// Index of half-spectrum point (kx, ky, kz) in the split layout.
+__host__ __device__ __forceinline__ size_t kindex(int Mx, int Ly, size_t Mh, int kx, int ky, int kz)
+{
+    return kx < Mx ? ((size_t)kz * Ly + ky) * Mx + kx : Mh + (size_t)kz * Ly + ky;
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red)
 {
     const int tid = threadIdx.x;
@@ -106,13 +118,16 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
                                              const cpx<T>* __restrict__ twx)
 {
     constexpr int RB = tile_of<LX>::value;
+    constexpr int ST = RB + 1;
     constexpr int CELLS = RB * LX;
     constexpr int CPT = (CELLS + NT - 1) / NT;
-    __shared__ cpx<T> buf[CELLS];
+    constexpr int MX = LX / 2;
+    __shared__ cpx<T> buf[LX * ST];
+    __shared__ cpx<T> tws[LX];
     const int tid = threadIdx.x;
     const int nrows = c.Ly * c.Lz;
     const int row0 = blockIdx.x * RB;
-    const int Kx = c.Kx;
+    for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
     double u[CPT][3], w[CPT][3];
     bool ok[CPT];
@@ -186,6 +201,7 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
             const int r = e / LX, x = e - r * LX;
             double va = 0.0, vb = 0.0;
             if (ok[i]) {
+#pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int fi = f0 + h;
                     if (h == 1 && !two) break;
@@ -193,29 +209,32 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
                     if (a.mode == XF_GENERIC) {
                         v = a.gen[fi][cell[i]];
                     } else if (fi < 3) {
-                        v = u[i][fi];
+                        v = fi == 0 ? u[i][0] : (fi == 1 ? u[i][1] : u[i][2]);
                     } else if (fi < 9) {                                 // dipoFFT.F:20-33
                         const int m = fi - 3;
-                        const int aa = (m < 3) ? m : (m == 3 ? 1 : (m == 4 ? 2 : 0));
-                        const int bb = (m < 3) ? m : (m == 3 ? 2 : (m == 4 ? 0 : 1));
-                        v = u[i][aa] * u[i][bb];
+                        const double ua = (m == 0 || m == 5) ? u[i][0] : (m == 1 || m == 3) ? u[i][1] : u[i][2];
+                        const double ub = (m == 0 || m == 4) ? u[i][0] : (m == 1 || m == 5) ? u[i][1] : u[i][2];
+                        v = ua * ub;
                     } else {
-                        v = w[i][fi - 9];
+                        const int q = fi - 9;
+                        v = q == 0 ? w[i][0] : (q == 1 ? w[i][1] : w[i][2]);
                     }
                     if (h == 0) va = v; else vb = v;
                 }
             }
-            buf[x * RB + r] = cpx<T>{T(va), T(vb)};
+            buf[x * ST + r] = cpx<T>{T(va), T(vb)};
         }
         __syncthreads();
-        fft_lds<T, LX, RB, NT, -1>(buf, twx, tid);
+        fft_lds<T, LX, RB, NT, -1, ST>(buf, tws, tid);
         // unpack Z = A + iB: A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] - conj Z[-k])/(2i)
-        for (int e = tid; e < RB * Kx; e += NT) {
-            const int r = e / Kx, kx = e - r * Kx;
+        for (int e = tid; e < RB * (MX + 1); e += NT) {
+            int r, kx;
+            if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
+            else { r = e - RB * MX; kx = MX; }
             if (row0 + r >= nrows) continue;
-            const cpx<T> zk = buf[kx * RB + r];
-            const cpx<T> zn = buf[((LX - kx) % LX) * RB + r];
-            const size_t o = (size_t)(row0 + r) * Kx + kx;
+            const cpx<T> zk = buf[kx * ST + r];
+            const cpx<T> zn = buf[((LX - kx) % LX) * ST + r];
+            const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : c.Mh + (size_t)(row0 + r);
             sp.s[f0][o] = cpx<T>{(zk.x + zn.x) * T(0.5), (zk.y - zn.y) * T(0.5)};
             if (two) sp.s[f0 + 1][o] = cpx<T>{(zk.y + zn.y) * T(0.5), (zn.x - zk.x) * T(0.5)};
         }
@@ -224,56 +243,99 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
 }
 
 // ---------------------------------------------------------------------------
-// Line transform along y (stride Kx), in place, one field per blockIdx.y.
+// Line transform along y, in place, one field per blockIdx.y.  Lines
+// l < ylines_main are (z, kx<Mx) columns of stride Mx; lines from
+// ylines_main on are the Nyquist column of plane z (contiguous).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool yline(const Consts& c, int l, size_t& base, int& stride)
+{
+    if (l < c.ylines_main) {
+        if (l >= c.Mx * c.Lz) return false;
+        const int z = l / c.Mx, kx = l - z * c.Mx;
+        base = (size_t)z * c.Ly * c.Mx + kx;
+        stride = c.Mx;
+        return true;
+    }
+    const int z = l - c.ylines_main;
+    if (z >= c.Lz) return false;
+    base = c.Mh + (size_t)z * c.Ly;
+    stride = 1;
+    return true;
+}
+
 template <typename T, int LY, int SIGN>
 __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0,
                                               const cpx<T>* __restrict__ twy)
 {
     constexpr int TB = tile_of<LY>::value;
     __shared__ cpx<T> buf[LY * TB];
+    __shared__ cpx<T> tws[LY];
     const int tid = threadIdx.x;
-    const int Kx = c.Kx;
-    const int nlines = Kx * c.Lz;
+    for (int i = tid; i < LY; i += NT) tws[i] = twy[i];
     const int l0 = blockIdx.x * TB;
     cpx<T>* s = sp.s[slot0 + blockIdx.y];
+    const bool nyq = l0 >= c.ylines_main;
     for (int e = tid; e < LY * TB; e += NT) {
-        const int t = e / TB, b = e - t * TB;
-        const int l = l0 + b;
-        if (l < nlines) {
-            const int z = l / Kx, kx = l - z * Kx;
-            buf[e] = s[((size_t)z * LY + t) * Kx + kx];
-        }
+        // main tiles: consecutive threads walk consecutive kx; Nyquist tiles:
+        // consecutive threads walk y, which is contiguous there
+        const int t = nyq ? e % LY : e / TB, b = nyq ? e / LY : e - (e / TB) * TB;
+        size_t base; int stride;
+        if (yline(c, l0 + b, base, stride)) buf[t * TB + b] = s[base + (size_t)t * stride];
     }
     __syncthreads();
-    fft_lds<T, LY, TB, NT, SIGN>(buf, twy, tid);
+    fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
     for (int e = tid; e < LY * TB; e += NT) {
-        const int t = e / TB, b = e - t * TB;
-        const int l = l0 + b;
-        if (l < nlines) {
-            const int z = l / Kx, kx = l - z * Kx;
-            s[((size_t)z * LY + t) * Kx + kx] = buf[e];
-        }
+        const int t = nyq ? e % LY : e / TB, b = nyq ? e / LY : e - (e / TB) * TB;
+        size_t base; int stride;
+        if (yline(c, l0 + b, base, stride)) s[base + (size_t)t * stride] = buf[t * TB + b];
     }
 }
 
 // ---------------------------------------------------------------------------
 // z pass: forward transform along z, k-space stage, inverse transform.
+// Lines l < zlines_main are (ky, kx<Mx) pencils of stride Mx*Ly; lines from
+// zlines_main on are the Nyquist pencils (ky) of stride Ly.
 // ---------------------------------------------------------------------------
 enum ZGroup { ZG_PLAIN = 0, ZG_DD = 1, ZG_STRAIN = 2, ZG_ACOU = 3 };
 
-struct ZArgsD {
-    const double* Ltab;
-    const cpx<double>* csx;
+template <typename T>
+struct ZArgs {
+    const T* Ltab;                   // [Kh][6] long-range kernel L(k): xx yy zz yz zx xy
+    const cpx<double>* csx;          // cos/sin tables (2 pi i / L) per axis
     const cpx<double>* csy;
     const cpx<double>* csz;
-    const cpx<double>* modk;         // FFT of the modulation field (or null)
+    const cpx<T>* modk;              // FFT of the modulation field (or null)
     cpx<double>* acouK;              // materialised optimised w(k) (or null)
     double* partials;                // [P_NSLOT][maxblocks]
     int maxblocks;
     int nplain;                      // ZG_PLAIN: number of slots
     int plain_sign;                  // ZG_PLAIN: -1 forward, +1 inverse
 };
+
+struct ZLine {
+    size_t base, stride;
+    int kx, ky;
+    bool ok;
+};
+
+__device__ __forceinline__ ZLine zline(const Consts& c, int l)
+{
+    ZLine z;
+    if (l < c.zlines_main) {
+        z.ok = l < c.Mx * c.Ly;
+        z.ky = l / c.Mx;
+        z.kx = l - z.ky * c.Mx;
+        z.base = (size_t)l;
+        z.stride = (size_t)c.Mx * c.Ly;
+        return z;
+    }
+    z.ky = l - c.zlines_main;
+    z.ok = z.ky < c.Ly;
+    z.kx = c.Mx;
+    z.base = c.Mh + (size_t)z.ky;
+    z.stride = (size_t)c.Ly;
+    return z;
+}
 
 // k vector of the half-spectrum point (elastic.F:83-89)
 __device__ __forceinline__ void kvec(const Consts& c, int kx, int ky, int kz, double k[3])
@@ -300,7 +362,7 @@ __device__ __forceinline__ double coup2(const Consts& c, const double k[3], int 
 }
 
 template <typename T, int LZ, int GROUP>
-__global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a, Scalars* sc,
+__global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
                                               const cpx<T>* __restrict__ twz)
 {
     constexpr int TB = tile_of<LZ>::value;
@@ -308,43 +370,64 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
     constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
     constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
     __shared__ cpx<T> buf[LZ * TB];
+    __shared__ cpx<T> tws[LZ];
     __shared__ double red[NT];
     const int tid = threadIdx.x;
-    const int Kx = c.Kx;
-    const size_t plane = (size_t)Kx * c.Ly;
-    const int nlines = (int)plane;
     const int l0 = blockIdx.x * TB;
+    for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
+
+    // consecutive lines are consecutive in HBM for both kinds of pencils
+#define FHIP_LOAD_TILE(SRC)                                                    \
+    for (int e = tid; e < LZ * TB; e += NT) {                                  \
+        const int t = e / TB, b = e - t * TB;                                  \
+        const ZLine zl = zline(c, l0 + b);                                     \
+        buf[e] = zl.ok ? (SRC)[zl.base + (size_t)t * zl.stride] : cpx<T>{T(0), T(0)}; \
+    }
+#define FHIP_STORE_TILE(DST)                                                   \
+    for (int e = tid; e < LZ * TB; e += NT) {                                  \
+        const int t = e / TB, b = e - t * TB;                                  \
+        const ZLine zl = zline(c, l0 + b);                                     \
+        if (zl.ok) (DST)[zl.base + (size_t)t * zl.stride] = buf[e];            \
+    }
 
     if constexpr (GROUP == ZG_PLAIN) {
         for (int sl = 0; sl < a.nplain; ++sl) {
-            cpx<T>* s = sp.s[sl];
-            for (int e = tid; e < LZ * TB; e += NT) {
-                const int t = e / TB, b = e - t * TB;
-                if (l0 + b < nlines) buf[e] = s[(size_t)t * plane + l0 + b];
-            }
+            FHIP_LOAD_TILE(sp.s[sl]);
             __syncthreads();
-            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1>(buf, twz, tid);
-            else fft_lds<T, LZ, TB, NT, +1>(buf, twz, tid);
-            for (int e = tid; e < LZ * TB; e += NT) {
-                const int t = e / TB, b = e - t * TB;
-                if (l0 + b < nlines) s[(size_t)t * plane + l0 + b] = buf[e];
-            }
+            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1>(buf, tws, tid);
+            else fft_lds<T, LZ, TB, NT, +1>(buf, tws, tid);
+            FHIP_STORE_TILE(sp.s[sl]);
             __syncthreads();
         }
         return;
     } else {
         cpx<double> v[NF][PPT];
-        // forward transforms, results kept in registers
+        // forward transforms, results kept in registers; the tile of the next
+        // field is fetched into registers while the current one is transformed
+        cpx<T> pf[PPT];
+        auto fetch = [&](const cpx<T>* __restrict__ src) {
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int e = tid + i * NT;
+                pf[i] = cpx<T>{T(0), T(0)};
+                if ((LZ * TB) % NT == 0 || e < LZ * TB) {
+                    const int t = e / TB, b = e - t * TB;
+                    const ZLine zl = zline(c, l0 + b);
+                    if (zl.ok) pf[i] = src[zl.base + (size_t)t * zl.stride];
+                }
+            }
+        };
+        fetch(sp.s[F0]);
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi) {
-            const cpx<T>* s = sp.s[F0 + fi];
-            for (int e = tid; e < LZ * TB; e += NT) {
-                const int t = e / TB, b = e - t * TB;
-                if (l0 + b < nlines) buf[e] = s[(size_t)t * plane + l0 + b];
-                else buf[e] = cpx<T>{T(0), T(0)};
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int e = tid + i * NT;
+                if ((LZ * TB) % NT == 0 || e < LZ * TB) buf[e] = pf[i];
             }
             __syncthreads();
-            fft_lds<T, LZ, TB, NT, -1>(buf, twz, tid);
+            if (fi + 1 < NF) fetch(sp.s[F0 + fi + 1]);
+            fft_lds<T, LZ, TB, NT, -1>(buf, tws, tid);
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
@@ -362,11 +445,11 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
             const int e = tid + i * NT;
             if (!((LZ * TB) % NT == 0 || e < LZ * TB)) continue;
             const int kz = e / TB, b = e - kz * TB;
-            const int l = l0 + b;
-            if (l >= nlines) continue;
-            const int ky = l / Kx, kx = l - ky * Kx;
-            const size_t kk = (size_t)kz * plane + l;
-            const bool edge = (kx == 0) || (2 * kx == c.Lx);
+            const ZLine zl = zline(c, l0 + b);
+            if (!zl.ok) continue;
+            const int kx = zl.kx, ky = zl.ky;
+            const size_t kk = zl.base + (size_t)kz * zl.stride;
+            const bool edge = (kx == 0) || (kx == c.Mx);
             if constexpr (GROUP == ZG_DD) {
                 // S(k): Fourier transform of the short-range couplings
                 // (dipole-dipole-short-range.F:35-83) in closed form.
@@ -384,8 +467,9 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
                 S[4] = -4 * j5 * szz * sx - 8 * j7 * szz * sx * cy;   // zx
                 S[5] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz;     // xy
                 double L[6];
+                const T* lt = a.Ltab + kk * 6;
 #pragma unroll
-                for (int m = 0; m < 6; ++m) L[m] = a.Ltab[(size_t)m * c.Kh + kk] * c.Z2;
+                for (int m = 0; m < 6; ++m) L[m] = (double)lt[m] * c.Z2;
                 const cpx<double> ux = v[0][i], uy = v[1][i], uz = v[2][i];
                 // F = (S + Z^2 L) u ; energies via Parseval with weight 1 (edge) or 2
                 cpx<double> fs[3], fl[3];
@@ -416,14 +500,15 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
                 }
                 // coupling rhs: c_mod + C uu  (optimize-inho-strain.F:36-42, dVdac.F:46-52)
                 cpx<double> cm[3], Cu[3];
+                cpx<double> mk = {0.0, 0.0};
+                if (c.modulation) {
+                    const cpx<T> m0 = a.modk[kk];
+                    mk = {(double)m0.x, (double)m0.y};
+                }
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    cm[q] = {0.0, 0.0};
-                    if (c.modulation) {
-                        const cpx<double> mk = a.modk[kk];
-                        const double fq = c.mod_c * k[q];
-                        cm[q] = {fq * mk.x, fq * mk.y};
-                    }
+                    const double fq = c.mod_c * k[q];
+                    cm[q] = c.modulation ? cpx<double>{fq * mk.x, fq * mk.y} : cpx<double>{0.0, 0.0};
                     double rr = 0.0, ri = 0.0;
 #pragma unroll
                     for (int m = 0; m < 6; ++m) {
@@ -457,11 +542,14 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
 #pragma unroll
                         for (int bb = 0; bb < 3; ++bb) {
                             double s2 = 0.0;
+#pragma unroll
                             for (int l2 = 0; l2 < bb; ++l2) s2 += G[bb][l2] * G[bb][l2];
                             G[bb][bb] = sqrt(K[bb][bb] - s2);
                             Gi[bb] = 1.0 / G[bb][bb];
+#pragma unroll
                             for (int aa = bb + 1; aa < 3; ++aa) {
                                 double s3 = 0.0;
+#pragma unroll
                                 for (int l2 = 0; l2 < bb; ++l2) s3 += G[aa][l2] * G[bb][l2];
                                 G[aa][bb] = (K[aa][bb] - s3) * Gi[bb];
                             }
@@ -471,12 +559,14 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
 #pragma unroll
                         for (int aa = 0; aa < 3; ++aa) {
                             double tr = 0.0, ti = 0.0;
+#pragma unroll
                             for (int bb = 0; bb < aa; ++bb) { tr += G[aa][bb] * w[bb].x; ti += G[aa][bb] * w[bb].y; }
                             w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
                         }
 #pragma unroll
                         for (int aa = 2; aa >= 0; --aa) {
                             double tr = 0.0, ti = 0.0;
+#pragma unroll
                             for (int bb = aa + 1; bb < 3; ++bb) { tr += w[bb].x * G[bb][aa]; ti += w[bb].y * G[bb][aa]; }
                             w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
                         }
@@ -533,14 +623,12 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
                 if ((LZ * TB) % NT == 0 || e < LZ * TB) buf[e] = cpx<T>{T(v[fi][i].x), T(v[fi][i].y)};
             }
             __syncthreads();
-            fft_lds<T, LZ, TB, NT, +1>(buf, twz, tid);
-            cpx<T>* s = sp.s[F0 + fi];
-            for (int e = tid; e < LZ * TB; e += NT) {
-                const int t = e / TB, b = e - t * TB;
-                if (l0 + b < nlines) s[(size_t)t * plane + l0 + b] = buf[e];
-            }
+            fft_lds<T, LZ, TB, NT, +1>(buf, tws, tid);
+            FHIP_STORE_TILE(sp.s[F0 + fi]);
             __syncthreads();
         }
+#undef FHIP_LOAD_TILE
+#undef FHIP_STORE_TILE
         const int nb = a.maxblocks;
         if constexpr (GROUP == ZG_DD) {
             double r0 = block_sum(acc0, red), r1 = block_sum(acc1, red);
@@ -560,64 +648,89 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgsD a,
 }
 
 // ---------------------------------------------------------------------------
-// xinv: c2r along x of every inverse field, then the local forces, energies
+// xinv: c2r along x of the NF inverse fields, then the local forces, energies
 // and the kick.
 // ---------------------------------------------------------------------------
 enum XinvMode { XI_FORCE = 0, XI_NP = 1, XI_LF = 2, XI_VS = 3, XI_GENERIC = 4 };
 
 struct XinvArgs {
     int mode;
-    int nf;                      // inverse fields (9 or 12)
     int materialize;             // write F (and p for NP, dV/dw) for the host
     double* partials;
     int maxblocks;
     double* gen[12];             // XI_GENERIC: real outputs (unnormalised c2r)
 };
 
-template <typename T, int LX>
+template <typename T, int LX, int NF>
 __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, XinvArgs a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* __restrict__ twx)
 {
     constexpr int RB = tile_of<LX>::value;
+    constexpr int ST = RB + 1;
     constexpr int CELLS = RB * LX;
     constexpr int CPT = (CELLS + NT - 1) / NT;
-    __shared__ cpx<T> buf[CELLS];
+    constexpr int MX = LX / 2;
+    __shared__ cpx<T> buf[LX * ST];
+    __shared__ cpx<T> tws[LX];
     __shared__ double red[NT];
     const int tid = threadIdx.x;
     const int nrows = c.Ly * c.Lz;
     const int row0 = blockIdx.x * RB;
-    const int Kx = c.Kx;
-    const int nf = a.nf;
+    for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
-    double acc[CPT][12];
-    for (int f0 = 0; f0 < nf; f0 += 2) {
-        const bool two = (f0 + 1 < nf);
+    // half-spectrum points of this block's rows, read once each; the pair of
+    // fields after the current one is fetched while the current one transforms
+    constexpr int HP = RB * (MX + 1);
+    constexpr int HPT = (HP + NT - 1) / NT;
+    cpx<T> pa[HPT], pb[HPT];
+    auto fetch = [&](int f0) {
+        const bool two = (f0 + 1 < NF);
+#pragma unroll
+        for (int i = 0; i < HPT; ++i) {
+            const int e = tid + i * NT;
+            pa[i] = cpx<T>{T(0), T(0)};
+            pb[i] = cpx<T>{T(0), T(0)};
+            if (HP % NT != 0 && e >= HP) continue;
+            int r, kx;
+            if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
+            else { r = e - RB * MX; kx = MX; }
+            if (row0 + r < nrows) {
+                const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : c.Mh + (size_t)(row0 + r);
+                pa[i] = sp.s[f0][o];
+                if (two) pb[i] = sp.s[f0 + 1][o];
+            }
+        }
+    };
+
+    double acc[CPT][NF];
+    fetch(0);
+#pragma unroll
+    for (int f0 = 0; f0 < NF; f0 += 2) {
+        const bool two = (f0 + 1 < NF);
         // Z = A + i B with both half spectra completed Hermitian; Im of
         // kx = 0 and kx = Lx/2 is dropped, as a half-complex c2r does.
-        for (int e = tid; e < CELLS; e += NT) {
-            const int r = e / LX, kx = e - r * LX;
-            cpx<T> z = {T(0), T(0)};
-            if (row0 + r < nrows) {
-                const bool lo = kx < Kx;
-                const int ks = lo ? kx : LX - kx;
-                const size_t o = (size_t)(row0 + r) * Kx + ks;
-                cpx<T> A = sp.s[f0][o];
-                cpx<T> B = two ? sp.s[f0 + 1][o] : cpx<T>{T(0), T(0)};
-                if (ks == 0 || 2 * ks == LX) { A.y = T(0); B.y = T(0); }
-                if (!lo) { A.y = -A.y; B.y = -B.y; }
-                z = cpx<T>{A.x - B.y, A.y + B.x};
-            }
-            buf[kx * RB + r] = z;
+#pragma unroll
+        for (int i = 0; i < HPT; ++i) {
+            const int e = tid + i * NT;
+            if (HP % NT != 0 && e >= HP) continue;
+            int r, kx;
+            if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
+            else { r = e - RB * MX; kx = MX; }
+            cpx<T> A = pa[i], B = pb[i];
+            if (kx == 0 || kx == MX) { A.y = T(0); B.y = T(0); }
+            buf[kx * ST + r] = cpx<T>{A.x - B.y, A.y + B.x};
+            if (kx != 0 && kx != MX) buf[(LX - kx) * ST + r] = cpx<T>{A.x + B.y, B.x - A.y};
         }
         __syncthreads();
-        fft_lds<T, LX, RB, NT, +1>(buf, twx, tid);
+        if (f0 + 2 < NF) fetch(f0 + 2);
+        fft_lds<T, LX, RB, NT, +1, ST>(buf, tws, tid);
 #pragma unroll
         for (int i = 0; i < CPT; ++i) {
             const int e = tid + i * NT;
             if (CELLS % NT != 0 && e >= CELLS) continue;
             const int r = e / LX, x = e - r * LX;
-            const cpx<T> z = buf[x * RB + r];
+            const cpx<T> z = buf[x * ST + r];
             acc[i][f0] = (double)z.x;
             if (two) acc[i][f0 + 1] = (double)z.y;
         }
@@ -633,106 +746,109 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
         if (row0 + r >= nrows) continue;
         const size_t cc = (size_t)(row0 + r) * LX + x;
         if (a.mode == XI_GENERIC) {
-            for (int q = 0; q < nf; ++q) a.gen[q][cc] = acc[i][q];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) a.gen[q][cc] = acc[i][q];
             continue;
         }
-        const unsigned char code = f.code[cc];
-        const int s = code & 3;
-        const bool counted = (code & 8) != 0;
-        const double minv = (code & 4) ? 0.0 : c.minv[s];
-        double u[3], F[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            u[q] = f.u[q][cc];
-            F[q] = acc[i][q] * c.Ninv;                            // dVddi-dipole-dipole.F:34-38
-        }
-        // on-site anharmonic terms (dVddi-unharmonic.F:34-65)
-        const double d0 = u[0] * u[0], d1 = u[1] * u[1], d2 = u[2] * u[2];
-        const double d[3] = {d0, d1, d2};
-        const double ss = d0 + d1 + d2;
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const int q1 = (q + 1) % 3, q2 = (q + 2) % 3;
-            F[q] = F[q] + u[q] * 2 * (c.kappa2[s] - c.kappa2[0]);
-            F[q] = F[q] + u[q] * (c.fa2g[s] * ss - c.tg[s] * d[q]);
-            F[q] = F[q] + u[q] * (c.sk[s] * (ss * ss) + c.k4x8[s] * (ss * ss * ss)
-                                  - c.tk2[s] * (d[q] * d[q]) - c.fk[s] * d[q1] * d[q2]);
-        }
-        eunh += (c.kappa2[s] - c.kappa2[0]) * ss + c.alpha[s] * (ss * ss)
-              + c.gamma[s] * (d0 * d1 + d1 * d2 + d2 * d0) + c.k1[s] * (ss * ss * ss)
-              + c.k2[s] * ((d0 * d0) * (d1 + d2) + (d1 * d1) * (d2 + d0) + (d2 * d2) * (d0 + d1))
-              + c.k3[s] * d0 * d1 * d2 + c.k4[s] * ((ss * ss) * (ss * ss));
-        // external and local field (mirror_force_mirror.F:39-65)
-        if (c.efield) {
-            double E[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) F[q] = F[q] - c.Zstar * E[q];
-            ede += c.Zstar * (u[0] * E[0] + u[1] * E[1] + u[2] * E[2]);
-        }
-        // homogeneous coupling (dVddi-homo-coupling.F:28-44)
-        {
-            const double* eh = sc->e_homo;
-            F[0] = F[0] + 2 * eh[0] * u[0] + eh[4] * u[2] + eh[5] * u[1];
-            F[1] = F[1] + 2 * eh[1] * u[1] + eh[3] * u[2] + eh[5] * u[0];
-            F[2] = F[2] + 2 * eh[2] * u[2] + eh[3] * u[1] + eh[4] * u[0];
-        }
-        // inhomogeneous coupling (dVddi_inho_coupling.F:54-82)
-        {
-            double ei[6];
-#pragma unroll
-            for (int m = 0; m < 6; ++m) {
-                double sum = 0.0;
-#pragma unroll
-                for (int jj = 0; jj < 6; ++jj) sum = sum + acc[i][3 + jj] * c.Ninv * c.hc2[jj + 6 * m];
-                ei[m] = sum;
-            }
-            F[0] = F[0] + 2 * ei[0] * u[0] + ei[4] * u[2] + ei[5] * u[1];
-            F[1] = F[1] + 2 * ei[1] * u[1] + ei[3] * u[2] + ei[5] * u[0];
-            F[2] = F[2] + 2 * ei[2] * u[2] + ei[3] * u[1] + ei[4] * u[0];
-        }
-        // kicks
-        if (a.mode == XI_NP) {
-            const double dt2s = c.dt_2 * sc->s;
+        if constexpr (NF >= 9) {
+            const unsigned char code = f.code[cc];
+            const int s = code & 3;
+            const bool counted = (code & 8) != 0;
+            const double minv = (code & 4) ? 0.0 : c.minv[s];
+            double u[3], F[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                const double p1 = f.pk[q][cc] - dt2s * F[q];            // nose-poincare.F:66-71
-                const double p2 = p1 - dt2s * F[q];                      // nose-poincare.F:24-29 (next step)
-                if (counted) { ke1 += p1 * p1 * minv; ke2 += p2 * p2 * minv; }
-                f.pk[q][cc] = p2;
-                if (a.materialize) f.p[q][cc] = p1;
+                u[q] = f.u[q][cc];
+                F[q] = acc[i][q] * c.Ninv;                            // dVddi-dipole-dipole.F:34-38
             }
-        } else if (a.mode == XI_LF) {
+            // on-site anharmonic terms (dVddi-unharmonic.F:34-65)
+            const double d0 = u[0] * u[0], d1 = u[1] * u[1], d2 = u[2] * u[2];
+            const double d[3] = {d0, d1, d2};
+            const double ss = d0 + d1 + d2;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                const double pq = f.p[q][cc] - c.dt * F[q];             // leapfrog.F:134-139
-                if (counted) ke1 += pq * pq * minv;
-                f.p[q][cc] = pq;
+                const int q1 = (q + 1) % 3, q2 = (q + 2) % 3;
+                F[q] = F[q] + u[q] * 2 * (c.kappa2[s] - c.kappa2[0]);
+                F[q] = F[q] + u[q] * (c.fa2g[s] * ss - c.tg[s] * d[q]);
+                F[q] = F[q] + u[q] * (c.sk[s] * (ss * ss) + c.k4x8[s] * (ss * ss * ss)
+                                      - c.tk2[s] * (d[q] * d[q]) - c.fk[s] * d[q1] * d[q2]);
             }
-        } else if (a.mode == XI_VS) {
+            eunh += (c.kappa2[s] - c.kappa2[0]) * ss + c.alpha[s] * (ss * ss)
+                  + c.gamma[s] * (d0 * d1 + d1 * d2 + d2 * d0) + c.k1[s] * (ss * ss * ss)
+                  + c.k2[s] * ((d0 * d0) * (d1 + d2) + (d1 * d1) * (d2 + d0) + (d2 * d2) * (d0 + d1))
+                  + c.k3[s] * d0 * d1 * d2 + c.k4[s] * ((ss * ss) * (ss * ss));
+            // external and local field (mirror_force_mirror.F:39-65)
+            if (c.efield) {
+                double E[3];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const double pq = counted ? f.p[q][cc] - c.dt * F[q] : 0.0;   // velocity_scaling.F:62-76
-                if (counted) ke1 += pq * pq * minv;
-                f.p[q][cc] = pq;
+                for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) F[q] = F[q] - c.Zstar * E[q];
+                ede += c.Zstar * (u[0] * E[0] + u[1] * E[1] + u[2] * E[2]);
             }
-        }
-        if (a.materialize || a.mode == XI_FORCE) {
+            // homogeneous coupling (dVddi-homo-coupling.F:28-44)
+            {
+                const double* eh = sc->e_homo;
+                F[0] = F[0] + 2 * eh[0] * u[0] + eh[4] * u[2] + eh[5] * u[1];
+                F[1] = F[1] + 2 * eh[1] * u[1] + eh[3] * u[2] + eh[5] * u[0];
+                F[2] = F[2] + 2 * eh[2] * u[2] + eh[3] * u[1] + eh[4] * u[0];
+            }
+            // inhomogeneous coupling (dVddi_inho_coupling.F:54-82)
+            {
+                double ei[6];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) f.F[q][cc] = F[q];
-        }
-        if (c.acoustic) {
-            const double mwinv = c.mwinv[s];
+                for (int m = 0; m < 6; ++m) {
+                    double sum = 0.0;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const double g = acc[i][9 + q] * c.Ninv;                  // dVdac.F:56-62
-                if (a.mode == XI_LF || a.mode == XI_VS) {
-                    const double pq = f.pw[q][cc] - c.dt * g;
-                    kea += pq * pq * mwinv;
-                    f.pw[q][cc] = pq;
+                    for (int jj = 0; jj < 6; ++jj) sum = sum + acc[i][3 + jj] * c.Ninv * c.hc2[jj + 6 * m];
+                    ei[m] = sum;
                 }
-                if (a.materialize || a.mode == XI_FORCE) f.dVdw[q][cc] = g;
+                F[0] = F[0] + 2 * ei[0] * u[0] + ei[4] * u[2] + ei[5] * u[1];
+                F[1] = F[1] + 2 * ei[1] * u[1] + ei[3] * u[2] + ei[5] * u[0];
+                F[2] = F[2] + 2 * ei[2] * u[2] + ei[3] * u[1] + ei[4] * u[0];
+            }
+            // kicks
+            if (a.mode == XI_NP) {
+                const double dt2s = c.dt_2 * sc->s;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double p1 = f.pk[q][cc] - dt2s * F[q];            // nose-poincare.F:66-71
+                    const double p2 = p1 - dt2s * F[q];                      // nose-poincare.F:24-29 (next step)
+                    if (counted) { ke1 += p1 * p1 * minv; ke2 += p2 * p2 * minv; }
+                    f.pk[q][cc] = p2;
+                    if (a.materialize) f.p[q][cc] = p1;
+                }
+            } else if (a.mode == XI_LF) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double pq = f.p[q][cc] - c.dt * F[q];             // leapfrog.F:134-139
+                    if (counted) ke1 += pq * pq * minv;
+                    f.p[q][cc] = pq;
+                }
+            } else if (a.mode == XI_VS) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double pq = counted ? f.p[q][cc] - c.dt * F[q] : 0.0;   // velocity_scaling.F:62-76
+                    if (counted) ke1 += pq * pq * minv;
+                    f.p[q][cc] = pq;
+                }
+            }
+            if (a.materialize || a.mode == XI_FORCE) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) f.F[q][cc] = F[q];
+            }
+            if constexpr (NF == 12) {
+                const double mwinv = c.mwinv[s];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double g = acc[i][9 + q] * c.Ninv;                  // dVdac.F:56-62
+                    if (a.mode == XI_LF || a.mode == XI_VS) {
+                        const double pq = f.pw[q][cc] - c.dt * g;
+                        kea += pq * pq * mwinv;
+                        f.pw[q][cc] = pq;
+                    }
+                    if (a.materialize || a.mode == XI_FORCE) f.dVdw[q][cc] = g;
+                }
             }
         }
     }
@@ -781,12 +897,24 @@ __global__ __launch_bounds__(NT) void k_kick(Consts c, Fields f, const Scalars* 
     if (threadIdx.x == 0) partials[P_KEA * maxblocks + blockIdx.x] = r;
 }
 
-// Deterministic fold of nb per-block partials of one slot by one workgroup.
-__device__ __forceinline__ double fold(const double* part, int nb, double* red)
+// Deterministic fold of the nb per-block partials of NS slots by one
+// workgroup: all loads are issued first, then one tree per slot.
+template <int NS>
+__device__ __forceinline__ void fold_slots(const double* part, int maxblocks, int nb, const int (&slots)[NS],
+                                           double (&out)[NS], double* red)
 {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += NT) s += part[i];
-    return block_sum(s, red);
+    double s[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) s[q] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += NT) {
+        double v[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) v[q] = part[slots[q] * maxblocks + i];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) s[q] += v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NS; ++q) out[q] = block_sum(s[q], red);
 }
 
 // Sum of the nine potential energies (energy_module.F:73-87).
@@ -817,6 +945,25 @@ __device__ __forceinline__ double np_hamiltonian(const ScalarConsts& k, const Sc
     return H * s;
 }
 
+// First half of a Nose-Poincare step up to the drift factor
+// (nose-poincare.F:30-41); en[E_DIPO_KIN] <- KE after the first half kick.
+__device__ __forceinline__ void np_begin_scalar(const ScalarConsts& k, Scalars* sc)
+{
+#pragma clang fp contract(off)
+    double* en = sc->en;
+    en[E_DIPO_KIN] = sc->ke_pk;
+    double s = sc->s;
+    double CB = k.n_freedom * k.T_eV * (1.0 + log(s)) - en[E_DIPO_KIN] / (s * s) + potential(en) - sc->H0;
+    CB = k.dt_2 * CB - sc->pi;
+    const double pi = -2 * CB / (1.0 + sqrt(1.0 - CB * k.dt / k.Q));
+    const double s_old = s;
+    s = s * (1.0 + k.dt_2 * pi / k.Q) / (1.0 - k.dt_2 * pi / k.Q);
+    sc->pi = pi;
+    sc->s = s;
+    sc->s_old = s_old;
+    sc->c_drift = k.dt_2 * (1 / s + 1 / s_old);
+}
+
 // Energies of the two z-pass groups, homogeneous strain and its coupling
 // (optimize-homo-strain.F:27-85, dVddi-homo-coupling.F:21-26).
 __global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* sc, const double* partials,
@@ -824,18 +971,16 @@ __global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* sc, const 
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
-    const double esr = fold(partials + P_ESR * maxblocks, zblocks, red);
-    const double elr = fold(partials + P_ELR * maxblocks, zblocks, red);
-    const double eis = fold(partials + P_EIS * maxblocks, zblocks, red);
-    const double eic = fold(partials + P_EIC * maxblocks, zblocks, red);
-    const double eim = fold(partials + P_EIM * maxblocks, zblocks, red);
+    const int slots[5] = {P_ESR, P_ELR, P_EIS, P_EIC, P_EIM};
+    double r[5];
+    fold_slots<5>(partials, maxblocks, zblocks, slots, r, red);
     if (threadIdx.x != 0) return;
     double* en = sc->en;
-    en[E_SR] = esr * k.Ninv / 2;
-    en[E_LR] = elr * k.Ninv / 2;
-    en[E_IS] = eis * k.Ninv;
-    en[E_IC] = eic * 2 * k.Ninv;
-    en[E_IM] = k.modulation_energy ? eim * 2 * k.Ninv : 0.0;
+    en[E_SR] = r[0] * k.Ninv / 2;
+    en[E_LR] = r[1] * k.Ninv / 2;
+    en[E_IS] = r[2] * k.Ninv;
+    en[E_IC] = r[3] * 2 * k.Ninv;
+    en[E_IM] = k.modulation_energy ? r[4] * 2 * k.Ninv : 0.0;
 
     double c2uu[6];
     for (int i = 0; i < 6; ++i) {
@@ -896,18 +1041,20 @@ __global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* sc, const 
 enum FinishMode { FIN_FORCE = 0, FIN_NP = 1, FIN_LF = 2, FIN_VS = 3 };
 
 // Local energies and kinetic energies of the xinv pass, then the scalar part
-// of the integrator that follows the force evaluation.
+// of the integrator that follows the force evaluation.  With FIN_NP and
+// next_step set, the first half of the next Nose-Poincare step (eq. 38 for
+// pi and s, the drift factor) follows in the same launch.
 __global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* sc, const double* partials,
-                                               int maxblocks, int xblocks, int mode, int acoustic)
+                                               int maxblocks, int xblocks, int mode, int acoustic,
+                                               int next_step)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
-    const double eunh = fold(partials + P_EUNH * maxblocks, xblocks, red);
-    const double ede = fold(partials + P_EDE * maxblocks, xblocks, red);
-    const double ke1 = fold(partials + P_KE1 * maxblocks, xblocks, red);
-    const double ke2 = fold(partials + P_KE2 * maxblocks, xblocks, red);
-    const double kea = fold(partials + P_KEA * maxblocks, xblocks, red);
+    const int slots[5] = {P_EUNH, P_EDE, P_KE1, P_KE2, P_KEA};
+    double r[5];
+    fold_slots<5>(partials, maxblocks, xblocks, slots, r, red);
     if (threadIdx.x != 0) return;
+    const double eunh = r[0], ede = r[1], ke1 = r[2], ke2 = r[3], kea = r[4];
     double* en = sc->en;
     en[E_UNH] = eunh;
     en[E_DE] = -ede;
@@ -934,31 +1081,22 @@ __global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* sc, cons
         }
     }
     en[E_TOTAL] = en[E_DIPO_KIN] / (sc->s * sc->s) + en[E_ACOU_KIN] + potential(en);
+    if (mode == FIN_NP && next_step) np_begin_scalar(k, sc);
 }
 
-// First half of a Nose-Poincare step up to the drift factor
-// (nose-poincare.F:30-41).  fold_ke: take KE from the kick kernel's partials.
+// First half of the first Nose-Poincare step of a call: KE from the kick
+// kernel's partials, then eq. 38.
 __global__ __launch_bounds__(NT) void k_np_begin(ScalarConsts k, Scalars* sc, const double* partials,
-                                                 int maxblocks, int kblocks, int fold_ke)
+                                                 int maxblocks, int kblocks)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
-    double ke = 0.0;
-    if (fold_ke) ke = fold(partials + P_KEPK * maxblocks, kblocks, red);
+    const int slots[1] = {P_KEPK};
+    double r[1];
+    fold_slots<1>(partials, maxblocks, kblocks, slots, r, red);
     if (threadIdx.x != 0) return;
-    if (fold_ke) sc->ke_pk = ke / (double)k.n_sum_p * (double)k.N / 2;
-    double* en = sc->en;
-    en[E_DIPO_KIN] = sc->ke_pk;
-    double s = sc->s;
-    double CB = k.n_freedom * k.T_eV * (1.0 + log(s)) - en[E_DIPO_KIN] / (s * s) + potential(en) - sc->H0;
-    CB = k.dt_2 * CB - sc->pi;
-    const double pi = -2 * CB / (1.0 + sqrt(1.0 - CB * k.dt / k.Q));
-    const double s_old = s;
-    s = s * (1.0 + k.dt_2 * pi / k.Q) / (1.0 - k.dt_2 * pi / k.Q);
-    sc->pi = pi;
-    sc->s = s;
-    sc->s_old = s_old;
-    sc->c_drift = k.dt_2 * (1 / s + 1 / s_old);
+    sc->ke_pk = r[0] / (double)k.n_sum_p * (double)k.N / 2;
+    np_begin_scalar(k, sc);
 }
 
 // Kinetic energies of the current momenta for the energy report
@@ -968,11 +1106,12 @@ __global__ __launch_bounds__(NT) void k_ke_report(ScalarConsts k, Scalars* sc, c
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
-    const double ke = fold(partials + P_KEPK * maxblocks, kblocks, red);
-    const double kea = fold(partials + P_KEA * maxblocks, kblocks, red);
+    const int slots[2] = {P_KEPK, P_KEA};
+    double r[2];
+    fold_slots<2>(partials, maxblocks, kblocks, slots, r, red);
     if (threadIdx.x != 0) return;
-    sc->en[E_DIPO_KIN] = ke / (double)k.n_sum_p * (double)k.N / 2;
-    sc->en[E_ACOU_KIN] = kea / 2;
+    sc->en[E_DIPO_KIN] = r[0] / (double)k.n_sum_p * (double)k.N / 2;
+    sc->en[E_ACOU_KIN] = r[1] / 2;
 }
 
 // Apply the pending velocity-scaling factors so the host sees scaled momenta.
@@ -1054,13 +1193,27 @@ __global__ __launch_bounds__(NT) void k_ewald_real(Consts c, EwaldArgs e, double
     for (int m = 0; m < 6; ++m) out[(size_t)m * c.N + cc] = T[m];
 }
 
-// L(k) = Re FFT(real part) + reciprocal sum, times 14.399644/eps_inf.
-__global__ __launch_bounds__(NT) void k_ewald_recip(Consts c, EwaldArgs e, Spectra<double> sp, double* Ltab)
+// L(k) = Re FFT(real part) + reciprocal sum, times 14.399644/eps_inf; stored
+// as 6 words per point, in FP64 and in the spectral type.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_ewald_recip(Consts c, EwaldArgs e, Spectra<double> sp, double* Ltab,
+                                                    T* Lt)
 {
 #pragma clang fp contract(off)
     const size_t kk = (size_t)blockIdx.x * NT + threadIdx.x;
     if (kk >= (size_t)c.Kh) return;
-    const int ix = (int)(kk % c.Kx), iy = (int)((kk / c.Kx) % c.Ly), iz = (int)(kk / ((size_t)c.Kx * c.Ly));
+    int ix, iy, iz;
+    if (kk < c.Mh) {
+        ix = (int)(kk % c.Mx);
+        const size_t r = kk / c.Mx;
+        iy = (int)(r % c.Ly);
+        iz = (int)(r / c.Ly);
+    } else {
+        const size_t r = kk - c.Mh;
+        ix = c.Mx;
+        iy = (int)(r % c.Ly);
+        iz = (int)(r / c.Ly);
+    }
     double k3 = (double)iz / c.Lz; if (k3 > 0.5) k3 = k3 - 1.0;
     double k2 = (double)iy / c.Ly; if (k2 > 0.5) k2 = k2 - 1.0;
     const double k1 = (double)ix / c.Lx;
@@ -1080,7 +1233,21 @@ __global__ __launch_bounds__(NT) void k_ewald_recip(Consts c, EwaldArgs e, Spect
                 }
             }
         }
-        Ltab[(size_t)m * c.Kh + kk] = acc * e.scale;
+        Ltab[kk * 6 + m] = acc * e.scale;
+        if (Lt) Lt[kk * 6 + m] = (T)(acc * e.scale);
+    }
+}
+
+// w reconstructed from the optimised w(k): slots 0..2 <- -i acouK
+// (coord_module.F:457-469).
+template <typename T>
+__global__ void k_prep_w(Consts c, Spectra<T> sp, const cpx<double>* acouK)
+{
+    const size_t kk = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (kk >= (size_t)c.Kh) return;
+    for (int q = 0; q < 3; ++q) {
+        const cpx<double> a = acouK[(size_t)q * c.Kh + kk];
+        sp.s[q][kk] = cpx<T>{(T)a.y, (T)(-a.x)};
     }
 }
 
