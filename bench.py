@@ -1,21 +1,26 @@
-"""NP-MD throughput of the per-step path on one MI355X.
+"""NP-MD throughput of the per-step path on one MI355X, FP32 mode vs FP64 plain mode.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--precision both|fp32|fp64]
 
 Workload: BaTiO3 bulk 128^3 supercell (2,097,152 cells), the bulk
 phase-transition parameter set, Nose-Poincare MD ('md') at 300 K, dt = 2 fs,
-acoustic displacement optimised each step, FP64 plain mode.  Initial u is
-uniform in [-0.05, 0.05] A per component from a fixed seed, p = 0.  State and
+acoustic displacement optimised each step.  Initial u is uniform in
+[-0.05, 0.05] A per component from a fixed seed, p = 0.  Q_Nose grows with N
+(1.0 at 64^3 as the reference's 64^3 md benchmark, scaled by N).  State and
 tables are resident in HBM before the timed region; K steps run back to back
-on the library's stream and are bracketed by device synchronisation.
+on the library's stream, bracketed by device synchronisation.
 
-One JSON line is printed.  ``roofline`` is the dominant kernel's algorithmic
-HBM bytes per launch over its average launch time, taken with HIP events on
-the library's stream in a second pass of the same K steps right after the
-timed one (events bracket every launch there, so that pass is not the one
-``value`` comes from).  ``cpu_baseline`` times the CPU FP64 oracle (the
-restatement of the reference in C, OpenMP over all host threads) on a bounded
-sample of the same workload.
+Two modes are timed on the same workload: FP64 plain (everything FP64) and
+FP32 (spectra, FFTs and the k-space stage in FP32; state, local forces,
+thermostat, energy sums and homogeneous strain in FP64).  ``value`` is the
+FP32-mode rate; the FP64 rate, their ratio and the FP32 force/energy error
+against FP64 on the initial configuration are in ``modes``.
+
+``roofline`` is the dominant kernel's algorithmic HBM bytes per launch over
+its average launch time, taken with HIP events on the library's stream in a
+second pass of the same K steps right after the timed one.  ``cpu_baseline``
+times the CPU FP64 oracle (C restatement of the reference, OpenMP over the
+host threads) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -31,26 +36,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+ENERGY_TERMS = ("short_range", "long_range", "dipole_E_field", "unharmonic", "homo_strain",
+                "homo_coupling", "inho_strain", "inho_coupling", "inho_modulation")
 
 
-def kernel_bytes(name: str, N: int, Kh: int, nf: int, materialize_frac: float) -> float:
-    """Algorithmic HBM bytes of one launch (compulsory reads + writes), FP64."""
+def kernel_bytes(name: str, N: int, Kh: int, nf: int, ws: int, materialize_frac: float) -> float:
+    """Algorithmic HBM bytes of one launch (compulsory reads + writes).
+
+    ws: bytes of one spectral word (8 FP64, 4 FP32); a complex point is 2 ws;
+    state fields are FP64 in both modes."""
+    cw = 2 * ws
     if name == "xfwd_drift_r2c":      # read u, pk, code; write u, nf half spectra
-        return 3 * 8 * N * 3 + N + nf * 16 * Kh
+        return 3 * 8 * N * 3 + N + nf * cw * Kh
     if name in ("y_forward", "y_inverse"):
-        return 2 * nf * 16 * Kh
+        return 2 * nf * cw * Kh
     if name == "z_dipole_dipole":     # 3 spectra in/out + 6-word L(k) table
-        return 2 * 3 * 16 * Kh + 6 * 8 * Kh
+        return 2 * 3 * cw * Kh + 6 * ws * Kh
     if name == "z_strain":            # 6 spectra in/out
-        return 2 * (nf - 3) * 16 * Kh
+        return 2 * (nf - 3) * cw * Kh
     if name == "xinv_c2r_local_kick":  # read nf spectra, u, pk, code; write pk (+p, F when materialised)
-        return nf * 16 * Kh + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
+        return nf * cw * Kh + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
     return 0.0
 
 
-def step_bytes(N: int, Kh: int) -> float:
-    """Compulsory bytes of a 3-sweep fused md step (18 N w_s + 78 Kh w_f, FP64)."""
-    return 18 * N * 8 + 78 * Kh * 8
+def step_bytes(N: int, Kh: int, ws: int) -> float:
+    """Compulsory bytes of a 3-sweep fused md step: 18 N w_s + 78 Kh w_f (w_s = 8)."""
+    return 18 * N * 8 + 78 * Kh * ws
 
 
 def cpu_baseline(p, u0, steps: int) -> dict:
@@ -69,12 +80,59 @@ def cpu_baseline(p, u0, steps: int) -> dict:
                       f"(OpenMP, {threads} threads), after its untimed table set-up"}
 
 
+def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
+    from feram_amd.system import System
+    g = System(p, precision=precision)
+    g.put("dipoR", u0)
+    e0 = g.md_prologue()
+    F0 = g.get("dVddi")
+    if warmup > 0:
+        g.nose_poincare(warmup)
+
+    def sync():
+        g.synchronize()
+        sync_extra()
+
+    sync()
+    t0 = time.perf_counter()
+    e = g.nose_poincare(steps)
+    sync()
+    dt = time.perf_counter() - t0
+    if not all(np.isfinite(v) for v in e.values()):
+        raise RuntimeError(f"non-finite energies after the timed steps: {e}")
+    g.profile(True)
+    g.nose_poincare(steps)
+    sync()
+    stats = g.kernel_stats()
+    g.profile(False)
+    L = p.Lx
+    N, Kh = p.N, (L // 2 + 1) * L * L
+    ws = 4 if precision == 1 else 8
+    per = {}
+    for name, (ms, cnt) in stats.items():
+        avg_ms = ms / cnt
+        b = kernel_bytes(name, N, Kh, 9, ws, 1.0 / steps)
+        per[name] = {"avg_us": avg_ms * 1e3, "launches": cnt,
+                     "GBs": (b / (avg_ms * 1e-3) / 1e9) if b else None, "bytes": b}
+    dom = max((k for k in per if per[k]["bytes"]), key=lambda k: per[k]["avg_us"] * per[k]["launches"])
+    sb = step_bytes(N, Kh, ws)
+    res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+           "kernel_sum_ms_per_step": sum(ms for ms, cnt in stats.values()) / steps,
+           "dominant": dom, "kernels": per,
+           "step_roofline": {"algorithmic_bytes_per_step": sb, "achieved_GBs": sb * steps / dt / 1e9,
+                             "frac": sb * steps / dt / 1e9 / HBM_PEAK_GBS},
+           "device_bytes": g.device_bytes, "total_energy_eV_per_uc": e["total"] / N}
+    g.close()
+    return res, e0, F0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--precision", choices=("both", "fp32", "fp64"), default="both")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
@@ -98,7 +156,6 @@ def main():
         sys.exit(2)
 
     from feram_amd.params import batio3_zvr
-    from feram_amd.system import System
 
     L = args.size
     p = batio3_zvr(L, method="md", kelvin=300.0)
@@ -109,92 +166,73 @@ def main():
     rng = np.random.default_rng(20241203)
     u0 = rng.uniform(-0.05, 0.05, size=(L, L, L, 3))
 
-    print(f"[bench] creating {L}^3 context", file=sys.stderr, flush=True)
-    g = System(p)
-    g.put("dipoR", u0)
-    g.md_prologue()
-    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
-    if args.warmup > 0:
-        g.nose_poincare(args.warmup)
     try:
         import torch
         have_torch = torch.cuda.is_available()
     except Exception:
         have_torch = False
 
-    def sync():
-        g.synchronize()
+    def sync_extra():
         if have_torch:
             torch.cuda.synchronize()
 
-    sync()
-    t0 = time.perf_counter()
-    e = g.nose_poincare(args.steps)
-    sync()
-    dt = time.perf_counter() - t0
-    steps_per_s = args.steps / dt
-    print(f"[bench] timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
-    if not all(np.isfinite(v) for v in e.values()):
-        print(f"[bench] non-finite energies after the timed steps: {e}", file=sys.stderr)
-        sys.exit(3)
+    modes = {}
+    refs = {}
+    for name, prec in (("fp64", 0), ("fp32", 1)):
+        if args.precision not in ("both", name):
+            continue
+        print(f"[bench] {L}^3 {name}: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+        res, e0, F0 = measure(p, u0, prec, args.steps, args.warmup, sync_extra)
+        print(f"[bench] {name}: {res['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
+        modes[name] = res
+        refs[name] = (e0, F0)
+    if "fp64" in refs and "fp32" in refs:
+        (e64, F64), (e32, F32) = refs["fp64"], refs["fp32"]
+        scale = sum(abs(e64[t]) for t in ENERGY_TERMS)
+        modes["fp32"]["force_rel_err_vs_fp64"] = float(np.max(np.abs(F32 - F64)) / np.max(np.abs(F64)))
+        modes["fp32"]["energy_rel_err_vs_fp64"] = float(
+            max(abs(e32[t] - e64[t]) for t in ENERGY_TERMS) / scale)
+        modes["fp32"]["speedup_vs_fp64"] = modes["fp32"]["steps_per_s"] / modes["fp64"]["steps_per_s"]
 
-    # per-kernel timing pass (events around every launch)
-    g.profile(True)
-    g.nose_poincare(args.steps)
-    sync()
-    stats = g.kernel_stats()
-    g.profile(False)
-    N = p.N
-    Kh = (L // 2 + 1) * L * L
-    nf = 9
-    per = {}
-    for name, (ms, cnt) in stats.items():
-        avg_ms = ms / cnt
-        b = kernel_bytes(name, N, Kh, nf, 1.0 / args.steps)
-        per[name] = {"avg_us": avg_ms * 1e3, "launches": cnt,
-                     "GBs": (b / (avg_ms * 1e-3) / 1e9) if b else None, "bytes": b}
-    step_kernel_ms = sum(ms for ms, cnt in stats.values()) / args.steps
-    dom = max((k for k in per if per[k]["bytes"]), key=lambda k: per[k]["avg_us"] * per[k]["launches"])
-    achieved = per[dom]["GBs"]
+    head = modes["fp32"] if "fp32" in modes else modes["fp64"]
+    head_name = "fp32" if "fp32" in modes else "fp64"
+    dom = head["dominant"]
+    kd = head["kernels"][dom]
+    achieved = kd["GBs"]
     traffic = None
     tr_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tr_path):
         with open(tr_path) as f:
-            traffic = json.load(f).get(f"{L}:{dom}")
+            traffic = json.load(f).get(f"{L}:{head_name}:{dom}")
 
     cpu = None
     if not args.no_cpu:
         print(f"[bench] cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(p, u0, args.cpu_steps)
 
-    sb = step_bytes(N, Kh)
+    N = p.N
     out = {
-        "metric": "NP-MD steps/s at 128^3 (FP64 plain mode)",
-        "value": steps_per_s,
+        "metric": "NP-MD steps/s at 128^3 (FP32 mode; FP64 plain mode alongside)",
+        "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if head_name == "fp32" else "f64",
         "data": "synthetic",
         "config": {"workload": f"BaTiO3 bulk {L}x{L}x{L} NP-MD ('md'), 300 K, dt=2 fs, optimised "
-                               "acoustic strain, FP64 plain, u0 ~ U[-0.05,0.05] A seeded",
-                   "L": L, "cells": N, "method": "md", "precision": "fp64", "Q_Nose": p.Q_Nose,
-                   "cell_updates_per_s": steps_per_s * N,
-                   "total_energy_eV_per_uc": e["total"] / N},
+                               "acoustic strain, u0 ~ U[-0.05,0.05] A seeded",
+                   "L": L, "cells": N, "method": "md", "precision": head_name, "Q_Nose": p.Q_Nose,
+                   "cell_updates_per_s": head["steps_per_s"] * N},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": traffic, "bytes_per_launch": per[dom]["bytes"],
-                     "avg_launch_us": per[dom]["avg_us"]},
-        "step_roofline": {"algorithmic_bytes_per_step": sb,
-                          "achieved_GBs": sb * steps_per_s / 1e9,
-                          "frac": sb * steps_per_s / 1e9 / HBM_PEAK_GBS,
-                          "kernel_sum_ms_per_step": step_kernel_ms},
-        "kernels": per,
+                     "traffic": traffic, "bytes_per_launch": kd["bytes"], "avg_launch_us": kd["avg_us"]},
+        "step_roofline": head["step_roofline"],
+        "modes": modes,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

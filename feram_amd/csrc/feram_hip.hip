@@ -34,12 +34,16 @@ bool size_supported(int n)
     }
 }
 
-int tile_rows(int n)   // must agree with tile_of<n>::value
+int tile_rows(int n, int cap = 1024)   // must agree with tile_of<n, cap>::value
 {
     int p = 1;
-    while (2 * p * n <= 1024) p *= 2;
+    while (2 * p * n <= cap) p *= 2;
     return p;
 }
+
+// Lines per y/z tile of spectral type T (line_tile_cap<T>() on the device).
+template <typename T>
+int line_tile_rows(int n) { return tile_rows(n, 16384 / (int)sizeof(cpx<T>)); }
 
 enum KernelId {
     K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
@@ -51,23 +55,33 @@ const char *kKernelNames[K_NKERNEL] = {
 
 }  // namespace
 
+template <typename T>
+struct TypedSet {
+    Spectra<T> sp{};
+    cpx<T> *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    T *Lt = nullptr;                  // L(k) in the spectral type, 6 words per point
+    cpx<T> *modk = nullptr;           // FFT of the modulation field
+};
+
 struct fhip_ctx {
     fhip_params p{};
     int Lx = 0, Ly = 0, Lz = 0, Kx = 0, Mx = 0, N = 0, Kh = 0, nslots = 0;
     size_t Mh = 0;
-    int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0, maxblocks = 0;
+    int xblocks = 0, kblocks = 0, maxblocks = 0;
+    int yblocks[2] = {0, 0}, zblocks[2] = {0, 0};   // per spectral type: [0] FP64, [1] FP32
     hipStream_t stream = nullptr;
     std::string err;
     Consts c{};
     ScalarConsts sk{};
     Fields f{};
-    Spectra<double> sp{};
+    int precision = FHIP_FP64;
+    TypedSet<double> dset;            // FP64 spectra (plain mode) and the FP64 set-up
+    TypedSet<float> fset;             // FP32 spectra (FP32 mode)
     Scalars *dsc = nullptr;
     Scalars *hsc = nullptr;           // pinned mirror
     double *partials = nullptr;
-    double *Ltab = nullptr;
-    cpx<double> *modk = nullptr, *acouK = nullptr;
-    cpx<double> *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    double *Ltab = nullptr;           // FP64 L(k), 6 words per point (master copy)
+    cpx<double> *acouK = nullptr;
     cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
     unsigned char *code = nullptr;
     std::vector<void *> allocs;
@@ -83,6 +97,11 @@ struct fhip_ctx {
 };
 
 namespace {
+
+template <typename T> constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
+template <typename T> TypedSet<T> &tset(fhip_ctx *ctx);
+template <> TypedSet<double> &tset<double>(fhip_ctx *ctx) { return ctx->dset; }
+template <> TypedSet<float> &tset<float>(fhip_ctx *ctx) { return ctx->fset; }
 
 int fail(fhip_ctx *ctx, int code, const std::string &msg)
 {
@@ -170,96 +189,100 @@ int collect_events(fhip_ctx *ctx)
     } while (0)
 
 // ---- size dispatch ---------------------------------------------------------
+template <typename T>
 int run_xfwd(fhip_ctx *ctx, const XfwdArgs &a)
 {
     Consts c = ctx->c;
     Fields f = ctx->f;
-    Spectra<double> sp = ctx->sp;
+    Spectra<T> sp = tset<T>(ctx).sp;
     const Scalars *sc = ctx->dsc;
-    const cpx<double> *tw = ctx->twx;
+    const cpx<T> *tw = tset<T>(ctx).twx;
     const int nb = ctx->xblocks;
     hipStream_t s = ctx->stream;
     switch (ctx->Lx) {
 #define FHIP_CASE(n)                                                                         \
     case n:                                                                                  \
-        return launch(ctx, K_XFWD, [&] { k_xfwd<double, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+        return launch(ctx, K_XFWD, [&] { k_xfwd<T, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx not supported");
 }
 
+template <typename T>
 int run_xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
 {
     Consts c = ctx->c;
     Fields f = ctx->f;
-    Spectra<double> sp = ctx->sp;
+    Spectra<T> sp = tset<T>(ctx).sp;
     const Scalars *sc = ctx->dsc;
-    const cpx<double> *tw = ctx->twx;
+    const cpx<T> *tw = tset<T>(ctx).twx;
     const int nb = ctx->xblocks;
     hipStream_t s = ctx->stream;
     switch (ctx->Lx) {
 #define FHIP_CASE(n)                                                                         \
     case n:                                                                                  \
         if (nf == 12)                                                                        \
-            return launch(ctx, K_XINV, [&] { k_xinv<double, n, 12><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
+            return launch(ctx, K_XINV, [&] { k_xinv<T, n, 12><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
         if (nf == 9)                                                                         \
-            return launch(ctx, K_XINV, [&] { k_xinv<double, n, 9><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
-        return launch(ctx, K_XINV, [&] { k_xinv<double, n, 3><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+            return launch(ctx, K_XINV, [&] { k_xinv<T, n, 9><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
+        return launch(ctx, K_XINV, [&] { k_xinv<T, n, 3><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx not supported");
 }
 
+template <typename T>
 int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf)
 {
     Consts c = ctx->c;
-    Spectra<double> sp = ctx->sp;
-    const cpx<double> *tw = ctx->twy;
-    dim3 grid(ctx->yblocks, nf);
+    Spectra<T> sp = tset<T>(ctx).sp;
+    const cpx<T> *tw = tset<T>(ctx).twy;
+    dim3 grid(ctx->yblocks[tix<T>()], nf);
     hipStream_t s = ctx->stream;
     const int id = sign < 0 ? K_YFWD : K_YINV;
     switch (ctx->Ly) {
 #define FHIP_CASE(n)                                                                                   \
     case n:                                                                                            \
         if (sign < 0)                                                                                  \
-            return launch(ctx, id, [&] { k_ypass<double, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }); \
-        return launch(ctx, id, [&] { k_ypass<double, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); });
+            return launch(ctx, id, [&] { k_ypass<T, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }); \
+        return launch(ctx, id, [&] { k_ypass<T, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); });
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Ly not supported");
 }
 
-template <int G>
-int run_zpass_g(fhip_ctx *ctx, const ZArgs<double> &a)
+template <typename T, int G>
+int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a)
 {
     Consts c = ctx->c;
-    Spectra<double> sp = ctx->sp;
+    Spectra<T> sp = tset<T>(ctx).sp;
     Scalars *sc = ctx->dsc;
-    const cpx<double> *tw = ctx->twz;
-    const int nb = ctx->zblocks;
+    const cpx<T> *tw = tset<T>(ctx).twz;
+    const int nb = ctx->zblocks[tix<T>()];
     hipStream_t s = ctx->stream;
     const int id = (G == ZG_DD) ? K_ZDD : (G == ZG_PLAIN ? K_SETUP : K_ZSTRAIN);
     switch (ctx->Lz) {
 #define FHIP_CASE(n)                                                                           \
     case n:                                                                                    \
-        return launch(ctx, id, [&] { k_zpass<double, n, G><<<nb, NT, 0, s>>>(c, sp, a, sc, tw); });
+        return launch(ctx, id, [&] { k_zpass<T, n, G><<<nb, NT, 0, s>>>(c, sp, a, sc, tw); });
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lz not supported");
 }
 
-ZArgs<double> zargs(fhip_ctx *ctx, bool want_acouK)
+template <typename T>
+ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
 {
-    ZArgs<double> a{};
-    a.Ltab = ctx->Ltab;
+    ZArgs<T> a{};
+    a.Ltab = tset<T>(ctx).Lt;
     a.csx = ctx->csx;
     a.csy = ctx->csy;
     a.csz = ctx->csz;
-    a.modk = ctx->modk;
+    a.modk = tset<T>(ctx).modk;
     a.acouK = want_acouK ? ctx->acouK : nullptr;
     a.partials = ctx->partials;
     a.maxblocks = ctx->maxblocks;
@@ -272,34 +295,35 @@ int nforward(const fhip_ctx *ctx) { return ctx->c.acoustic ? 12 : 9; }
 
 enum StepKind { SK_FORCE = 0, SK_NP = 1, SK_LF = 2, SK_VS = 3 };
 
-int force_pipeline(fhip_ctx *ctx, int kind, bool materialize, int next_step = 0)
+template <typename T>
+int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
 {
     const int nf = nforward(ctx);
     XfwdArgs xa{};
     xa.mode = (kind == SK_NP) ? XF_NP : (kind == SK_LF ? XF_LF : (kind == SK_VS ? XF_VS : XF_FORCE));
     xa.nf = nf;
-    RC(run_xfwd(ctx, xa));
-    RC(run_ypass(ctx, -1, 0, nf));
-    ZArgs<double> za = zargs(ctx, materialize && !ctx->c.acoustic);
-    RC(run_zpass_g<ZG_DD>(ctx, za));
-    if (ctx->c.acoustic) RC(run_zpass_g<ZG_ACOU>(ctx, za));
-    else RC(run_zpass_g<ZG_STRAIN>(ctx, za));
+    RC(run_xfwd<T>(ctx, xa));
+    RC(run_ypass<T>(ctx, -1, 0, nf));
+    ZArgs<T> za = zargs<T>(ctx, materialize && !ctx->c.acoustic);
+    RC((run_zpass_g<T, ZG_DD>(ctx, za)));
+    if (ctx->c.acoustic) RC((run_zpass_g<T, ZG_ACOU>(ctx, za)));
+    else RC((run_zpass_g<T, ZG_STRAIN>(ctx, za)));
     if (materialize && !ctx->c.acoustic) ctx->acouK_valid = true;
     {
         ScalarConsts k = ctx->sk;
         Scalars *sc = ctx->dsc;
         double *part = ctx->partials;
-        int mb = ctx->maxblocks, zb = ctx->zblocks;
+        int mb = ctx->maxblocks, zb = ctx->zblocks[tix<T>()];
         hipStream_t s = ctx->stream;
         RC(launch(ctx, K_HOMO, [&] { k_homo<<<1, NT, 0, s>>>(k, sc, part, mb, zb); }));
     }
-    RC(run_ypass(ctx, +1, 0, nf));
+    RC(run_ypass<T>(ctx, +1, 0, nf));
     XinvArgs ia{};
     ia.mode = (kind == SK_NP) ? XI_NP : (kind == SK_LF ? XI_LF : (kind == SK_VS ? XI_VS : XI_FORCE));
     ia.materialize = materialize ? 1 : 0;
     ia.partials = ctx->partials;
     ia.maxblocks = ctx->maxblocks;
-    RC(run_xinv(ctx, ia, nf));
+    RC(run_xinv<T>(ctx, ia, nf));
     {
         ScalarConsts k = ctx->sk;
         Scalars *sc = ctx->dsc;
@@ -311,6 +335,12 @@ int force_pipeline(fhip_ctx *ctx, int kind, bool materialize, int next_step = 0)
         RC(launch(ctx, K_FINISH, [&] { k_finish<<<1, NT, 0, s>>>(k, sc, part, mb, xb, mode, ac, next_step); }));
     }
     return FHIP_OK;
+}
+
+int force_pipeline(fhip_ctx *ctx, int kind, bool materialize, int next_step = 0)
+{
+    if (ctx->precision == FHIP_FP32) return force_pipeline_t<float>(ctx, kind, materialize, next_step);
+    return force_pipeline_t<double>(ctx, kind, materialize, next_step);
 }
 
 int pull_scalars(fhip_ctx *ctx)
@@ -386,7 +416,10 @@ void fill_consts(fhip_ctx *ctx)
     c.Lx = p.Lx; c.Ly = p.Ly; c.Lz = p.Lz;
     c.Mx = ctx->Mx; c.Mh = ctx->Mh; c.N = ctx->N; c.Kh = ctx->Kh;
     {
-        const int ty = tile_rows(ctx->Ly), tz = tile_rows(ctx->Lz);
+        // whole tiles of the pipeline's spectral type; the FP32 tile is a
+        // multiple of the FP64 one, so FP64 set-up transforms share the layout
+        const int ty = p.precision == FHIP_FP32 ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
+        const int tz = p.precision == FHIP_FP32 ? line_tile_rows<float>(ctx->Lz) : line_tile_rows<double>(ctx->Lz);
         c.ylines_main = (ctx->Mx * ctx->Lz + ty - 1) / ty * ty;
         c.zlines_main = (ctx->Mx * ctx->Ly + tz - 1) / tz * tz;
     }
@@ -468,33 +501,59 @@ void fill_consts(fhip_ctx *ctx)
     for (int i = 0; i < 36; ++i) k.hc2[i] = c.hc2[i];
 }
 
-int upload_table(fhip_ctx *ctx, cpx<double> **dst, int n, bool twiddle)
+template <typename T>
+int upload_table(fhip_ctx *ctx, cpx<T> **dst, int n, bool twiddle)
 {
-    std::vector<cpx<double>> h(n);
+    std::vector<cpx<T>> h(n);
     for (int j = 0; j < n; ++j) {
         const double a = 2.0 * M_PI_D * (double)j / (double)n;
-        h[j] = twiddle ? cpx<double>{std::cos(a), -std::sin(a)} : cpx<double>{std::cos(a), std::sin(a)};
+        const double cs = std::cos(a), sn = twiddle ? -std::sin(a) : std::sin(a);
+        h[j] = cpx<T>{(T)cs, (T)sn};
     }
     RC(dalloc(ctx, dst, n));
-    HIPCHK(hipMemcpyAsync(*dst, h.data(), sizeof(cpx<double>) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(*dst, h.data(), sizeof(cpx<T>) * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FHIP_OK;
 }
 
-// Forward 3-D r2c of nf real device fields into spectral slots 0..nf-1.
+// Forward 3-D r2c, in FP64, of nf real device fields into the FP64 spectral
+// slots 0..nf-1 (set-up transforms run in FP64 in every precision mode).
 int forward_generic(fhip_ctx *ctx, const double *const *fields, int nf)
 {
     XfwdArgs xa{};
     xa.mode = XF_GENERIC;
     xa.nf = nf;
     for (int i = 0; i < nf; ++i) xa.gen[i] = fields[i];
-    RC(run_xfwd(ctx, xa));
-    RC(run_ypass(ctx, -1, 0, nf));
-    ZArgs<double> za = zargs(ctx, false);
+    RC(run_xfwd<double>(ctx, xa));
+    RC(run_ypass<double>(ctx, -1, 0, nf));
+    ZArgs<double> za = zargs<double>(ctx, false);
     za.nplain = nf;
     za.plain_sign = -1;
-    return run_zpass_g<ZG_PLAIN>(ctx, za);
+    return run_zpass_g<double, ZG_PLAIN>(ctx, za);
 }
+
+// FP64 spectral slots for set-up work in FP32 mode, released afterwards.
+struct SetupSlots {
+    fhip_ctx *ctx;
+    void *raw = nullptr;
+    int open(int n)
+    {
+        if (ctx->precision == FHIP_FP64) return FHIP_OK;
+        hipError_t e = hipMalloc(&raw, sizeof(cpx<double>) * (size_t)n * ctx->Kh);
+        if (e != hipSuccess) return fail(ctx, FHIP_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+        for (int q = 0; q < MAX_SLOTS; ++q)
+            ctx->dset.sp.s[q] = q < n ? (cpx<double> *)raw + (size_t)q * ctx->Kh : nullptr;
+        return FHIP_OK;
+    }
+    ~SetupSlots()
+    {
+        if (raw) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(raw);
+            for (int q = 0; q < MAX_SLOTS; ++q) ctx->dset.sp.s[q] = nullptr;
+        }
+    }
+};
 
 int build_long_range(fhip_ctx *ctx)
 {
@@ -514,6 +573,8 @@ int build_long_range(fhip_ctx *ctx)
     e.scale = HARTREE_BOHR / p.epsilon_inf;
     Consts c = ctx->c;
     hipStream_t s = ctx->stream;
+    SetupSlots slots{ctx};
+    RC(slots.open(6));
     int rc = launch(ctx, K_SETUP, [&] { k_ewald_real<<<ctx->kblocks, NT, 0, s>>>(c, e, tmp); });
     if (rc == FHIP_OK) {
         const double *flds[6];
@@ -521,10 +582,11 @@ int build_long_range(fhip_ctx *ctx)
         rc = forward_generic(ctx, flds, 6);
     }
     if (rc == FHIP_OK) {
-        Spectra<double> sp = ctx->sp;
+        Spectra<double> sp = ctx->dset.sp;
         double *L = ctx->Ltab;
+        float *Lf = ctx->fset.Lt;
         const int kb = (ctx->Kh + NT - 1) / NT;
-        rc = launch(ctx, K_SETUP, [&] { k_ewald_recip<double><<<kb, NT, 0, s>>>(c, e, sp, L, nullptr); });
+        rc = launch(ctx, K_SETUP, [&] { k_ewald_recip<float><<<kb, NT, 0, s>>>(c, e, sp, L, Lf); });
     }
     hipError_t e2 = hipStreamSynchronize(ctx->stream);
     hipFree(raw);
@@ -546,7 +608,8 @@ int check_params(fhip_ctx *ctx, const fhip_params *p)
     if (p->bulk_or_film == FHIP_FILM || p->bulk_or_film == FHIP_EPIT)
         return fail(ctx, FHIP_ERR_UNSUPPORTED, "film and epit geometries (mirror) are not supported");
     if (p->bulk_or_film < 0 || p->bulk_or_film > 3) return fail(ctx, FHIP_ERR_ARG, "bad geometry");
-    if (p->precision != FHIP_FP64) return fail(ctx, FHIP_ERR_UNSUPPORTED, "only FP64 is built");
+    if (p->precision != FHIP_FP64 && p->precision != FHIP_FP32)
+        return fail(ctx, FHIP_ERR_ARG, "precision must be FHIP_FP64 or FHIP_FP32");
     if (p->method == FHIP_MD && p->mass_acou[0] > 0.0)
         return fail(ctx, FHIP_ERR_ARG, "method 'md' cannot be used with acoustic_mass_amu (param_module.F:238-242)");
     for (int s = 0; s < 3; ++s)
@@ -605,10 +668,14 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     fill_consts(ctx);
     ctx->nslots = ctx->c.acoustic ? 12 : 9;
     ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
-    ctx->yblocks = (ctx->c.ylines_main + ctx->Lz + tile_rows(ctx->Ly) - 1) / tile_rows(ctx->Ly);
-    ctx->zblocks = (ctx->c.zlines_main + ctx->Ly + tile_rows(ctx->Lz) - 1) / tile_rows(ctx->Lz);
+    for (int t = 0; t < 2; ++t) {
+        const int ty = t ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
+        const int tz = t ? line_tile_rows<float>(ctx->Lz) : line_tile_rows<double>(ctx->Lz);
+        ctx->yblocks[t] = (ctx->c.ylines_main + ctx->Lz + ty - 1) / ty;
+        ctx->zblocks[t] = (ctx->c.zlines_main + ctx->Ly + tz - 1) / tz;
+    }
     ctx->kblocks = (ctx->N + NT - 1) / NT;
-    ctx->maxblocks = std::max(std::max(ctx->xblocks, ctx->zblocks), ctx->kblocks);
+    ctx->maxblocks = std::max(std::max(ctx->xblocks, std::max(ctx->zblocks[0], ctx->zblocks[1])), ctx->kblocks);
 
     const size_t N = ctx->N, Kh = ctx->Kh;
     double *blk = nullptr;
@@ -633,10 +700,19 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     ctx->f.code = ctx->code;
-    cpx<double> *spec = nullptr;
-    RC(dalloc(ctx, &spec, (size_t)ctx->nslots * Kh));
-    for (int s = 0; s < MAX_SLOTS; ++s) ctx->sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * Kh : nullptr;
+    ctx->precision = p->precision;
+    if (ctx->precision == FHIP_FP32) {
+        cpx<float> *spec = nullptr;
+        RC(dalloc(ctx, &spec, (size_t)ctx->nslots * Kh));
+        for (int s = 0; s < MAX_SLOTS; ++s) ctx->fset.sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * Kh : nullptr;
+        RC(dalloc(ctx, &ctx->fset.Lt, 6 * Kh));
+    } else {
+        cpx<double> *spec = nullptr;
+        RC(dalloc(ctx, &spec, (size_t)ctx->nslots * Kh));
+        for (int s = 0; s < MAX_SLOTS; ++s) ctx->dset.sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * Kh : nullptr;
+    }
     RC(dalloc(ctx, &ctx->Ltab, 6 * Kh));
+    ctx->dset.Lt = ctx->Ltab;
     RC(dalloc(ctx, &ctx->acouK, 3 * Kh));
     RC(dalloc(ctx, &ctx->partials, (size_t)P_NSLOT * ctx->maxblocks));
     RC(dalloc(ctx, &ctx->dsc, 1));
@@ -651,9 +727,14 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
         for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = p->external_E_field[q];
         RC(push_scalars(ctx));
     }
-    RC(upload_table(ctx, &ctx->twx, ctx->Lx, true));
-    RC(upload_table(ctx, &ctx->twy, ctx->Ly, true));
-    RC(upload_table(ctx, &ctx->twz, ctx->Lz, true));
+    RC(upload_table(ctx, &ctx->dset.twx, ctx->Lx, true));
+    RC(upload_table(ctx, &ctx->dset.twy, ctx->Ly, true));
+    RC(upload_table(ctx, &ctx->dset.twz, ctx->Lz, true));
+    if (ctx->precision == FHIP_FP32) {
+        RC(upload_table(ctx, &ctx->fset.twx, ctx->Lx, true));
+        RC(upload_table(ctx, &ctx->fset.twy, ctx->Ly, true));
+        RC(upload_table(ctx, &ctx->fset.twz, ctx->Lz, true));
+    }
     RC(upload_table(ctx, &ctx->csx, ctx->Lx, false));
     RC(upload_table(ctx, &ctx->csy, ctx->Ly, false));
     RC(upload_table(ctx, &ctx->csz, ctx->Lz, false));
@@ -661,7 +742,8 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     if (p->modulation_constant != 0.0) {
         // the reference stops when modulation_constant is set without a .modulation
         // file (elastic.F:226-233); the table is filled by fhip_set_modulation.
-        RC(dalloc(ctx, &ctx->modk, Kh));
+        RC(dalloc(ctx, &ctx->dset.modk, Kh));
+        if (ctx->precision == FHIP_FP32) RC(dalloc(ctx, &ctx->fset.modk, Kh));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FHIP_OK;
@@ -682,6 +764,12 @@ int fhip_set_long_range(fhip_ctx *ctx, const double *lr)
                 for (int m = 0; m < 6; ++m) h[k * 6 + m] = lr[base + ia[m] + 3 * ib[m]];
             }
     HIPCHK(hipMemcpyAsync(ctx->Ltab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<float> hf;
+    if (ctx->precision == FHIP_FP32) {
+        hf.resize(h.size());
+        for (size_t q = 0; q < h.size(); ++q) hf[q] = (float)h[q];
+        HIPCHK(hipMemcpyAsync(ctx->fset.Lt, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FHIP_OK;
 }
@@ -722,10 +810,24 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
     double *d = ctx->f.F[0];      // scratch: overwritten by the next force evaluation
     HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
     const double *flds[1] = {d};
-    RC(forward_generic(ctx, flds, 1));
-    HIPCHK(hipMemcpyAsync(ctx->modk, ctx->sp.s[0], sizeof(cpx<double>) * ctx->Kh, hipMemcpyDeviceToDevice,
-                          ctx->stream));
+    {
+        SetupSlots slots{ctx};
+        RC(slots.open(1));
+        RC(forward_generic(ctx, flds, 1));
+        HIPCHK(hipMemcpyAsync(ctx->dset.modk, ctx->dset.sp.s[0], sizeof(cpx<double>) * ctx->Kh,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    }
     HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * ctx->N, ctx->stream));
+    if (ctx->precision == FHIP_FP32) {
+        std::vector<cpx<double>> md(ctx->Kh);
+        std::vector<cpx<float>> mf(ctx->Kh);
+        HIPCHK(hipMemcpyAsync(md.data(), ctx->dset.modk, sizeof(cpx<double>) * ctx->Kh, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < md.size(); ++i) mf[i] = cpx<float>{(float)md[i].x, (float)md[i].y};
+        HIPCHK(hipMemcpyAsync(ctx->fset.modk, mf.data(), sizeof(cpx<float>) * ctx->Kh, hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->c.modulation = 1;
     return collect_events(ctx);
@@ -1004,7 +1106,8 @@ int fhip_synchronize(fhip_ctx *ctx)
 
 // w reconstructed from the optimised w(k): acouR = c2r(-i acouK)/N
 // (coord_module.F:457-469).
-static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
+template <typename T>
+static int reconstruct_w_t(fhip_ctx *ctx, std::vector<double> &h)
 {
     if (ctx->c.acoustic) return fail(ctx, FHIP_ERR_ARG, "w is dynamical here: read FHIP_ACOU_R");
     if (!ctx->acouK_valid) return fail(ctx, FHIP_ERR_STATE, "no force evaluation yet");
@@ -1014,25 +1117,25 @@ static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
     HIPCHK(hipMalloc(&raw, sizeof(double) * 3 * N));
     tmp = (double *)raw;
     Consts c = ctx->c;
-    Spectra<double> sp = ctx->sp;
+    Spectra<T> sp = tset<T>(ctx).sp;
     const cpx<double> *aK = ctx->acouK;
     hipStream_t s = ctx->stream;
     const int kb = (ctx->Kh + NT - 1) / NT;
-    int rc = launch(ctx, K_SETUP, [&] { k_prep_w<double><<<kb, NT, 0, s>>>(c, sp, aK); });
+    int rc = launch(ctx, K_SETUP, [&] { k_prep_w<T><<<kb, NT, 0, s>>>(c, sp, aK); });
     if (rc == FHIP_OK) {
-        ZArgs<double> za = zargs(ctx, false);
+        ZArgs<T> za = zargs<T>(ctx, false);
         za.nplain = 3;
         za.plain_sign = +1;
-        rc = run_zpass_g<ZG_PLAIN>(ctx, za);
+        rc = run_zpass_g<T, ZG_PLAIN>(ctx, za);
     }
-    if (rc == FHIP_OK) rc = run_ypass(ctx, +1, 0, 3);
+    if (rc == FHIP_OK) rc = run_ypass<T>(ctx, +1, 0, 3);
     if (rc == FHIP_OK) {
         XinvArgs ia{};
         ia.mode = XI_GENERIC;
         ia.partials = ctx->partials;
         ia.maxblocks = ctx->maxblocks;
         for (int q = 0; q < 3; ++q) ia.gen[q] = tmp + q * N;
-        rc = run_xinv(ctx, ia, 3);
+        rc = run_xinv<T>(ctx, ia, 3);
     }
     if (rc == FHIP_OK) {
         hipError_t e = hipMemcpyAsync(h.data(), tmp, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s);
@@ -1044,4 +1147,10 @@ static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
     for (size_t i = 0; i < 3 * N; ++i) h[i] = h[i] * ctx->c.Ninv;
     // the spectral slots were used as scratch: the next force evaluation refills them
     return collect_events(ctx);
+}
+
+static int reconstruct_w(fhip_ctx *ctx, std::vector<double> &h)
+{
+    if (ctx->precision == FHIP_FP32) return reconstruct_w_t<float>(ctx, h);
+    return reconstruct_w_t<double>(ctx, h);
 }

@@ -79,6 +79,9 @@ struct Spectra {
     cpx<T>* s[MAX_SLOTS];
 };
 
+template <typename T>
+__device__ __forceinline__ cpx<double> todbl(cpx<T> a) { return cpx<double>{(double)a.x, (double)a.y}; }
+
 // Index of half-spectrum point (kx, ky, kz) in the split layout.
 __host__ __device__ __forceinline__ size_t kindex(int Mx, int Ly, size_t Mh, int kx, int ky, int kz)
 {
@@ -263,11 +266,16 @@ __device__ __forceinline__ bool yline(const Consts& c, int l, size_t& base, int&
     return true;
 }
 
+// Tiles of the y and z passes hold 16 KB: 8 FP64 or 16 FP32 complex lines at
+// N = 128, so that every tile row is a whole 128-byte cache line.
+template <typename T>
+constexpr int line_tile_cap() { return 16384 / (int)sizeof(cpx<T>); }
+
 template <typename T, int LY, int SIGN>
 __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0,
                                               const cpx<T>* __restrict__ twy)
 {
-    constexpr int TB = tile_of<LY>::value;
+    constexpr int TB = tile_of<LY, line_tile_cap<T>()>::value;
     __shared__ cpx<T> buf[LY * TB];
     __shared__ cpx<T> tws[LY];
     const int tid = threadIdx.x;
@@ -365,7 +373,7 @@ template <typename T, int LZ, int GROUP>
 __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
                                               const cpx<T>* __restrict__ twz)
 {
-    constexpr int TB = tile_of<LZ>::value;
+    constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value;
     constexpr int PPT = (LZ * TB + NT - 1) / NT;
     constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
     constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
         }
         return;
     } else {
-        cpx<double> v[NF][PPT];
+        cpx<T> v[NF][PPT];
         // forward transforms, results kept in registers; the tile of the next
         // field is fetched into registers while the current one is transformed
         cpx<T> pf[PPT];
@@ -431,10 +439,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if ((LZ * TB) % NT == 0 || e < LZ * TB) {
-                    cpx<T> x = buf[e];
-                    v[fi][i] = cpx<double>{(double)x.x, (double)x.y};
-                }
+                if ((LZ * TB) % NT == 0 || e < LZ * TB) v[fi][i] = buf[e];
             }
             __syncthreads();
         }
@@ -470,7 +475,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                 const T* lt = a.Ltab + kk * 6;
 #pragma unroll
                 for (int m = 0; m < 6; ++m) L[m] = (double)lt[m] * c.Z2;
-                const cpx<double> ux = v[0][i], uy = v[1][i], uz = v[2][i];
+                const cpx<double> ux = todbl(v[0][i]), uy = todbl(v[1][i]), uz = todbl(v[2][i]);
                 // F = (S + Z^2 L) u ; energies via Parseval with weight 1 (edge) or 2
                 cpx<double> fs[3], fl[3];
                 fs[0] = {S[0] * ux.x + S[5] * uy.x + S[4] * uz.x, S[0] * ux.y + S[5] * uy.y + S[4] * uz.y};
@@ -483,10 +488,10 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                 double es = 0.0, el = 0.0;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const cpx<double> uq = v[q][i];
+                    const cpx<double> uq = todbl(v[q][i]);
                     es += uq.x * fs[q].x + uq.y * fs[q].y;
                     el += uq.x * fl[q].x + uq.y * fl[q].y;
-                    v[q][i] = cpx<double>{fs[q].x + fl[q].x, fs[q].y + fl[q].y};
+                    v[q][i] = cpx<T>{T(fs[q].x + fl[q].x), T(fs[q].y + fl[q].y)};
                 }
                 acc0 += wgt * es;
                 acc1 += wgt * el;
@@ -496,7 +501,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                 const bool k0 = (kx == 0 && ky == 0 && kz == 0);
                 if (k0) {
 #pragma unroll
-                    for (int m = 0; m < 6; ++m) sc->dc[m] = v[m][i].x;
+                    for (int m = 0; m < 6; ++m) sc->dc[m] = (double)v[m][i].x;
                 }
                 // coupling rhs: c_mod + C uu  (optimize-inho-strain.F:36-42, dVdac.F:46-52)
                 cpx<double> cm[3], Cu[3];
@@ -513,8 +518,8 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
 #pragma unroll
                     for (int m = 0; m < 6; ++m) {
                         const double cq = coup2(c, k, q, m);
-                        rr = rr + cq * v[m][i].x;
-                        ri = ri + cq * v[m][i].y;
+                        rr = rr + cq * (double)v[m][i].x;
+                        ri = ri + cq * (double)v[m][i].y;
                     }
                     Cu[q] = {rr, ri};
                 }
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                     // acoustic MD: w(k) = i FFT(w) (acouFFT.F:19-24);
                     // dV/dw = -i [c_mod + C uu + K w] (dVdac.F:44-54)
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) w[q] = {-v[6 + q][i].y, v[6 + q][i].x};
+                    for (int q = 0; q < 3; ++q) w[q] = {-(double)v[6 + q][i].y, (double)v[6 + q][i].x};
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
                         double kr = 0.0, ki = 0.0;
@@ -587,7 +592,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                         for (int bb = 0; bb < 3; ++bb) { kr += K[q][bb] * w[bb].x; ki += K[q][bb] * w[bb].y; }
                         const double vr = ((0.0 + cm[q].x) + Cu[q].x) + kr;
                         const double vi = ((0.0 + cm[q].y) + Cu[q].y) + ki;
-                        v[6 + q][i] = cpx<double>{vi, -vr};
+                        v[6 + q][i] = cpx<T>{T(vi), T(-vr)};
                     }
                 }
                 // energies with weight 1/2 on the kx = 0 and kx = Lx/2 planes
@@ -606,12 +611,12 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                 acc1 += h * eic;
                 acc2 += h * eim;
                 // eta(k) = k (x) w (dVddi_inho_coupling.F:41-47)
-                v[0][i] = {k[0] * w[0].x, k[0] * w[0].y};
-                v[1][i] = {k[1] * w[1].x, k[1] * w[1].y};
-                v[2][i] = {k[2] * w[2].x, k[2] * w[2].y};
-                v[3][i] = {k[2] * w[1].x + k[1] * w[2].x, k[2] * w[1].y + k[1] * w[2].y};
-                v[4][i] = {k[0] * w[2].x + k[2] * w[0].x, k[0] * w[2].y + k[2] * w[0].y};
-                v[5][i] = {k[1] * w[0].x + k[0] * w[1].x, k[1] * w[0].y + k[0] * w[1].y};
+                v[0][i] = cpx<T>{T(k[0] * w[0].x), T(k[0] * w[0].y)};
+                v[1][i] = cpx<T>{T(k[1] * w[1].x), T(k[1] * w[1].y)};
+                v[2][i] = cpx<T>{T(k[2] * w[2].x), T(k[2] * w[2].y)};
+                v[3][i] = cpx<T>{T(k[2] * w[1].x + k[1] * w[2].x), T(k[2] * w[1].y + k[1] * w[2].y)};
+                v[4][i] = cpx<T>{T(k[0] * w[2].x + k[2] * w[0].x), T(k[0] * w[2].y + k[2] * w[0].y)};
+                v[5][i] = cpx<T>{T(k[1] * w[0].x + k[0] * w[1].x), T(k[1] * w[0].y + k[0] * w[1].y)};
             }
         }
         // inverse transforms
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if ((LZ * TB) % NT == 0 || e < LZ * TB) buf[e] = cpx<T>{T(v[fi][i].x), T(v[fi][i].y)};
+                if ((LZ * TB) % NT == 0 || e < LZ * TB) buf[e] = v[fi][i];
             }
             __syncthreads();
             fft_lds<T, LZ, TB, NT, +1>(buf, tws, tid);

@@ -312,3 +312,80 @@ def test_128_cubed_force_is_gradient():
         vm = V()
         fd = (vp - vm) / (2 * h)
         assert fd == pytest.approx(F[idx], rel=1e-4, abs=1e-6 * np.max(np.abs(F)))
+
+
+# ---- FP32 mode: spectra, FFTs and the k-space stage in FP32; state, local
+# forces, thermostat and energy sums in FP64.  Bound from the FP32-mode target:
+# force error <= 1e-5 relative (normwise) against the FP64 oracle.
+FP32_TOL = 1e-5
+
+
+def _system32(p):
+    from feram_amd.system import System
+    print(f"[gpu] System FP32 L=({p.Lx},{p.Ly},{p.Lz}) method={p.method}", flush=True)
+    return System(p, precision=1)
+
+
+@pytest.mark.parametrize("L", [12, 16, (8, 12, 16), 32])
+def test_fp32_force_and_energies(oracle_mod, L):
+    p = batio3_zvr(L)
+    o = oracle_mod.Oracle(p)
+    g = _system32(p)
+    u = _random(p, 70)
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    err = _normwise(g.get("dVddi"), o.field("dVddi"))
+    print(f"[gpu] FP32 force error {err:.2e}")
+    assert err <= FP32_TOL
+    _check_energies(eg, eo, tol=FP32_TOL)
+    lo = o.field("long_range")
+    assert _normwise(g.get_long_range()[:, :p.Ly], lo) <= 1e-12      # the table is built in FP64
+
+
+def test_fp32_acoustic_force(oracle_mod):
+    p = batio3_zvr((8, 12, 8), method="lf", acoustic_mass_amu=46.44)
+    o = oracle_mod.Oracle(p)
+    g = _system32(p)
+    u = _random(p, 71)
+    w = 0.02 * np.random.default_rng(72).standard_normal(u.shape)
+    for name, v in (("dipoR", u), ("acouR", w)):
+        o.field(name)[...] = v
+        g.put(name, v)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FP32_TOL
+    assert _normwise(g.get("dVdacR"), o.field("dVdacR")) <= FP32_TOL
+    _check_energies(eg, eo, tol=FP32_TOL)
+
+
+def test_fp32_np_trajectory(oracle_mod):
+    p = batio3_zvr(16, kelvin=300.0)
+    o = oracle_mod.Oracle(p)
+    g = _system32(p)
+    u = np.random.default_rng(73).uniform(-0.05, 0.05, size=(16, 16, 16, 3))
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    g.md_prologue()
+    for _ in range(20):
+        o.nose_poincare(e_o)
+    eg = g.nose_poincare(20)
+    assert _normwise(g.get("dipoR"), o.field("dipoR")) <= FP32_TOL
+    assert _normwise(g.get("dipoP"), o.field("dipoP")) <= FP32_TOL
+    _check_energies(eg, e_o.as_dict(), tol=FP32_TOL)
+    assert g.nose[0] == pytest.approx(o.nose[0], rel=FP32_TOL)
+
+
+def test_fp32_modulation_energies():
+    g_ = load_golden()["modulation_check"]
+    p = param_from_dict(g_["params"])
+    mod = np.zeros((p.Lz, p.Ly, p.Lx))
+    for x, y, z, a in g_["modulation"]:
+        mod[z, y, x] = a
+    g = _system32(p)
+    g.set_modulation(mod)
+    e = g.md_prologue()
+    assert e["inho_strain"] / p.N == pytest.approx(0.000072004971, rel=1e-6)
+    assert e["inho_modulation"] / p.N == pytest.approx(-0.000144009942, rel=1e-6)
