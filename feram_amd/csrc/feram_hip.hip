@@ -83,6 +83,8 @@ struct fhip_ctx {
     double *Ltab = nullptr;           // FP64 L(k), 6 words per point (master copy)
     cpx<double> *acouK = nullptr;
     cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
+    double *kvx = nullptr, *kvy = nullptr, *kvz = nullptr;
+    size_t Ks = 0;                    // spectral slot stride: Kh k-space points + the y-major Nyquist copy
     unsigned char *code = nullptr;
     std::vector<void *> allocs;
     int64_t bytes = 0;
@@ -282,6 +284,9 @@ ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
     a.csx = ctx->csx;
     a.csy = ctx->csy;
     a.csz = ctx->csz;
+    a.kvx = ctx->kvx;
+    a.kvy = ctx->kvy;
+    a.kvz = ctx->kvz;
     a.modk = tset<T>(ctx).modk;
     a.acouK = want_acouK ? ctx->acouK : nullptr;
     a.partials = ctx->partials;
@@ -416,12 +421,10 @@ void fill_consts(fhip_ctx *ctx)
     c.Lx = p.Lx; c.Ly = p.Ly; c.Lz = p.Lz;
     c.Mx = ctx->Mx; c.Mh = ctx->Mh; c.N = ctx->N; c.Kh = ctx->Kh;
     {
-        // whole tiles of the pipeline's spectral type; the FP32 tile is a
+        // whole y tiles of the pipeline's spectral type; the FP32 tile is a
         // multiple of the FP64 one, so FP64 set-up transforms share the layout
         const int ty = p.precision == FHIP_FP32 ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
-        const int tz = p.precision == FHIP_FP32 ? line_tile_rows<float>(ctx->Lz) : line_tile_rows<double>(ctx->Lz);
         c.ylines_main = (ctx->Mx * ctx->Lz + ty - 1) / ty * ty;
-        c.zlines_main = (ctx->Mx * ctx->Ly + tz - 1) / tz * tz;
     }
     c.acoustic = p.mass_acou[0] > 0.0;
     c.efield = !p.E_field_always_zero;
@@ -516,6 +519,17 @@ int upload_table(fhip_ctx *ctx, cpx<T> **dst, int n, bool twiddle)
     return FHIP_OK;
 }
 
+// k components kcomp(i, L) for i < n along one axis.
+int upload_kcomp(fhip_ctx *ctx, double **dst, int n, int L, bool fold)
+{
+    std::vector<double> h(n);
+    for (int i = 0; i < n; ++i) h[i] = kcomp(i, L, ctx->p.a0, fold);
+    RC(dalloc(ctx, dst, n));
+    HIPCHK(hipMemcpyAsync(*dst, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FHIP_OK;
+}
+
 // Forward 3-D r2c, in FP64, of nf real device fields into the FP64 spectral
 // slots 0..nf-1 (set-up transforms run in FP64 in every precision mode).
 int forward_generic(fhip_ctx *ctx, const double *const *fields, int nf)
@@ -539,10 +553,10 @@ struct SetupSlots {
     int open(int n)
     {
         if (ctx->precision == FHIP_FP64) return FHIP_OK;
-        hipError_t e = hipMalloc(&raw, sizeof(cpx<double>) * (size_t)n * ctx->Kh);
+        hipError_t e = hipMalloc(&raw, sizeof(cpx<double>) * (size_t)n * ctx->Ks);
         if (e != hipSuccess) return fail(ctx, FHIP_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
         for (int q = 0; q < MAX_SLOTS; ++q)
-            ctx->dset.sp.s[q] = q < n ? (cpx<double> *)raw + (size_t)q * ctx->Kh : nullptr;
+            ctx->dset.sp.s[q] = q < n ? (cpx<double> *)raw + (size_t)q * ctx->Ks : nullptr;
         return FHIP_OK;
     }
     ~SetupSlots()
@@ -664,6 +678,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     ctx->Mh = (size_t)ctx->Mx * p->Ly * p->Lz;
     ctx->N = p->Lx * p->Ly * p->Lz;
     ctx->Kh = ctx->Kx * p->Ly * p->Lz;
+    ctx->Ks = (size_t)ctx->Kh + (size_t)p->Ly * p->Lz;
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     fill_consts(ctx);
     ctx->nslots = ctx->c.acoustic ? 12 : 9;
@@ -672,7 +687,8 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
         const int ty = t ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
         const int tz = t ? line_tile_rows<float>(ctx->Lz) : line_tile_rows<double>(ctx->Lz);
         ctx->yblocks[t] = (ctx->c.ylines_main + ctx->Lz + ty - 1) / ty;
-        ctx->zblocks[t] = (ctx->c.zlines_main + ctx->Ly + tz - 1) / tz;
+        // z tiles: TB main pencils each, the Ly Nyquist pencils folded in one per tile
+        ctx->zblocks[t] = std::max((ctx->Mx * ctx->Ly + tz - 1) / tz, ctx->Ly);
     }
     ctx->kblocks = (ctx->N + NT - 1) / NT;
     ctx->maxblocks = std::max(std::max(ctx->xblocks, std::max(ctx->zblocks[0], ctx->zblocks[1])), ctx->kblocks);
@@ -703,13 +719,13 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     ctx->precision = p->precision;
     if (ctx->precision == FHIP_FP32) {
         cpx<float> *spec = nullptr;
-        RC(dalloc(ctx, &spec, (size_t)ctx->nslots * Kh));
-        for (int s = 0; s < MAX_SLOTS; ++s) ctx->fset.sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * Kh : nullptr;
+        RC(dalloc(ctx, &spec, (size_t)ctx->nslots * ctx->Ks));
+        for (int s = 0; s < MAX_SLOTS; ++s) ctx->fset.sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * ctx->Ks : nullptr;
         RC(dalloc(ctx, &ctx->fset.Lt, 6 * Kh));
     } else {
         cpx<double> *spec = nullptr;
-        RC(dalloc(ctx, &spec, (size_t)ctx->nslots * Kh));
-        for (int s = 0; s < MAX_SLOTS; ++s) ctx->dset.sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * Kh : nullptr;
+        RC(dalloc(ctx, &spec, (size_t)ctx->nslots * ctx->Ks));
+        for (int s = 0; s < MAX_SLOTS; ++s) ctx->dset.sp.s[s] = (s < ctx->nslots) ? spec + (size_t)s * ctx->Ks : nullptr;
     }
     RC(dalloc(ctx, &ctx->Ltab, 6 * Kh));
     ctx->dset.Lt = ctx->Ltab;
@@ -738,6 +754,9 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     RC(upload_table(ctx, &ctx->csx, ctx->Lx, false));
     RC(upload_table(ctx, &ctx->csy, ctx->Ly, false));
     RC(upload_table(ctx, &ctx->csz, ctx->Lz, false));
+    RC(upload_kcomp(ctx, &ctx->kvx, ctx->Kx, ctx->Lx, false));
+    RC(upload_kcomp(ctx, &ctx->kvy, ctx->Ly, ctx->Ly, true));
+    RC(upload_kcomp(ctx, &ctx->kvz, ctx->Lz, ctx->Lz, true));
     RC(build_long_range(ctx));
     if (p->modulation_constant != 0.0) {
         // the reference stops when modulation_constant is set without a .modulation
@@ -759,7 +778,7 @@ int fhip_set_long_range(fhip_ctx *ctx, const double *lr)
     for (int z = 0; z < Lz; ++z)
         for (int y = 0; y < Ly; ++y)
             for (int x = 0; x < Kx; ++x) {
-                const size_t k = kindex(ctx->Mx, Ly, ctx->Mh, x, y, z);
+                const size_t k = kindex(ctx->Mx, Ly, Lz, ctx->Mh, x, y, z);
                 const size_t base = 9 * (x + (size_t)Kx * (y + dy * z));
                 for (int m = 0; m < 6; ++m) h[k * 6 + m] = lr[base + ia[m] + 3 * ib[m]];
             }
@@ -786,7 +805,7 @@ int fhip_get_long_range(fhip_ctx *ctx, double *lr)
     for (int z = 0; z < Lz; ++z)
         for (int y = 0; y < Ly; ++y)
             for (int x = 0; x < Kx; ++x) {
-                const size_t k = kindex(ctx->Mx, Ly, ctx->Mh, x, y, z);
+                const size_t k = kindex(ctx->Mx, Ly, Lz, ctx->Mh, x, y, z);
                 const size_t base = 9 * (x + (size_t)Kx * (y + dy * z));
                 for (int m = 0; m < 6; ++m) {
                     const double v = h[k * 6 + m];

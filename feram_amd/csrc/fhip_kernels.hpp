@@ -3,10 +3,14 @@
 // Device layout (structure of arrays, unpadded, x fastest):
 //   real fields   f[c],  c = x + Lx*(y + Ly*z)                 (one array per component)
 //   half spectra  Mx = Lx/2 columns kx = 0..Mx-1 in rows of Mx points,
-//                 s[(kz*Ly + ky)*Mx + kx], followed by the Nyquist column
-//                 kx = Mx at s[Mh + kz*Ly + ky], Mh = Mx*Ly*Lz.  Rows of
-//                 Mx points keep every tile of the y and z passes aligned to
-//                 whole cache lines.
+//                 s[(kz*Ly + ky)*Mx + kx], Mh = Mx*Ly*Lz points, followed by
+//                 the Nyquist column kx = Mx in two orders: z-major
+//                 s[Mh + ky*Lz + kz] (the k-space copy, so that every point
+//                 of the half spectrum has an index kk < Kh = Mh + Ly*Lz) and
+//                 y-major s[Kh + z*Ly + y] (the staging copy the x passes
+//                 write and read).  The y pass converts between the two.
+//                 Rows of Mx points keep every tile of the y and z passes
+//                 aligned to whole cache lines.
 // Spectral slots of one step:
 //   0..2  u_x,u_y,u_z        -> (S + Z*^2 L) u   (dipole-dipole force before 1/N)
 //   3..8  uu_xx..uu_xy       -> eta(k) = k (x) w(k)  (inhomogeneous strain)
@@ -29,9 +33,8 @@ enum PartialSlot {
 // Constants shared by all kernels (by value in the kernel arguments).
 struct Consts {
     int Lx, Ly, Lz, Mx, N, Kh;
-    size_t Mh;           // Mx*Ly*Lz: start of the Nyquist column
+    size_t Mh;           // Mx*Ly*Lz: start of the z-major Nyquist column (Kh: the y-major one)
     int ylines_main;     // Mx*Lz rounded up to whole y tiles
-    int zlines_main;     // Mx*Ly rounded up to whole z tiles
     int acoustic;        // mass_acou > 0: w is dynamical
     int efield;          // !E_field_always_zero
     int has_eloc;        // per-cell local field present
@@ -82,10 +85,20 @@ struct Spectra {
 template <typename T>
 __device__ __forceinline__ cpx<double> todbl(cpx<T> a) { return cpx<double>{(double)a.x, (double)a.y}; }
 
-// Index of half-spectrum point (kx, ky, kz) in the split layout.
-__host__ __device__ __forceinline__ size_t kindex(int Mx, int Ly, size_t Mh, int kx, int ky, int kz)
+// Index of half-spectrum point (kx, ky, kz) in the k-space order.
+__host__ __device__ __forceinline__ size_t kindex(int Mx, int Ly, int Lz, size_t Mh, int kx, int ky, int kz)
 {
-    return kx < Mx ? ((size_t)kz * Ly + ky) * Mx + kx : Mh + (size_t)kz * Ly + ky;
+    return kx < Mx ? ((size_t)kz * Ly + ky) * Mx + kx : Mh + (size_t)ky * Lz + kz;
+}
+
+// One component of the k vector of index i along an axis of length L
+// (elastic.F:83-89): folded to (-1/2, 1/2] except along x.
+__host__ __device__ __forceinline__ double kcomp(int i, int L, double a0, bool fold)
+{
+    const double M2PI = 6.28318530717958647693;
+    double f = (double)i / L;
+    if (fold && f > 0.5) f = f - 1.0;
+    return f * M2PI / a0;
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red)
@@ -237,7 +250,7 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
             if (row0 + r >= nrows) continue;
             const cpx<T> zk = buf[kx * ST + r];
             const cpx<T> zn = buf[((LX - kx) % LX) * ST + r];
-            const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : c.Mh + (size_t)(row0 + r);
+            const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : (size_t)c.Kh + (size_t)(row0 + r);
             sp.s[f0][o] = cpx<T>{(zk.x + zn.x) * T(0.5), (zk.y - zn.y) * T(0.5)};
             if (two) sp.s[f0 + 1][o] = cpx<T>{(zk.y + zn.y) * T(0.5), (zn.x - zk.x) * T(0.5)};
         }
@@ -246,25 +259,14 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
 }
 
 // ---------------------------------------------------------------------------
-// Line transform along y, in place, one field per blockIdx.y.  Lines
-// l < ylines_main are (z, kx<Mx) columns of stride Mx; lines from
-// ylines_main on are the Nyquist column of plane z (contiguous).
+// Line transform along y, one field per blockIdx.y.  Tiles below
+// ylines_main hold TB columns (z, kx<Mx) of stride Mx, transformed in place.
+// Tiles from ylines_main on hold the Nyquist columns of TB planes z: the
+// forward transform reads the y-major staging copy and writes the z-major
+// k-space copy, the inverse transform the other way round.  Both copies are
+// read and written in whole rows of TB points (consecutive z) or whole
+// columns (consecutive y).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool yline(const Consts& c, int l, size_t& base, int& stride)
-{
-    if (l < c.ylines_main) {
-        if (l >= c.Mx * c.Lz) return false;
-        const int z = l / c.Mx, kx = l - z * c.Mx;
-        base = (size_t)z * c.Ly * c.Mx + kx;
-        stride = c.Mx;
-        return true;
-    }
-    const int z = l - c.ylines_main;
-    if (z >= c.Lz) return false;
-    base = c.Mh + (size_t)z * c.Ly;
-    stride = 1;
-    return true;
-}
 
 // Tiles of the y and z passes hold 16 KB: 8 FP64 or 16 FP32 complex lines at
 // N = 128, so that every tile row is a whole 128-byte cache line.
@@ -282,27 +284,59 @@ __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0
     for (int i = tid; i < LY; i += NT) tws[i] = twy[i];
     const int l0 = blockIdx.x * TB;
     cpx<T>* s = sp.s[slot0 + blockIdx.y];
-    const bool nyq = l0 >= c.ylines_main;
+    if (l0 < c.ylines_main) {
+        const int nmain = c.Mx * c.Lz;
+        for (int e = tid; e < LY * TB; e += NT) {
+            const int t = e / TB, b = e - t * TB, l = l0 + b;
+            if (l < nmain) {
+                const int z = l / c.Mx, kx = l - z * c.Mx;
+                buf[e] = s[((size_t)z * c.Ly + t) * c.Mx + kx];
+            }
+        }
+        __syncthreads();
+        fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
+        for (int e = tid; e < LY * TB; e += NT) {
+            const int t = e / TB, b = e - t * TB, l = l0 + b;
+            if (l < nmain) {
+                const int z = l / c.Mx, kx = l - z * c.Mx;
+                s[((size_t)z * c.Ly + t) * c.Mx + kx] = buf[e];
+            }
+        }
+        return;
+    }
+    const int z0 = l0 - c.ylines_main;
+    cpx<T>* ymaj = s + c.Kh;          // [z][y]
+    cpx<T>* zmaj = s + c.Mh;          // [ky][z]
+    // element (t, b): point t of the Nyquist column of plane z0 + b
     for (int e = tid; e < LY * TB; e += NT) {
-        // main tiles: consecutive threads walk consecutive kx; Nyquist tiles:
-        // consecutive threads walk y, which is contiguous there
-        const int t = nyq ? e % LY : e / TB, b = nyq ? e / LY : e - (e / TB) * TB;
-        size_t base; int stride;
-        if (yline(c, l0 + b, base, stride)) buf[t * TB + b] = s[base + (size_t)t * stride];
+        if (SIGN < 0) {
+            const int b = e / LY, t = e - b * LY;
+            if (z0 + b < c.Lz) buf[t * TB + b] = ymaj[(size_t)(z0 + b) * LY + t];
+        } else {
+            const int t = e / TB, b = e - t * TB;
+            if (z0 + b < c.Lz) buf[e] = zmaj[(size_t)t * c.Lz + z0 + b];
+        }
     }
     __syncthreads();
     fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
     for (int e = tid; e < LY * TB; e += NT) {
-        const int t = nyq ? e % LY : e / TB, b = nyq ? e / LY : e - (e / TB) * TB;
-        size_t base; int stride;
-        if (yline(c, l0 + b, base, stride)) s[base + (size_t)t * stride] = buf[t * TB + b];
+        if (SIGN < 0) {
+            const int t = e / TB, b = e - t * TB;
+            if (z0 + b < c.Lz) zmaj[(size_t)t * c.Lz + z0 + b] = buf[e];
+        } else {
+            const int b = e / LY, t = e - b * LY;
+            if (z0 + b < c.Lz) ymaj[(size_t)(z0 + b) * LY + t] = buf[t * TB + b];
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // z pass: forward transform along z, k-space stage, inverse transform.
-// Lines l < zlines_main are (ky, kx<Mx) pencils of stride Mx*Ly; lines from
-// zlines_main on are the Nyquist pencils (ky) of stride Ly.
+// Workgroup w holds TB main pencils (ky, kx<Mx), lines l = w*TB + b of
+// stride Mx*Ly, and, as line TB of its tile, the Nyquist pencil ky = w
+// (kx = Mx) of the z-major copy, contiguous along kz.  Folding the Ly
+// Nyquist pencils into the main tiles keeps the number of workgroups at the
+// number of main tiles, so that the grid fits the resident workgroups.
 // ---------------------------------------------------------------------------
 enum ZGroup { ZG_PLAIN = 0, ZG_DD = 1, ZG_STRAIN = 2, ZG_ACOU = 3 };
 
@@ -312,6 +346,9 @@ struct ZArgs {
     const cpx<double>* csx;          // cos/sin tables (2 pi i / L) per axis
     const cpx<double>* csy;
     const cpx<double>* csz;
+    const double* kvx;               // k components per axis (kcomp)
+    const double* kvy;
+    const double* kvz;
     const cpx<T>* modk;              // FFT of the modulation field (or null)
     cpx<double>* acouK;              // materialised optimised w(k) (or null)
     double* partials;                // [P_NSLOT][maxblocks]
@@ -320,126 +357,300 @@ struct ZArgs {
     int plain_sign;                  // ZG_PLAIN: -1 forward, +1 inverse
 };
 
-struct ZLine {
-    size_t base, stride;
-    int kx, ky;
+// Element e of the tile of workgroup w: its point of the half spectrum and
+// its place in the LDS tile (line b of TB+1 at t*(TB+1) + b).
+struct ZElem {
+    size_t kk;
+    int kx, ky, kz, lds;
     bool ok;
 };
 
-__device__ __forceinline__ ZLine zline(const Consts& c, int l)
+template <int LZ, int TB>
+__device__ __forceinline__ ZElem zelem(const Consts& c, int w, int e)
 {
-    ZLine z;
-    if (l < c.zlines_main) {
+    ZElem z;
+    if (e < LZ * TB) {
+        const int t = e / TB, b = e - t * TB, l = w * TB + b;
         z.ok = l < c.Mx * c.Ly;
         z.ky = l / c.Mx;
         z.kx = l - z.ky * c.Mx;
-        z.base = (size_t)l;
-        z.stride = (size_t)c.Mx * c.Ly;
-        return z;
+        z.kz = t;
+        z.kk = (size_t)l + (size_t)t * ((size_t)c.Mx * c.Ly);
+        z.lds = t * (TB + 1) + b;
+    } else {
+        const int t = e - LZ * TB;
+        z.ok = w < c.Ly;
+        z.kx = c.Mx;
+        z.ky = w;
+        z.kz = t;
+        z.kk = c.Mh + (size_t)w * LZ + t;
+        z.lds = t * (TB + 1) + TB;
     }
-    z.ky = l - c.zlines_main;
-    z.ok = z.ky < c.Ly;
-    z.kx = c.Mx;
-    z.base = c.Mh + (size_t)z.ky;
-    z.stride = (size_t)c.Ly;
     return z;
 }
 
-// k vector of the half-spectrum point (elastic.F:83-89)
-__device__ __forceinline__ void kvec(const Consts& c, int kx, int ky, int kz, double k[3])
-{
-    const double M2PI = 6.28318530717958647693;
-    double f3 = (double)kz / c.Lz; if (f3 > 0.5) f3 = f3 - 1.0;
-    double f2 = (double)ky / c.Ly; if (f2 > 0.5) f2 = f2 - 1.0;
-    k[2] = f3 * M2PI / c.a0;
-    k[1] = f2 * M2PI / c.a0;
-    k[0] = (double)kx / c.Lx * M2PI / c.a0;
-}
-
 // coup2(a, i) (elastic.F:148-175)
-__device__ __forceinline__ double coup2(const Consts& c, const double k[3], int a, int i)
+template <typename KT>
+__device__ __forceinline__ KT coup2(const Consts& c, const KT k[3], int a, int i)
 {
-    if (i < 3) return (i == a) ? k[a] * c.B1xx / 2 : k[a] * c.B1yy / 2;
+    if (i < 3) return (i == a) ? k[a] * KT(c.B1xx) / 2 : k[a] * KT(c.B1yy) / 2;
     // ki = ((0,3,2),(3,0,1),(2,1,0)) by column: component of k multiplying w_a in eta_{i}
     const int m = i - 3;
     int q;
     if (m == 0) q = (a == 0) ? -1 : (a == 1 ? 2 : 1);
     else if (m == 1) q = (a == 0) ? 2 : (a == 1 ? -1 : 0);
     else q = (a == 0) ? 1 : (a == 1 ? 0 : -1);
-    return (q < 0 ? 0.0 : k[q]) * c.B4yz;
+    return (q < 0 ? KT(0) : k[q]) * KT(c.B4yz);
 }
+
+// k-space stage of one point, arithmetic in KT (FP64, or FP32 in FP32 mode).
+// pv holds the NF forward spectra of the point on entry and the spectra to
+// transform back on exit; acc0..2 collect the Parseval energy sums.
+template <typename KT, int GROUP, typename T, int NF>
+__device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a, Scalars* sc, cpx<T> (&pv)[NF],
+                                             const ZElem& z, double& acc0, double& acc1, double& acc2)
+{
+    const int kx = z.kx, ky = z.ky, kz = z.kz;
+    const size_t kk = z.kk;
+    const bool edge = (kx == 0) || (kx == c.Mx);
+    if constexpr (GROUP == ZG_DD) {
+        // S(k): Fourier transform of the short-range couplings
+        // (dipole-dipole-short-range.F:35-83) in closed form.
+        const cpx<double> ex = a.csx[kx], ey = a.csy[ky], ez = a.csz[kz];
+        const KT cx = KT(ex.x), cy = KT(ey.x), cz = KT(ez.x), sx = KT(ex.y), sy = KT(ey.y), szz = KT(ez.y);
+        const KT j1 = KT(c.j[0]), j2 = KT(c.j[1]), j3 = KT(c.j[2]), j4 = KT(c.j[3]), j5 = KT(c.j[4]),
+                 j6 = KT(c.j[5]), j7 = KT(c.j[6]);
+        const KT k2t = KT(2.0 * c.kappa2[0]);
+        const KT cxyz = cx * cy * cz;
+        KT S[6];
+        S[0] = k2t + 2 * j2 * cx + 2 * j1 * (cy + cz) + 4 * j3 * cx * (cy + cz) + 4 * j4 * cy * cz + 8 * j6 * cxyz;
+        S[1] = k2t + 2 * j2 * cy + 2 * j1 * (cz + cx) + 4 * j3 * cy * (cz + cx) + 4 * j4 * cz * cx + 8 * j6 * cxyz;
+        S[2] = k2t + 2 * j2 * cz + 2 * j1 * (cx + cy) + 4 * j3 * cz * (cx + cy) + 4 * j4 * cx * cy + 8 * j6 * cxyz;
+        S[3] = -4 * j5 * sy * szz - 8 * j7 * sy * szz * cx;   // yz
+        S[4] = -4 * j5 * szz * sx - 8 * j7 * szz * sx * cy;   // zx
+        S[5] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz;     // xy
+        KT L[6];
+        const T* lt = a.Ltab + kk * 6;
+        const KT Z2 = KT(c.Z2);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) L[m] = KT(lt[m]) * Z2;
+        const cpx<KT> ux = {KT(pv[0].x), KT(pv[0].y)}, uy = {KT(pv[1].x), KT(pv[1].y)},
+                      uz = {KT(pv[2].x), KT(pv[2].y)};
+        // F = (S + Z^2 L) u ; energies via Parseval with weight 1 (edge) or 2
+        cpx<KT> fs[3], fl[3];
+        fs[0] = {S[0] * ux.x + S[5] * uy.x + S[4] * uz.x, S[0] * ux.y + S[5] * uy.y + S[4] * uz.y};
+        fs[1] = {S[5] * ux.x + S[1] * uy.x + S[3] * uz.x, S[5] * ux.y + S[1] * uy.y + S[3] * uz.y};
+        fs[2] = {S[4] * ux.x + S[3] * uy.x + S[2] * uz.x, S[4] * ux.y + S[3] * uy.y + S[2] * uz.y};
+        fl[0] = {L[0] * ux.x + L[5] * uy.x + L[4] * uz.x, L[0] * ux.y + L[5] * uy.y + L[4] * uz.y};
+        fl[1] = {L[5] * ux.x + L[1] * uy.x + L[3] * uz.x, L[5] * ux.y + L[1] * uy.y + L[3] * uz.y};
+        fl[2] = {L[4] * ux.x + L[3] * uy.x + L[2] * uz.x, L[4] * ux.y + L[3] * uy.y + L[2] * uz.y};
+        const cpx<KT> uq[3] = {ux, uy, uz};
+        KT es = KT(0), el = KT(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            es += uq[q].x * fs[q].x + uq[q].y * fs[q].y;
+            el += uq[q].x * fl[q].x + uq[q].y * fl[q].y;
+            pv[q] = cpx<T>{T(fs[q].x + fl[q].x), T(fs[q].y + fl[q].y)};
+        }
+        const double wgt = edge ? 1.0 : 2.0;
+        acc0 += wgt * (double)es;
+        acc1 += wgt * (double)el;
+    } else {
+        const KT k[3] = {KT(a.kvx[kx]), KT(a.kvy[ky]), KT(a.kvz[kz])};
+        const bool k0 = (kx == 0 && ky == 0 && kz == 0);
+        if (k0) {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) sc->dc[m] = (double)pv[m].x;
+        }
+        // coupling rhs: c_mod + C uu  (optimize-inho-strain.F:36-42, dVdac.F:46-52)
+        cpx<KT> cm[3], Cu[3];
+        cpx<KT> mk = {KT(0), KT(0)};
+        if (c.modulation) {
+            const cpx<T> m0 = a.modk[kk];
+            mk = {KT(m0.x), KT(m0.y)};
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const KT fq = KT(c.mod_c) * k[q];
+            cm[q] = c.modulation ? cpx<KT>{fq * mk.x, fq * mk.y} : cpx<KT>{KT(0), KT(0)};
+            KT rr = KT(0), ri = KT(0);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                const KT cq = coup2<KT>(c, k, q, m);
+                rr = rr + cq * KT(pv[m].x);
+                ri = ri + cq * KT(pv[m].y);
+            }
+            Cu[q] = {rr, ri};
+        }
+        // K(k) (elastic.F:90-104)
+        const KT B11 = KT(c.B11), B12 = KT(c.B12), B44 = KT(c.B44);
+        KT K[3][3];
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb) {
+            const int o1 = (bb + 2) % 3, o2 = (bb + 1) % 3;
+#pragma unroll
+            for (int aa = 0; aa < 3; ++aa) {
+                if (aa == bb)
+                    K[aa][bb] = B11 * k[aa] * k[aa] + B44 * k[o1] * k[o1] + B44 * k[o2] * k[o2];
+                else
+                    K[aa][bb] = B12 * k[aa] * k[bb] + B44 * k[aa] * k[bb];
+            }
+        }
+        cpx<KT> w[3];
+        if constexpr (GROUP == ZG_STRAIN) {
+            if (k0) {
+                w[0] = w[1] = w[2] = cpx<KT>{KT(0), KT(0)};
+            } else {
+                // Cholesky K = G G^T (elastic.F:131-141), then the two
+                // triangular solves of optimize-inho-strain.F:44-57
+                KT G[3][3], Gi[3];
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb) {
+                    KT s2 = KT(0);
+#pragma unroll
+                    for (int l2 = 0; l2 < bb; ++l2) s2 += G[bb][l2] * G[bb][l2];
+                    G[bb][bb] = sqrt(K[bb][bb] - s2);
+                    Gi[bb] = KT(1) / G[bb][bb];
+#pragma unroll
+                    for (int aa = bb + 1; aa < 3; ++aa) {
+                        KT s3 = KT(0);
+#pragma unroll
+                        for (int l2 = 0; l2 < bb; ++l2) s3 += G[aa][l2] * G[bb][l2];
+                        G[aa][bb] = (K[aa][bb] - s3) * Gi[bb];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 3; ++q) w[q] = {-cm[q].x - Cu[q].x, -cm[q].y - Cu[q].y};
+#pragma unroll
+                for (int aa = 0; aa < 3; ++aa) {
+                    KT tr = KT(0), ti = KT(0);
+#pragma unroll
+                    for (int bb = 0; bb < aa; ++bb) { tr += G[aa][bb] * w[bb].x; ti += G[aa][bb] * w[bb].y; }
+                    w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
+                }
+#pragma unroll
+                for (int aa = 2; aa >= 0; --aa) {
+                    KT tr = KT(0), ti = KT(0);
+#pragma unroll
+                    for (int bb = aa + 1; bb < 3; ++bb) { tr += w[bb].x * G[bb][aa]; ti += w[bb].y * G[bb][aa]; }
+                    w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
+                }
+            }
+            if (a.acouK) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    a.acouK[(size_t)q * c.Kh + kk] = cpx<double>{(double)w[q].x, (double)w[q].y};
+            }
+        } else {
+            // acoustic MD: w(k) = i FFT(w) (acouFFT.F:19-24);
+            // dV/dw = -i [c_mod + C uu + K w] (dVdac.F:44-54)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) w[q] = {-KT(pv[6 + q].y), KT(pv[6 + q].x)};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                KT kr = KT(0), ki = KT(0);
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb) { kr += K[q][bb] * w[bb].x; ki += K[q][bb] * w[bb].y; }
+                const KT vr = ((KT(0) + cm[q].x) + Cu[q].x) + kr;
+                const KT vi = ((KT(0) + cm[q].y) + Cu[q].y) + ki;
+                pv[6 + q] = cpx<T>{T(vi), T(-vr)};
+            }
+        }
+        // energies with weight 1/2 on the kx = 0 and kx = Lx/2 planes
+        // (energy_module.F:162-239)
+        KT eis = KT(0), eic = KT(0), eim = KT(0);
+#pragma unroll
+        for (int aa = 0; aa < 3; ++aa) {
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+                eis += K[aa][bb] * (w[aa].x * w[bb].x + w[aa].y * w[bb].y);
+            eic += w[aa].x * Cu[aa].x + w[aa].y * Cu[aa].y;
+            eim += w[aa].x * cm[aa].x + w[aa].y * cm[aa].y;
+        }
+        const double h = edge ? 0.5 : 1.0;
+        acc0 += h * (double)eis;
+        acc1 += h * (double)eic;
+        acc2 += h * (double)eim;
+        // eta(k) = k (x) w (dVddi_inho_coupling.F:41-47)
+        pv[0] = cpx<T>{T(k[0] * w[0].x), T(k[0] * w[0].y)};
+        pv[1] = cpx<T>{T(k[1] * w[1].x), T(k[1] * w[1].y)};
+        pv[2] = cpx<T>{T(k[2] * w[2].x), T(k[2] * w[2].y)};
+        pv[3] = cpx<T>{T(k[2] * w[1].x + k[1] * w[2].x), T(k[2] * w[1].y + k[1] * w[2].y)};
+        pv[4] = cpx<T>{T(k[0] * w[2].x + k[2] * w[0].x), T(k[0] * w[2].y + k[2] * w[0].y)};
+        pv[5] = cpx<T>{T(k[1] * w[0].x + k[0] * w[1].x), T(k[1] * w[0].y + k[0] * w[1].y)};
+    }
+}
+
+// Arithmetic type of the k-space stage: FP64 with FP64 spectra, FP32 with FP32.
+template <typename T> struct kspace_type { using type = double; };
+template <> struct kspace_type<float> { using type = float; };
 
 template <typename T, int LZ, int GROUP>
 __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
                                               const cpx<T>* __restrict__ twz)
 {
     constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value;
-    constexpr int PPT = (LZ * TB + NT - 1) / NT;
+    constexpr int TL = TB + 1;                 // lines per tile: TB main + the Nyquist pencil
+    constexpr int EL = LZ * TL;
+    constexpr int PPT = (EL + NT - 1) / NT;
     constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
     constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
-    __shared__ cpx<T> buf[LZ * TB];
+    using KT = typename kspace_type<T>::type;
+    __shared__ cpx<T> buf[LZ * TL];
     __shared__ cpx<T> tws[LZ];
     __shared__ double red[NT];
     const int tid = threadIdx.x;
-    const int l0 = blockIdx.x * TB;
+    const int w = blockIdx.x;
     for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
-
-    // consecutive lines are consecutive in HBM for both kinds of pencils
-#define FHIP_LOAD_TILE(SRC)                                                    \
-    for (int e = tid; e < LZ * TB; e += NT) {                                  \
-        const int t = e / TB, b = e - t * TB;                                  \
-        const ZLine zl = zline(c, l0 + b);                                     \
-        buf[e] = zl.ok ? (SRC)[zl.base + (size_t)t * zl.stride] : cpx<T>{T(0), T(0)}; \
-    }
-#define FHIP_STORE_TILE(DST)                                                   \
-    for (int e = tid; e < LZ * TB; e += NT) {                                  \
-        const int t = e / TB, b = e - t * TB;                                  \
-        const ZLine zl = zline(c, l0 + b);                                     \
-        if (zl.ok) (DST)[zl.base + (size_t)t * zl.stride] = buf[e];            \
-    }
 
     if constexpr (GROUP == ZG_PLAIN) {
         for (int sl = 0; sl < a.nplain; ++sl) {
-            FHIP_LOAD_TILE(sp.s[sl]);
+            cpx<T>* s = sp.s[sl];
+            for (int e = tid; e < EL; e += NT) {
+                const ZElem z = zelem<LZ, TB>(c, w, e);
+                buf[z.lds] = z.ok ? s[z.kk] : cpx<T>{T(0), T(0)};
+            }
             __syncthreads();
-            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1>(buf, tws, tid);
-            else fft_lds<T, LZ, TB, NT, +1>(buf, tws, tid);
-            FHIP_STORE_TILE(sp.s[sl]);
+            if (a.plain_sign < 0) fft_lds<T, LZ, TL, NT, -1, TL>(buf, tws, tid);
+            else fft_lds<T, LZ, TL, NT, +1, TL>(buf, tws, tid);
+            for (int e = tid; e < EL; e += NT) {
+                const ZElem z = zelem<LZ, TB>(c, w, e);
+                if (z.ok) s[z.kk] = buf[z.lds];
+            }
             __syncthreads();
         }
         return;
     } else {
         cpx<T> v[NF][PPT];
-        // forward transforms, results kept in registers; the tile of the next
-        // field is fetched into registers while the current one is transformed
-        cpx<T> pf[PPT];
-        auto fetch = [&](const cpx<T>* __restrict__ src) {
+        // every field's tile is requested from HBM up front, so that one load
+        // latency covers the whole tile; the forward transforms then run
+        // field by field through LDS and the results stay in registers
+#pragma unroll
+        for (int fi = 0; fi < NF; ++fi) {
+            const cpx<T>* __restrict__ src = sp.s[F0 + fi];
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                pf[i] = cpx<T>{T(0), T(0)};
-                if ((LZ * TB) % NT == 0 || e < LZ * TB) {
-                    const int t = e / TB, b = e - t * TB;
-                    const ZLine zl = zline(c, l0 + b);
-                    if (zl.ok) pf[i] = src[zl.base + (size_t)t * zl.stride];
+                v[fi][i] = cpx<T>{T(0), T(0)};
+                if (EL % NT == 0 || e < EL) {
+                    const ZElem z = zelem<LZ, TB>(c, w, e);
+                    if (z.ok) v[fi][i] = src[z.kk];
                 }
             }
-        };
-        fetch(sp.s[F0]);
+        }
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi) {
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if ((LZ * TB) % NT == 0 || e < LZ * TB) buf[e] = pf[i];
+                if (EL % NT == 0 || e < EL) buf[zelem<LZ, TB>(c, w, e).lds] = v[fi][i];
             }
             __syncthreads();
-            if (fi + 1 < NF) fetch(sp.s[F0 + fi + 1]);
-            fft_lds<T, LZ, TB, NT, -1>(buf, tws, tid);
+            fft_lds<T, LZ, TL, NT, -1, TL>(buf, tws, tid);
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if ((LZ * TB) % NT == 0 || e < LZ * TB) v[fi][i] = buf[e];
+                if (EL % NT == 0 || e < EL) v[fi][i] = buf[zelem<LZ, TB>(c, w, e).lds];
             }
             __syncthreads();
         }
@@ -448,176 +659,15 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const int e = tid + i * NT;
-            if (!((LZ * TB) % NT == 0 || e < LZ * TB)) continue;
-            const int kz = e / TB, b = e - kz * TB;
-            const ZLine zl = zline(c, l0 + b);
-            if (!zl.ok) continue;
-            const int kx = zl.kx, ky = zl.ky;
-            const size_t kk = zl.base + (size_t)kz * zl.stride;
-            const bool edge = (kx == 0) || (kx == c.Mx);
-            if constexpr (GROUP == ZG_DD) {
-                // S(k): Fourier transform of the short-range couplings
-                // (dipole-dipole-short-range.F:35-83) in closed form.
-                const cpx<double> ex = a.csx[kx], ey = a.csy[ky], ez = a.csz[kz];
-                const double cx = ex.x, cy = ey.x, cz = ez.x, sx = ex.y, sy = ey.y, szz = ez.y;
-                const double j1 = c.j[0], j2 = c.j[1], j3 = c.j[2], j4 = c.j[3], j5 = c.j[4],
-                             j6 = c.j[5], j7 = c.j[6];
-                const double k2t = 2.0 * c.kappa2[0];
-                const double cxyz = cx * cy * cz;
-                double S[6];
-                S[0] = k2t + 2 * j2 * cx + 2 * j1 * (cy + cz) + 4 * j3 * cx * (cy + cz) + 4 * j4 * cy * cz + 8 * j6 * cxyz;
-                S[1] = k2t + 2 * j2 * cy + 2 * j1 * (cz + cx) + 4 * j3 * cy * (cz + cx) + 4 * j4 * cz * cx + 8 * j6 * cxyz;
-                S[2] = k2t + 2 * j2 * cz + 2 * j1 * (cx + cy) + 4 * j3 * cz * (cx + cy) + 4 * j4 * cx * cy + 8 * j6 * cxyz;
-                S[3] = -4 * j5 * sy * szz - 8 * j7 * sy * szz * cx;   // yz
-                S[4] = -4 * j5 * szz * sx - 8 * j7 * szz * sx * cy;   // zx
-                S[5] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz;     // xy
-                double L[6];
-                const T* lt = a.Ltab + kk * 6;
+            if (!(EL % NT == 0 || e < EL)) continue;
+            const ZElem z = zelem<LZ, TB>(c, w, e);
+            if (!z.ok) continue;
+            cpx<T> pv[NF];
 #pragma unroll
-                for (int m = 0; m < 6; ++m) L[m] = (double)lt[m] * c.Z2;
-                const cpx<double> ux = todbl(v[0][i]), uy = todbl(v[1][i]), uz = todbl(v[2][i]);
-                // F = (S + Z^2 L) u ; energies via Parseval with weight 1 (edge) or 2
-                cpx<double> fs[3], fl[3];
-                fs[0] = {S[0] * ux.x + S[5] * uy.x + S[4] * uz.x, S[0] * ux.y + S[5] * uy.y + S[4] * uz.y};
-                fs[1] = {S[5] * ux.x + S[1] * uy.x + S[3] * uz.x, S[5] * ux.y + S[1] * uy.y + S[3] * uz.y};
-                fs[2] = {S[4] * ux.x + S[3] * uy.x + S[2] * uz.x, S[4] * ux.y + S[3] * uy.y + S[2] * uz.y};
-                fl[0] = {L[0] * ux.x + L[5] * uy.x + L[4] * uz.x, L[0] * ux.y + L[5] * uy.y + L[4] * uz.y};
-                fl[1] = {L[5] * ux.x + L[1] * uy.x + L[3] * uz.x, L[5] * ux.y + L[1] * uy.y + L[3] * uz.y};
-                fl[2] = {L[4] * ux.x + L[3] * uy.x + L[2] * uz.x, L[4] * ux.y + L[3] * uy.y + L[2] * uz.y};
-                const double wgt = edge ? 1.0 : 2.0;
-                double es = 0.0, el = 0.0;
+            for (int fi = 0; fi < NF; ++fi) pv[fi] = v[fi][i];
+            kspace_point<KT, GROUP>(c, a, sc, pv, z, acc0, acc1, acc2);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const cpx<double> uq = todbl(v[q][i]);
-                    es += uq.x * fs[q].x + uq.y * fs[q].y;
-                    el += uq.x * fl[q].x + uq.y * fl[q].y;
-                    v[q][i] = cpx<T>{T(fs[q].x + fl[q].x), T(fs[q].y + fl[q].y)};
-                }
-                acc0 += wgt * es;
-                acc1 += wgt * el;
-            } else {
-                double k[3];
-                kvec(c, kx, ky, kz, k);
-                const bool k0 = (kx == 0 && ky == 0 && kz == 0);
-                if (k0) {
-#pragma unroll
-                    for (int m = 0; m < 6; ++m) sc->dc[m] = (double)v[m][i].x;
-                }
-                // coupling rhs: c_mod + C uu  (optimize-inho-strain.F:36-42, dVdac.F:46-52)
-                cpx<double> cm[3], Cu[3];
-                cpx<double> mk = {0.0, 0.0};
-                if (c.modulation) {
-                    const cpx<T> m0 = a.modk[kk];
-                    mk = {(double)m0.x, (double)m0.y};
-                }
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double fq = c.mod_c * k[q];
-                    cm[q] = c.modulation ? cpx<double>{fq * mk.x, fq * mk.y} : cpx<double>{0.0, 0.0};
-                    double rr = 0.0, ri = 0.0;
-#pragma unroll
-                    for (int m = 0; m < 6; ++m) {
-                        const double cq = coup2(c, k, q, m);
-                        rr = rr + cq * (double)v[m][i].x;
-                        ri = ri + cq * (double)v[m][i].y;
-                    }
-                    Cu[q] = {rr, ri};
-                }
-                // K(k) (elastic.F:90-104)
-                double K[3][3];
-#pragma unroll
-                for (int bb = 0; bb < 3; ++bb) {
-                    const int o1 = (bb + 2) % 3, o2 = (bb + 1) % 3;
-#pragma unroll
-                    for (int aa = 0; aa < 3; ++aa) {
-                        if (aa == bb)
-                            K[aa][bb] = c.B11 * k[aa] * k[aa] + c.B44 * k[o1] * k[o1] + c.B44 * k[o2] * k[o2];
-                        else
-                            K[aa][bb] = c.B12 * k[aa] * k[bb] + c.B44 * k[aa] * k[bb];
-                    }
-                }
-                cpx<double> w[3];
-                if constexpr (GROUP == ZG_STRAIN) {
-                    if (k0) {
-                        w[0] = w[1] = w[2] = cpx<double>{0.0, 0.0};
-                    } else {
-                        // Cholesky K = G G^T (elastic.F:131-141), then the two
-                        // triangular solves of optimize-inho-strain.F:44-57
-                        double G[3][3], Gi[3];
-#pragma unroll
-                        for (int bb = 0; bb < 3; ++bb) {
-                            double s2 = 0.0;
-#pragma unroll
-                            for (int l2 = 0; l2 < bb; ++l2) s2 += G[bb][l2] * G[bb][l2];
-                            G[bb][bb] = sqrt(K[bb][bb] - s2);
-                            Gi[bb] = 1.0 / G[bb][bb];
-#pragma unroll
-                            for (int aa = bb + 1; aa < 3; ++aa) {
-                                double s3 = 0.0;
-#pragma unroll
-                                for (int l2 = 0; l2 < bb; ++l2) s3 += G[aa][l2] * G[bb][l2];
-                                G[aa][bb] = (K[aa][bb] - s3) * Gi[bb];
-                            }
-                        }
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) w[q] = {-cm[q].x - Cu[q].x, -cm[q].y - Cu[q].y};
-#pragma unroll
-                        for (int aa = 0; aa < 3; ++aa) {
-                            double tr = 0.0, ti = 0.0;
-#pragma unroll
-                            for (int bb = 0; bb < aa; ++bb) { tr += G[aa][bb] * w[bb].x; ti += G[aa][bb] * w[bb].y; }
-                            w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
-                        }
-#pragma unroll
-                        for (int aa = 2; aa >= 0; --aa) {
-                            double tr = 0.0, ti = 0.0;
-#pragma unroll
-                            for (int bb = aa + 1; bb < 3; ++bb) { tr += w[bb].x * G[bb][aa]; ti += w[bb].y * G[bb][aa]; }
-                            w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
-                        }
-                    }
-                    if (a.acouK) {
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) a.acouK[(size_t)q * c.Kh + kk] = w[q];
-                    }
-                } else {
-                    // acoustic MD: w(k) = i FFT(w) (acouFFT.F:19-24);
-                    // dV/dw = -i [c_mod + C uu + K w] (dVdac.F:44-54)
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) w[q] = {-(double)v[6 + q][i].y, (double)v[6 + q][i].x};
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        double kr = 0.0, ki = 0.0;
-#pragma unroll
-                        for (int bb = 0; bb < 3; ++bb) { kr += K[q][bb] * w[bb].x; ki += K[q][bb] * w[bb].y; }
-                        const double vr = ((0.0 + cm[q].x) + Cu[q].x) + kr;
-                        const double vi = ((0.0 + cm[q].y) + Cu[q].y) + ki;
-                        v[6 + q][i] = cpx<T>{T(vi), T(-vr)};
-                    }
-                }
-                // energies with weight 1/2 on the kx = 0 and kx = Lx/2 planes
-                // (energy_module.F:162-239)
-                const double h = edge ? 0.5 : 1.0;
-                double eis = 0.0, eic = 0.0, eim = 0.0;
-#pragma unroll
-                for (int aa = 0; aa < 3; ++aa) {
-#pragma unroll
-                    for (int bb = 0; bb < 3; ++bb)
-                        eis += K[aa][bb] * (w[aa].x * w[bb].x + w[aa].y * w[bb].y);
-                    eic += w[aa].x * Cu[aa].x + w[aa].y * Cu[aa].y;
-                    eim += w[aa].x * cm[aa].x + w[aa].y * cm[aa].y;
-                }
-                acc0 += h * eis;
-                acc1 += h * eic;
-                acc2 += h * eim;
-                // eta(k) = k (x) w (dVddi_inho_coupling.F:41-47)
-                v[0][i] = cpx<T>{T(k[0] * w[0].x), T(k[0] * w[0].y)};
-                v[1][i] = cpx<T>{T(k[1] * w[1].x), T(k[1] * w[1].y)};
-                v[2][i] = cpx<T>{T(k[2] * w[2].x), T(k[2] * w[2].y)};
-                v[3][i] = cpx<T>{T(k[2] * w[1].x + k[1] * w[2].x), T(k[2] * w[1].y + k[1] * w[2].y)};
-                v[4][i] = cpx<T>{T(k[0] * w[2].x + k[2] * w[0].x), T(k[0] * w[2].y + k[2] * w[0].y)};
-                v[5][i] = cpx<T>{T(k[1] * w[0].x + k[0] * w[1].x), T(k[1] * w[0].y + k[0] * w[1].y)};
-            }
+            for (int fi = 0; fi < NF; ++fi) v[fi][i] = pv[fi];
         }
         // inverse transforms
 #pragma unroll
@@ -625,15 +675,21 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if ((LZ * TB) % NT == 0 || e < LZ * TB) buf[e] = v[fi][i];
+                if (EL % NT == 0 || e < EL) buf[zelem<LZ, TB>(c, w, e).lds] = v[fi][i];
             }
             __syncthreads();
-            fft_lds<T, LZ, TB, NT, +1>(buf, tws, tid);
-            FHIP_STORE_TILE(sp.s[F0 + fi]);
+            fft_lds<T, LZ, TL, NT, +1, TL>(buf, tws, tid);
+            cpx<T>* __restrict__ dst = sp.s[F0 + fi];
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                const int e = tid + i * NT;
+                if (EL % NT == 0 || e < EL) {
+                    const ZElem z = zelem<LZ, TB>(c, w, e);
+                    if (z.ok) dst[z.kk] = buf[z.lds];
+                }
+            }
             __syncthreads();
         }
-#undef FHIP_LOAD_TILE
-#undef FHIP_STORE_TILE
         const int nb = a.maxblocks;
         if constexpr (GROUP == ZG_DD) {
             double r0 = block_sum(acc0, red), r1 = block_sum(acc1, red);
@@ -684,32 +740,29 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
     const int row0 = blockIdx.x * RB;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
-    // half-spectrum points of this block's rows, read once each; the pair of
-    // fields after the current one is fetched while the current one transforms
+    // half-spectrum points of this block's rows: all NF fields are requested
+    // from HBM up front, so that one load latency covers the block
     constexpr int HP = RB * (MX + 1);
     constexpr int HPT = (HP + NT - 1) / NT;
-    cpx<T> pa[HPT], pb[HPT];
-    auto fetch = [&](int f0) {
-        const bool two = (f0 + 1 < NF);
+    cpx<T> h[NF][HPT];
+#pragma unroll
+    for (int fi = 0; fi < NF; ++fi) {
 #pragma unroll
         for (int i = 0; i < HPT; ++i) {
             const int e = tid + i * NT;
-            pa[i] = cpx<T>{T(0), T(0)};
-            pb[i] = cpx<T>{T(0), T(0)};
+            h[fi][i] = cpx<T>{T(0), T(0)};
             if (HP % NT != 0 && e >= HP) continue;
             int r, kx;
             if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
             else { r = e - RB * MX; kx = MX; }
             if (row0 + r < nrows) {
-                const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : c.Mh + (size_t)(row0 + r);
-                pa[i] = sp.s[f0][o];
-                if (two) pb[i] = sp.s[f0 + 1][o];
+                const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : (size_t)c.Kh + (size_t)(row0 + r);
+                h[fi][i] = sp.s[fi][o];
             }
         }
-    };
+    }
 
-    double acc[CPT][NF];
-    fetch(0);
+    T acc[CPT][NF];
 #pragma unroll
     for (int f0 = 0; f0 < NF; f0 += 2) {
         const bool two = (f0 + 1 < NF);
@@ -722,13 +775,12 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
             int r, kx;
             if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
             else { r = e - RB * MX; kx = MX; }
-            cpx<T> A = pa[i], B = pb[i];
+            cpx<T> A = h[f0][i], B = two ? h[f0 + 1][i] : cpx<T>{T(0), T(0)};
             if (kx == 0 || kx == MX) { A.y = T(0); B.y = T(0); }
             buf[kx * ST + r] = cpx<T>{A.x - B.y, A.y + B.x};
             if (kx != 0 && kx != MX) buf[(LX - kx) * ST + r] = cpx<T>{A.x + B.y, B.x - A.y};
         }
         __syncthreads();
-        if (f0 + 2 < NF) fetch(f0 + 2);
         fft_lds<T, LX, RB, NT, +1, ST>(buf, tws, tid);
 #pragma unroll
         for (int i = 0; i < CPT; ++i) {
@@ -736,8 +788,8 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
             if (CELLS % NT != 0 && e >= CELLS) continue;
             const int r = e / LX, x = e - r * LX;
             const cpx<T> z = buf[x * ST + r];
-            acc[i][f0] = (double)z.x;
-            if (two) acc[i][f0 + 1] = (double)z.y;
+            acc[i][f0] = z.x;
+            if (two) acc[i][f0 + 1] = z.y;
         }
         __syncthreads();
     }
@@ -752,7 +804,7 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
         const size_t cc = (size_t)(row0 + r) * LX + x;
         if (a.mode == XI_GENERIC) {
 #pragma unroll
-            for (int q = 0; q < NF; ++q) a.gen[q][cc] = acc[i][q];
+            for (int q = 0; q < NF; ++q) a.gen[q][cc] = (double)acc[i][q];
             continue;
         }
         if constexpr (NF >= 9) {
@@ -764,7 +816,7 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 u[q] = f.u[q][cc];
-                F[q] = acc[i][q] * c.Ninv;                            // dVddi-dipole-dipole.F:34-38
+                F[q] = (double)acc[i][q] * c.Ninv;                    // dVddi-dipole-dipole.F:34-38
             }
             // on-site anharmonic terms (dVddi-unharmonic.F:34-65)
             const double d0 = u[0] * u[0], d1 = u[1] * u[1], d2 = u[2] * u[2];
@@ -805,7 +857,7 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
                 for (int m = 0; m < 6; ++m) {
                     double sum = 0.0;
 #pragma unroll
-                    for (int jj = 0; jj < 6; ++jj) sum = sum + acc[i][3 + jj] * c.Ninv * c.hc2[jj + 6 * m];
+                    for (int jj = 0; jj < 6; ++jj) sum = sum + (double)acc[i][3 + jj] * c.Ninv * c.hc2[jj + 6 * m];
                     ei[m] = sum;
                 }
                 F[0] = F[0] + 2 * ei[0] * u[0] + ei[4] * u[2] + ei[5] * u[1];
@@ -846,7 +898,7 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
                 const double mwinv = c.mwinv[s];
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double g = acc[i][9 + q] * c.Ninv;                  // dVdac.F:56-62
+                    const double g = (double)acc[i][9 + q] * c.Ninv;          // dVdac.F:56-62
                     if (a.mode == XI_LF || a.mode == XI_VS) {
                         const double pq = f.pw[q][cc] - c.dt * g;
                         kea += pq * pq * mwinv;
@@ -1216,8 +1268,8 @@ __global__ __launch_bounds__(NT) void k_ewald_recip(Consts c, EwaldArgs e, Spect
     } else {
         const size_t r = kk - c.Mh;
         ix = c.Mx;
-        iy = (int)(r % c.Ly);
-        iz = (int)(r / c.Ly);
+        iy = (int)(r / c.Lz);
+        iz = (int)(r % c.Lz);
     }
     double k3 = (double)iz / c.Lz; if (k3 > 0.5) k3 = k3 - 1.0;
     double k2 = (double)iy / c.Ly; if (k2 > 0.5) k2 = k2 - 1.0;
