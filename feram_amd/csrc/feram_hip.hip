@@ -41,9 +41,12 @@ int tile_rows(int n, int cap = 1024)   // must agree with tile_of<n, cap>::value
     return p;
 }
 
-// Lines per y/z tile of spectral type T (line_tile_cap<T>() on the device).
+// Lines per y tile of spectral type T (line_tile_cap<T>() on the device) and
+// per z tile (ZTile<T, n>::TB).
 template <typename T>
 int line_tile_rows(int n) { return tile_rows(n, 16384 / (int)sizeof(cpx<T>)); }
+template <typename T>
+int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX); }
 
 enum KernelId {
     K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
@@ -60,7 +63,6 @@ struct TypedSet {
     Spectra<T> sp{};
     cpx<T> *twx = nullptr, *twy = nullptr, *twz = nullptr;
     T *Lt = nullptr;                  // L(k) in the spectral type, 6 words per point
-    cpx<T> *modk = nullptr;           // FFT of the modulation field
 };
 
 struct fhip_ctx {
@@ -82,9 +84,10 @@ struct fhip_ctx {
     double *partials = nullptr;
     double *Ltab = nullptr;           // FP64 L(k), 6 words per point (master copy)
     cpx<double> *acouK = nullptr;
+    cpx<double> *modk = nullptr;      // FFT of the modulation field on the k array
     cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
     double *kvx = nullptr, *kvy = nullptr, *kvz = nullptr;
-    size_t Ks = 0;                    // spectral slot stride: Kh k-space points + the y-major Nyquist copy
+    size_t Ks = 0;                    // spectral slot stride: Mh packed points
     unsigned char *code = nullptr;
     std::vector<void *> allocs;
     int64_t bytes = 0;
@@ -266,10 +269,22 @@ int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a)
     const int nb = ctx->zblocks[tix<T>()];
     hipStream_t s = ctx->stream;
     const int id = (G == ZG_DD) ? K_ZDD : (G == ZG_PLAIN ? K_SETUP : K_ZSTRAIN);
+    // side buffer of the kx = 0 pencils: two NF-field values per point of up
+    // to (TB - 1) / Mx + 1 pencils per tile
+    const int nf = (G == ZG_DD) ? 3 : (G == ZG_STRAIN ? 6 : 9);
+    const int tz = ztile_rows<T>(ctx->Lz);
+    const ZArgs<T> za = a;
+    const size_t shm = (G == ZG_PLAIN) ? 0 : 2 * (size_t)((tz - 1) / ctx->Mx + 1) * ctx->Lz * nf * sizeof(cpx<T>);
+    if (shm > 96 * 1024)
+        return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
     switch (ctx->Lz) {
 #define FHIP_CASE(n)                                                                           \
     case n:                                                                                    \
-        return launch(ctx, id, [&] { k_zpass<T, n, G><<<nb, NT, 0, s>>>(c, sp, a, sc, tw); });
+        if (shm > 64 * 1024 &&                                                                 \
+            hipFuncSetAttribute((const void *)k_zpass<T, n, G>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)shm) != hipSuccess)                                       \
+            return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: side buffer exceeds the LDS limit"); \
+        return launch(ctx, id, [&] { k_zpass<T, n, G><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -287,7 +302,7 @@ ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
     a.kvx = ctx->kvx;
     a.kvy = ctx->kvy;
     a.kvz = ctx->kvz;
-    a.modk = tset<T>(ctx).modk;
+    a.modk = ctx->modk;
     a.acouK = want_acouK ? ctx->acouK : nullptr;
     a.partials = ctx->partials;
     a.maxblocks = ctx->maxblocks;
@@ -420,12 +435,6 @@ void fill_consts(fhip_ctx *ctx)
     Consts &c = ctx->c;
     c.Lx = p.Lx; c.Ly = p.Ly; c.Lz = p.Lz;
     c.Mx = ctx->Mx; c.Mh = ctx->Mh; c.N = ctx->N; c.Kh = ctx->Kh;
-    {
-        // whole y tiles of the pipeline's spectral type; the FP32 tile is a
-        // multiple of the FP64 one, so FP64 set-up transforms share the layout
-        const int ty = p.precision == FHIP_FP32 ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
-        c.ylines_main = (ctx->Mx * ctx->Lz + ty - 1) / ty * ty;
-    }
     c.acoustic = p.mass_acou[0] > 0.0;
     c.efield = !p.E_field_always_zero;
     c.has_eloc = 0;
@@ -678,17 +687,16 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     ctx->Mh = (size_t)ctx->Mx * p->Ly * p->Lz;
     ctx->N = p->Lx * p->Ly * p->Lz;
     ctx->Kh = ctx->Kx * p->Ly * p->Lz;
-    ctx->Ks = (size_t)ctx->Kh + (size_t)p->Ly * p->Lz;
+    ctx->Ks = ctx->Mh;
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     fill_consts(ctx);
     ctx->nslots = ctx->c.acoustic ? 12 : 9;
     ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
     for (int t = 0; t < 2; ++t) {
         const int ty = t ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
-        const int tz = t ? line_tile_rows<float>(ctx->Lz) : line_tile_rows<double>(ctx->Lz);
-        ctx->yblocks[t] = (ctx->c.ylines_main + ctx->Lz + ty - 1) / ty;
-        // z tiles: TB main pencils each, the Ly Nyquist pencils folded in one per tile
-        ctx->zblocks[t] = std::max((ctx->Mx * ctx->Ly + tz - 1) / tz, ctx->Ly);
+        const int tz = t ? ztile_rows<float>(ctx->Lz) : ztile_rows<double>(ctx->Lz);
+        ctx->yblocks[t] = (ctx->Mx * ctx->Lz + ty - 1) / ty;
+        ctx->zblocks[t] = (ctx->Mx * ctx->Ly + tz - 1) / tz;
     }
     ctx->kblocks = (ctx->N + NT - 1) / NT;
     ctx->maxblocks = std::max(std::max(ctx->xblocks, std::max(ctx->zblocks[0], ctx->zblocks[1])), ctx->kblocks);
@@ -761,8 +769,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     if (p->modulation_constant != 0.0) {
         // the reference stops when modulation_constant is set without a .modulation
         // file (elastic.F:226-233); the table is filled by fhip_set_modulation.
-        RC(dalloc(ctx, &ctx->dset.modk, Kh));
-        if (ctx->precision == FHIP_FP32) RC(dalloc(ctx, &ctx->fset.modk, Kh));
+        RC(dalloc(ctx, &ctx->modk, Kh));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return FHIP_OK;
@@ -833,20 +840,14 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
         SetupSlots slots{ctx};
         RC(slots.open(1));
         RC(forward_generic(ctx, flds, 1));
-        HIPCHK(hipMemcpyAsync(ctx->dset.modk, ctx->dset.sp.s[0], sizeof(cpx<double>) * ctx->Kh,
-                              hipMemcpyDeviceToDevice, ctx->stream));
+        Consts c = ctx->c;
+        const cpx<double> *s0 = ctx->dset.sp.s[0];
+        cpx<double> *mk = ctx->modk;
+        hipStream_t st = ctx->stream;
+        const int kb = (ctx->Kh + NT - 1) / NT;
+        RC(launch(ctx, K_SETUP, [&] { k_unpack<double><<<kb, NT, 0, st>>>(c, s0, mk); }));
     }
     HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * ctx->N, ctx->stream));
-    if (ctx->precision == FHIP_FP32) {
-        std::vector<cpx<double>> md(ctx->Kh);
-        std::vector<cpx<float>> mf(ctx->Kh);
-        HIPCHK(hipMemcpyAsync(md.data(), ctx->dset.modk, sizeof(cpx<double>) * ctx->Kh, hipMemcpyDeviceToHost,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < md.size(); ++i) mf[i] = cpx<float>{(float)md[i].x, (float)md[i].y};
-        HIPCHK(hipMemcpyAsync(ctx->fset.modk, mf.data(), sizeof(cpx<float>) * ctx->Kh, hipMemcpyHostToDevice,
-                              ctx->stream));
-    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->c.modulation = 1;
     return collect_events(ctx);
@@ -1139,7 +1140,7 @@ static int reconstruct_w_t(fhip_ctx *ctx, std::vector<double> &h)
     Spectra<T> sp = tset<T>(ctx).sp;
     const cpx<double> *aK = ctx->acouK;
     hipStream_t s = ctx->stream;
-    const int kb = (ctx->Kh + NT - 1) / NT;
+    const int kb = (int)((ctx->Mh + NT - 1) / NT);
     int rc = launch(ctx, K_SETUP, [&] { k_prep_w<T><<<kb, NT, 0, s>>>(c, sp, aK); });
     if (rc == FHIP_OK) {
         ZArgs<T> za = zargs<T>(ctx, false);

@@ -2,15 +2,15 @@
 //
 // Device layout (structure of arrays, unpadded, x fastest):
 //   real fields   f[c],  c = x + Lx*(y + Ly*z)                 (one array per component)
-//   half spectra  Mx = Lx/2 columns kx = 0..Mx-1 in rows of Mx points,
-//                 s[(kz*Ly + ky)*Mx + kx], Mh = Mx*Ly*Lz points, followed by
-//                 the Nyquist column kx = Mx in two orders: z-major
-//                 s[Mh + ky*Lz + kz] (the k-space copy, so that every point
-//                 of the half spectrum has an index kk < Kh = Mh + Ly*Lz) and
-//                 y-major s[Kh + z*Ly + y] (the staging copy the x passes
-//                 write and read).  The y pass converts between the two.
+//   spectra       packed half spectra of Mx = Lx/2 columns, s[(kz*Ly + ky)*Mx + kx],
+//                 Mh = Mx*Ly*Lz = N/2 points: the real kx = 0 and kx = Lx/2
+//                 values of every x row share column 0 as A[0] + i A[Lx/2]
+//                 (the y and z passes describe how that column is carried).
 //                 Rows of Mx points keep every tile of the y and z passes
 //                 aligned to whole cache lines.
+//   k arrays      L(k), the modulation spectrum and w(k) hold all
+//                 (Mx+1)*Ly*Lz = Kh points of the half spectrum: kx < Mx at
+//                 (kz*Ly + ky)*Mx + kx, kx = Mx at Mh + ky*Lz + kz.
 // Spectral slots of one step:
 //   0..2  u_x,u_y,u_z        -> (S + Z*^2 L) u   (dipole-dipole force before 1/N)
 //   3..8  uu_xx..uu_xy       -> eta(k) = k (x) w(k)  (inhomogeneous strain)
@@ -34,7 +34,6 @@ enum PartialSlot {
 struct Consts {
     int Lx, Ly, Lz, Mx, N, Kh;
     size_t Mh;           // Mx*Ly*Lz: start of the z-major Nyquist column (Kh: the y-major one)
-    int ylines_main;     // Mx*Lz rounded up to whole y tiles
     int acoustic;        // mass_acou > 0: w is dynamical
     int efield;          // !E_field_always_zero
     int has_eloc;        // per-cell local field present
@@ -89,6 +88,11 @@ __device__ __forceinline__ cpx<double> todbl(cpx<T> a) { return cpx<double>{(dou
 __host__ __device__ __forceinline__ size_t kindex(int Mx, int Ly, int Lz, size_t Mh, int kx, int ky, int kz)
 {
     return kx < Mx ? ((size_t)kz * Ly + ky) * Mx + kx : Mh + (size_t)ky * Lz + kz;
+}
+
+__device__ __forceinline__ int kindex32(const Consts& c, int kx, int ky, int kz)
+{
+    return kx < c.Mx ? (kz * c.Ly + ky) * c.Mx + kx : (int)c.Mh + ky * c.Lz + kz;
 }
 
 // One component of the k vector of index i along an axis of length L
@@ -242,36 +246,53 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
         }
         __syncthreads();
         fft_lds<T, LX, RB, NT, -1, ST>(buf, tws, tid);
-        // unpack Z = A + iB: A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] - conj Z[-k])/(2i)
-        for (int e = tid; e < RB * (MX + 1); e += NT) {
-            int r, kx;
-            if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
-            else { r = e - RB * MX; kx = MX; }
+        // unpack Z = A + iB: A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] - conj Z[-k])/(2i);
+        // the real A[0] and A[Lx/2] are packed as A[0] + i A[Lx/2] at kx = 0
+        for (int e = tid; e < RB * MX; e += NT) {
+            const int r = e / MX, kx = e - r * MX;
             if (row0 + r >= nrows) continue;
             const cpx<T> zk = buf[kx * ST + r];
-            const cpx<T> zn = buf[((LX - kx) % LX) * ST + r];
-            const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : (size_t)c.Kh + (size_t)(row0 + r);
-            sp.s[f0][o] = cpx<T>{(zk.x + zn.x) * T(0.5), (zk.y - zn.y) * T(0.5)};
-            if (two) sp.s[f0 + 1][o] = cpx<T>{(zk.y + zn.y) * T(0.5), (zn.x - zk.x) * T(0.5)};
+            const cpx<T> zn = buf[(kx == 0 ? MX : LX - kx) * ST + r];
+            const size_t o = (size_t)(row0 + r) * MX + kx;
+            if (kx == 0) {
+                sp.s[f0][o] = cpx<T>{zk.x, zn.x};
+                if (two) sp.s[f0 + 1][o] = cpx<T>{zk.y, zn.y};
+            } else {
+                sp.s[f0][o] = cpx<T>{(zk.x + zn.x) * T(0.5), (zk.y - zn.y) * T(0.5)};
+                if (two) sp.s[f0 + 1][o] = cpx<T>{(zk.y + zn.y) * T(0.5), (zn.x - zk.x) * T(0.5)};
+            }
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Line transform along y, one field per blockIdx.y.  Tiles below
-// ylines_main hold TB columns (z, kx<Mx) of stride Mx, transformed in place.
-// Tiles from ylines_main on hold the Nyquist columns of TB planes z: the
-// forward transform reads the y-major staging copy and writes the z-major
-// k-space copy, the inverse transform the other way round.  Both copies are
-// read and written in whole rows of TB points (consecutive z) or whole
-// columns (consecutive y).
+// Line transform along y, in place, one field per blockIdx.y: TB columns
+// (z, kx) of stride Mx per tile.  The kx = 0 column carries C = P + iQ, the
+// y transform of the packed real kx = 0 and kx = Lx/2 values.  The forward
+// transform splits it into P(ky) at ky < Ly/2 and Q(ky) at Ly - ky, leaving
+// P(0) + iQ(0) and P(Ly/2) + iQ(Ly/2) (all four real) packed at 0 and Ly/2;
+// the inverse transform reassembles C.
 // ---------------------------------------------------------------------------
 
 // Tiles of the y and z passes hold 16 KB: 8 FP64 or 16 FP32 complex lines at
 // N = 128, so that every tile row is a whole 128-byte cache line.
 template <typename T>
 constexpr int line_tile_cap() { return 16384 / (int)sizeof(cpx<T>); }
+
+// Pairs (v at t, m at L - t) of a packed line: split C into P (t < L/2) or
+// Q(L - t) (t > L/2), and the reverse.
+template <typename T>
+__device__ __forceinline__ cpx<T> split_pq(cpx<T> v, cpx<T> m, bool lower)
+{
+    return lower ? cpx<T>{(v.x + m.x) * T(0.5), (v.y - m.y) * T(0.5)}
+                 : cpx<T>{(m.y + v.y) * T(0.5), (v.x - m.x) * T(0.5)};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> join_pq(cpx<T> v, cpx<T> m, bool lower)
+{
+    return lower ? cpx<T>{v.x - m.y, v.y + m.x} : cpx<T>{m.x + v.y, v.x - m.y};
+}
 
 template <typename T, int LY, int SIGN>
 __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0,
@@ -283,60 +304,45 @@ __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0
     const int tid = threadIdx.x;
     for (int i = tid; i < LY; i += NT) tws[i] = twy[i];
     const int l0 = blockIdx.x * TB;
+    const int nlines = c.Mx * c.Lz;
     cpx<T>* s = sp.s[slot0 + blockIdx.y];
-    if (l0 < c.ylines_main) {
-        const int nmain = c.Mx * c.Lz;
-        for (int e = tid; e < LY * TB; e += NT) {
-            const int t = e / TB, b = e - t * TB, l = l0 + b;
-            if (l < nmain) {
-                const int z = l / c.Mx, kx = l - z * c.Mx;
-                buf[e] = s[((size_t)z * c.Ly + t) * c.Mx + kx];
-            }
-        }
-        __syncthreads();
-        fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
-        for (int e = tid; e < LY * TB; e += NT) {
-            const int t = e / TB, b = e - t * TB, l = l0 + b;
-            if (l < nmain) {
-                const int z = l / c.Mx, kx = l - z * c.Mx;
-                s[((size_t)z * c.Ly + t) * c.Mx + kx] = buf[e];
-            }
-        }
-        return;
-    }
-    const int z0 = l0 - c.ylines_main;
-    cpx<T>* ymaj = s + c.Kh;          // [z][y]
-    cpx<T>* zmaj = s + c.Mh;          // [ky][z]
-    // element (t, b): point t of the Nyquist column of plane z0 + b
     for (int e = tid; e < LY * TB; e += NT) {
-        if (SIGN < 0) {
-            const int b = e / LY, t = e - b * LY;
-            if (z0 + b < c.Lz) buf[t * TB + b] = ymaj[(size_t)(z0 + b) * LY + t];
-        } else {
-            const int t = e / TB, b = e - t * TB;
-            if (z0 + b < c.Lz) buf[e] = zmaj[(size_t)t * c.Lz + z0 + b];
+        const int t = e / TB, b = e - t * TB, l = l0 + b;
+        if (l < nlines) {
+            const int z = l / c.Mx, kx = l - z * c.Mx;
+            const size_t col = (size_t)z * LY * c.Mx + kx;
+            cpx<T> v = s[col + (size_t)t * c.Mx];
+            if (SIGN > 0 && kx == 0 && t != 0 && 2 * t != LY)
+                v = join_pq(v, s[col + (size_t)(LY - t) * c.Mx], 2 * t < LY);
+            buf[e] = v;
         }
     }
     __syncthreads();
     fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
     for (int e = tid; e < LY * TB; e += NT) {
-        if (SIGN < 0) {
-            const int t = e / TB, b = e - t * TB;
-            if (z0 + b < c.Lz) zmaj[(size_t)t * c.Lz + z0 + b] = buf[e];
-        } else {
-            const int b = e / LY, t = e - b * LY;
-            if (z0 + b < c.Lz) ymaj[(size_t)(z0 + b) * LY + t] = buf[t * TB + b];
+        const int t = e / TB, b = e - t * TB, l = l0 + b;
+        if (l < nlines) {
+            const int z = l / c.Mx, kx = l - z * c.Mx;
+            cpx<T> v = buf[e];
+            if (SIGN < 0 && kx == 0 && t != 0 && 2 * t != LY) v = split_pq(v, buf[(LY - t) * TB + b], 2 * t < LY);
+            s[((size_t)z * LY + t) * c.Mx + kx] = v;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// z pass: forward transform along z, k-space stage, inverse transform.
-// Workgroup w holds TB main pencils (ky, kx<Mx), lines l = w*TB + b of
-// stride Mx*Ly, and, as line TB of its tile, the Nyquist pencil ky = w
-// (kx = Mx) of the z-major copy, contiguous along kz.  Folding the Ly
-// Nyquist pencils into the main tiles keeps the number of workgroups at the
-// number of main tiles, so that the grid fits the resident workgroups.
+// z pass: forward transform along z, k-space stage, inverse transform, on
+// tiles of TB pencils (ky, kx) of stride Mx*Ly.  Pencils with kx = 0 carry,
+// after the y pass, P(ky) (ky < Ly/2: the points (0, ky, kz)), Q(Ly - ky)
+// (ky > Ly/2: the points (Lx/2, Ly - ky, kz)), or, at ky = 0 and Ly/2, R =
+// P + iQ with both real before the z transform; the k-space stage splits R
+// into P(kz) (kz < Lz/2) and Q(Lz - kz) (kz > Lz/2) and joins it back.
+//
+// Every stored point stands for itself and its mirror partner -k.  The
+// reference evaluates both on its half spectrum and its c2r keeps the
+// Hermitian part; where the partner's k vector is not exactly -k (the
+// kx = Lx/2 plane, ky = Ly/2, kz = Lz/2) the strain stage is evaluated at
+// both and averaged, elsewhere once with the partner's energy share.
 // ---------------------------------------------------------------------------
 enum ZGroup { ZG_PLAIN = 0, ZG_DD = 1, ZG_STRAIN = 2, ZG_ACOU = 3 };
 
@@ -346,48 +352,16 @@ struct ZArgs {
     const cpx<double>* csx;          // cos/sin tables (2 pi i / L) per axis
     const cpx<double>* csy;
     const cpx<double>* csz;
-    const double* kvx;               // k components per axis (kcomp)
+    const double* kvx;               // k components per axis (kcomp), Mx + 1 along x
     const double* kvy;
     const double* kvz;
-    const cpx<T>* modk;              // FFT of the modulation field (or null)
-    cpx<double>* acouK;              // materialised optimised w(k) (or null)
+    const cpx<double>* modk;         // FFT of the modulation field on the k array (or null)
+    cpx<double>* acouK;              // materialised optimised w(k) on the k array (or null)
     double* partials;                // [P_NSLOT][maxblocks]
     int maxblocks;
     int nplain;                      // ZG_PLAIN: number of slots
     int plain_sign;                  // ZG_PLAIN: -1 forward, +1 inverse
 };
-
-// Element e of the tile of workgroup w: its point of the half spectrum and
-// its place in the LDS tile (line b of TB+1 at t*(TB+1) + b).
-struct ZElem {
-    size_t kk;
-    int kx, ky, kz, lds;
-    bool ok;
-};
-
-template <int LZ, int TB>
-__device__ __forceinline__ ZElem zelem(const Consts& c, int w, int e)
-{
-    ZElem z;
-    if (e < LZ * TB) {
-        const int t = e / TB, b = e - t * TB, l = w * TB + b;
-        z.ok = l < c.Mx * c.Ly;
-        z.ky = l / c.Mx;
-        z.kx = l - z.ky * c.Mx;
-        z.kz = t;
-        z.kk = (size_t)l + (size_t)t * ((size_t)c.Mx * c.Ly);
-        z.lds = t * (TB + 1) + b;
-    } else {
-        const int t = e - LZ * TB;
-        z.ok = w < c.Ly;
-        z.kx = c.Mx;
-        z.ky = w;
-        z.kz = t;
-        z.kk = c.Mh + (size_t)w * LZ + t;
-        z.lds = t * (TB + 1) + TB;
-    }
-    return z;
-}
 
 // coup2(a, i) (elastic.F:148-175)
 template <typename KT>
@@ -403,16 +377,22 @@ __device__ __forceinline__ KT coup2(const Consts& c, const KT k[3], int a, int i
     return (q < 0 ? KT(0) : k[q]) * KT(c.B4yz);
 }
 
-// k-space stage of one point, arithmetic in KT (FP64, or FP32 in FP32 mode).
-// pv holds the NF forward spectra of the point on entry and the spectra to
-// transform back on exit; acc0..2 collect the Parseval energy sums.
+// Modes of one evaluation (see kspace_elem): N, the point itself; C, its
+// mirror partner on the conjugate input, conjugated back; RP / RQ, the real
+// P or Q of an R pencil at kz = 0 or Lz/2, each its own partner.
+enum { EV_N = 0, EV_C = 1, EV_RP = 2, EV_RQ = 3 };
+
+// One evaluation of the k-space stage at the point (kx, ky, kz), k-array
+// index kk, arithmetic in KT, on the NF inputs X; outputs to O.  The point's
+// energies are added with weight wm times that of a point off the kx = 0
+// and kx = Lx/2 planes, and w(k) goes to acouK when it is materialised
+// (with -conj w at kkp, the implied partner, when kkp >= 0).
 template <typename KT, int GROUP, typename T, int NF>
-__device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a, Scalars* sc, cpx<T> (&pv)[NF],
-                                             const ZElem& z, double& acc0, double& acc1, double& acc2)
+__device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalars* sc, int kx, int ky, int kz,
+                                      int kk, int kkp, const cpx<KT> (&X)[NF], cpx<KT> (&O)[NF], double wm,
+                                      double& acc0, double& acc1, double& acc2)
 {
-    const int kx = z.kx, ky = z.ky, kz = z.kz;
-    const size_t kk = z.kk;
-    const bool edge = (kx == 0) || (kx == c.Mx);
+    auto put = [&](int q, cpx<KT> o) { O[q] = o; };
     if constexpr (GROUP == ZG_DD) {
         // S(k): Fourier transform of the short-range couplings
         // (dipole-dipole-short-range.F:35-83) in closed form.
@@ -434,9 +414,8 @@ __device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a,
         const KT Z2 = KT(c.Z2);
 #pragma unroll
         for (int m = 0; m < 6; ++m) L[m] = KT(lt[m]) * Z2;
-        const cpx<KT> ux = {KT(pv[0].x), KT(pv[0].y)}, uy = {KT(pv[1].x), KT(pv[1].y)},
-                      uz = {KT(pv[2].x), KT(pv[2].y)};
-        // F = (S + Z^2 L) u ; energies via Parseval with weight 1 (edge) or 2
+        const cpx<KT> ux = X[0], uy = X[1], uz = X[2];
+        // F = (S + Z^2 L) u ; energies via Parseval
         cpx<KT> fs[3], fl[3];
         fs[0] = {S[0] * ux.x + S[5] * uy.x + S[4] * uz.x, S[0] * ux.y + S[5] * uy.y + S[4] * uz.y};
         fs[1] = {S[5] * ux.x + S[1] * uy.x + S[3] * uz.x, S[5] * ux.y + S[1] * uy.y + S[3] * uz.y};
@@ -444,29 +423,27 @@ __device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a,
         fl[0] = {L[0] * ux.x + L[5] * uy.x + L[4] * uz.x, L[0] * ux.y + L[5] * uy.y + L[4] * uz.y};
         fl[1] = {L[5] * ux.x + L[1] * uy.x + L[3] * uz.x, L[5] * ux.y + L[1] * uy.y + L[3] * uz.y};
         fl[2] = {L[4] * ux.x + L[3] * uy.x + L[2] * uz.x, L[4] * ux.y + L[3] * uy.y + L[2] * uz.y};
-        const cpx<KT> uq[3] = {ux, uy, uz};
         KT es = KT(0), el = KT(0);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            es += uq[q].x * fs[q].x + uq[q].y * fs[q].y;
-            el += uq[q].x * fl[q].x + uq[q].y * fl[q].y;
-            pv[q] = cpx<T>{T(fs[q].x + fl[q].x), T(fs[q].y + fl[q].y)};
+            es += X[q].x * fs[q].x + X[q].y * fs[q].y;
+            el += X[q].x * fl[q].x + X[q].y * fl[q].y;
+            put(q, cpx<KT>{fs[q].x + fl[q].x, fs[q].y + fl[q].y});
         }
-        const double wgt = edge ? 1.0 : 2.0;
-        acc0 += wgt * (double)es;
-        acc1 += wgt * (double)el;
+        acc0 += 2.0 * wm * (double)es;
+        acc1 += 2.0 * wm * (double)el;
     } else {
         const KT k[3] = {KT(a.kvx[kx]), KT(a.kvy[ky]), KT(a.kvz[kz])};
         const bool k0 = (kx == 0 && ky == 0 && kz == 0);
         if (k0) {
 #pragma unroll
-            for (int m = 0; m < 6; ++m) sc->dc[m] = (double)pv[m].x;
+            for (int m = 0; m < 6; ++m) sc->dc[m] = (double)X[m].x;
         }
         // coupling rhs: c_mod + C uu  (optimize-inho-strain.F:36-42, dVdac.F:46-52)
         cpx<KT> cm[3], Cu[3];
         cpx<KT> mk = {KT(0), KT(0)};
         if (c.modulation) {
-            const cpx<T> m0 = a.modk[kk];
+            const cpx<double> m0 = a.modk[kk];
             mk = {KT(m0.x), KT(m0.y)};
         }
 #pragma unroll
@@ -477,8 +454,8 @@ __device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a,
 #pragma unroll
             for (int m = 0; m < 6; ++m) {
                 const KT cq = coup2<KT>(c, k, q, m);
-                rr = rr + cq * KT(pv[m].x);
-                ri = ri + cq * KT(pv[m].y);
+                rr = rr + cq * X[m].x;
+                ri = ri + cq * X[m].y;
             }
             Cu[q] = {rr, ri};
         }
@@ -536,16 +513,11 @@ __device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a,
                     w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
                 }
             }
-            if (a.acouK) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    a.acouK[(size_t)q * c.Kh + kk] = cpx<double>{(double)w[q].x, (double)w[q].y};
-            }
         } else {
             // acoustic MD: w(k) = i FFT(w) (acouFFT.F:19-24);
             // dV/dw = -i [c_mod + C uu + K w] (dVdac.F:44-54)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) w[q] = {-KT(pv[6 + q].y), KT(pv[6 + q].x)};
+            for (int q = 0; q < 3; ++q) w[q] = {-X[6 + q].y, X[6 + q].x};
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 KT kr = KT(0), ki = KT(0);
@@ -553,7 +525,17 @@ __device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a,
                 for (int bb = 0; bb < 3; ++bb) { kr += K[q][bb] * w[bb].x; ki += K[q][bb] * w[bb].y; }
                 const KT vr = ((KT(0) + cm[q].x) + Cu[q].x) + kr;
                 const KT vi = ((KT(0) + cm[q].y) + Cu[q].y) + ki;
-                pv[6 + q] = cpx<T>{T(vi), T(-vr)};
+                put(6 + q, cpx<KT>{vi, -vr});
+            }
+        }
+        if constexpr (GROUP == ZG_STRAIN) {
+            if (a.acouK) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    a.acouK[(size_t)q * c.Kh + kk] = cpx<double>{(double)w[q].x, (double)w[q].y};
+                    if (kkp >= 0)                     // w(-k) = -conj w(k)
+                        a.acouK[(size_t)q * c.Kh + kkp] = cpx<double>{-(double)w[q].x, (double)w[q].y};
+                }
             }
         }
         // energies with weight 1/2 on the kx = 0 and kx = Lx/2 planes
@@ -567,60 +549,161 @@ __device__ __forceinline__ void kspace_point(const Consts& c, const ZArgs<T>& a,
             eic += w[aa].x * Cu[aa].x + w[aa].y * Cu[aa].y;
             eim += w[aa].x * cm[aa].x + w[aa].y * cm[aa].y;
         }
-        const double h = edge ? 0.5 : 1.0;
-        acc0 += h * (double)eis;
-        acc1 += h * (double)eic;
-        acc2 += h * (double)eim;
+        acc0 += wm * (double)eis;
+        acc1 += wm * (double)eic;
+        acc2 += wm * (double)eim;
         // eta(k) = k (x) w (dVddi_inho_coupling.F:41-47)
-        pv[0] = cpx<T>{T(k[0] * w[0].x), T(k[0] * w[0].y)};
-        pv[1] = cpx<T>{T(k[1] * w[1].x), T(k[1] * w[1].y)};
-        pv[2] = cpx<T>{T(k[2] * w[2].x), T(k[2] * w[2].y)};
-        pv[3] = cpx<T>{T(k[2] * w[1].x + k[1] * w[2].x), T(k[2] * w[1].y + k[1] * w[2].y)};
-        pv[4] = cpx<T>{T(k[0] * w[2].x + k[2] * w[0].x), T(k[0] * w[2].y + k[2] * w[0].y)};
-        pv[5] = cpx<T>{T(k[1] * w[0].x + k[0] * w[1].x), T(k[1] * w[0].y + k[0] * w[1].y)};
+        put(0, cpx<KT>{k[0] * w[0].x, k[0] * w[0].y});
+        put(1, cpx<KT>{k[1] * w[1].x, k[1] * w[1].y});
+        put(2, cpx<KT>{k[2] * w[2].x, k[2] * w[2].y});
+        put(3, cpx<KT>{k[2] * w[1].x + k[1] * w[2].x, k[2] * w[1].y + k[1] * w[2].y});
+        put(4, cpx<KT>{k[0] * w[2].x + k[2] * w[0].x, k[0] * w[2].y + k[2] * w[0].y});
+        put(5, cpx<KT>{k[1] * w[0].x + k[0] * w[1].x, k[1] * w[0].y + k[0] * w[1].y});
     }
+}
+
+// Evaluation plan of a point of a kx = 0 pencil (slot s, kz index t): one
+// evaluation, or two where the partner's k vector is not exactly -k.
+struct KPlan {
+    int nev, x0, y0, z0, m0, x1, y1, z1, m1;
+    double wm;                    // energy weight of each evaluation
+    int yp, zp;                   // implied partner (0, yp, zp) of a single evaluation (yp < 0: none)
+};
+
+// Input of an evaluation in mode md, and its output folded into the stored value.
+template <typename KT>
+__device__ __forceinline__ cpx<KT> ev_in(cpx<KT> x, int md)
+{
+    return md == EV_N ? x : md == EV_C ? cpx<KT>{x.x, -x.y} : md == EV_RP ? cpx<KT>{x.x, KT(0)} : cpx<KT>{x.y, KT(0)};
+}
+template <typename KT>
+__device__ __forceinline__ cpx<KT> ev_out(cpx<KT> acc, cpx<KT> o, int md, KT half)
+{
+    if (md == EV_N) return {acc.x + half * o.x, acc.y + half * o.y};
+    if (md == EV_C) return {acc.x + half * o.x, acc.y - half * o.y};
+    if (md == EV_RP) return {o.x, acc.y};
+    return {acc.x, o.x};
+}
+
+template <bool DD>
+__device__ __forceinline__ KPlan kplan(const Consts& c, int s, int t)
+{
+    const int Ly = c.Ly, Lz = c.Lz, Mx = c.Mx;
+    const int tp = (t == 0) ? 0 : Lz - t;          // kz of the mirror partner
+    KPlan p{1, 0, s, t, EV_N, 0, 0, 0, EV_C, 1.0, -1, 0};
+    if (s != 0 && 2 * s < Ly) {                    // P pencil: (0, s, t)
+        if (!DD && 2 * t == Lz) { p.nev = 2; p.y1 = Ly - s; p.z1 = t; p.wm = 0.5; }
+        else { p.yp = Ly - s; p.zp = tp; }
+    } else if (2 * s > Ly) {                       // Q pencil: (Lx/2, Ly - s, t)
+        p.x0 = Mx; p.y0 = Ly - s;
+        if (!DD) { p.nev = 2; p.x1 = Mx; p.y1 = s; p.z1 = tp; p.wm = 0.5; }
+    } else if (t == 0 || 2 * t == Lz) {            // R pencil, real P and Q
+        p.nev = 2; p.m0 = EV_RP; p.m1 = EV_RQ; p.x1 = Mx; p.y1 = s; p.z1 = t; p.wm = 0.5;
+    } else if (2 * t < Lz) {                       // R pencil, P(t): (0, s, t)
+        if (!DD && s != 0) { p.nev = 2; p.y1 = s; p.z1 = tp; p.wm = 0.5; }
+        else { p.yp = s; p.zp = tp; }
+    } else {                                       // R pencil, Q(Lz - t): (Lx/2, s, Lz - t)
+        p.x0 = Mx; p.z0 = Lz - t;
+        if (!DD) { p.nev = 2; p.x1 = Mx; p.y1 = s; p.z1 = t; p.wm = 0.5; }
+    }
+    return p;
 }
 
 // Arithmetic type of the k-space stage: FP64 with FP64 spectra, FP32 with FP32.
 template <typename T> struct kspace_type { using type = double; };
 template <> struct kspace_type<float> { using type = float; };
 
+// z-pass tile geometry: TB pencils of LZ points per tile (at most 64, which
+// bounds the kx = 0 pencils a tile holds on small supercells), LDS rows of
+// TB+1 (so that the line-order reads of the k-space stage hit every bank),
+// and the side buffer of the kx = 0 pencils, two NF-field values per point,
+// that the host sizes as dynamic LDS.
+constexpr int ZTILE_MAX = 64;
+template <typename T, int LZ>
+struct ZTile {
+    static constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value < ZTILE_MAX
+                                  ? tile_of<LZ, line_tile_cap<T>()>::value : ZTILE_MAX;
+    static constexpr int ST = TB + 1;
+    static constexpr int EL = LZ * TB;
+    static constexpr int PPT = (EL + NT - 1) / NT;
+};
+
 template <typename T, int LZ, int GROUP>
 __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
                                               const cpx<T>* __restrict__ twz)
 {
-    constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value;
-    constexpr int TL = TB + 1;                 // lines per tile: TB main + the Nyquist pencil
-    constexpr int EL = LZ * TL;
-    constexpr int PPT = (EL + NT - 1) / NT;
+    constexpr int TB = ZTile<T, LZ>::TB, ST = ZTile<T, LZ>::ST, EL = ZTile<T, LZ>::EL, PPT = ZTile<T, LZ>::PPT;
     constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
     constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
     using KT = typename kspace_type<T>::type;
-    __shared__ cpx<T> buf[LZ * TL];
+    __shared__ cpx<T> buf[LZ * ST];
     __shared__ cpx<T> tws[LZ];
     __shared__ double red[NT];
+    __shared__ int lkx[TB], lky[TB], lord[TB];    // per pencil: kx, ky, ordinal among the kx = 0 pencils (or -1)
+    extern __shared__ char zdyn[];
+    cpx<T>* side = reinterpret_cast<cpx<T>*>(zdyn);   // [ordinal*LZ + t][NF]
     const int tid = threadIdx.x;
     const int w = blockIdx.x;
+    const int nlines = c.Mx * c.Ly;
+    const size_t zstride = (size_t)nlines;
     for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
+
+    // element e of the tile in HBM order (b fastest, coalesced)
+    auto hbm_at = [&](int e, int& lds, size_t& off) -> bool {
+        const int t = e / TB, b = e - t * TB, l = w * TB + b;
+        lds = t * ST + b;
+        off = (size_t)l + (size_t)t * zstride;
+        return l < nlines;
+    };
+    // element tid + i*NT in HBM order: a per-thread base plus a uniform step in i
+    auto hbm_i = [&](int i, int& lds, size_t& off) -> bool {
+        int t, b;
+        if constexpr (NT % TB == 0) {
+            b = tid % TB;
+            t = tid / TB + i * (NT / TB);
+        } else {
+            const int e = tid + i * NT;
+            t = e / TB;
+            b = e - t * TB;
+        }
+        const int l = w * TB + b;
+        lds = t * ST + b;
+        off = (size_t)l + (size_t)t * zstride;
+        return l < nlines;
+    };
 
     if constexpr (GROUP == ZG_PLAIN) {
         for (int sl = 0; sl < a.nplain; ++sl) {
             cpx<T>* s = sp.s[sl];
             for (int e = tid; e < EL; e += NT) {
-                const ZElem z = zelem<LZ, TB>(c, w, e);
-                buf[z.lds] = z.ok ? s[z.kk] : cpx<T>{T(0), T(0)};
+                int lds; size_t off;
+                const bool ok = hbm_at(e, lds, off);
+                buf[lds] = ok ? s[off] : cpx<T>{T(0), T(0)};
             }
             __syncthreads();
-            if (a.plain_sign < 0) fft_lds<T, LZ, TL, NT, -1, TL>(buf, tws, tid);
-            else fft_lds<T, LZ, TL, NT, +1, TL>(buf, tws, tid);
+            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1, ST>(buf, tws, tid);
+            else fft_lds<T, LZ, TB, NT, +1, ST>(buf, tws, tid);
             for (int e = tid; e < EL; e += NT) {
-                const ZElem z = zelem<LZ, TB>(c, w, e);
-                if (z.ok) s[z.kk] = buf[z.lds];
+                int lds; size_t off;
+                if (hbm_at(e, lds, off)) s[off] = buf[lds];
             }
             __syncthreads();
         }
         return;
     } else {
+        // pencil metadata; the kx = 0 pencils of the tile are numbered in order
+        const int l0 = w * TB;
+        const int b0 = (c.Mx - l0 % c.Mx) % c.Mx;     // first kx = 0 pencil of the tile
+        for (int b = tid; b < TB; b += NT) {
+            const int l = l0 + b;
+            const int ky = l / c.Mx, kx = l - ky * c.Mx;
+            const bool ok = l < nlines;
+            lkx[b] = ok ? kx : -1;
+            lky[b] = ky;
+            lord[b] = (ok && kx == 0) ? (b - b0) / c.Mx : -1;
+        }
+        const int nsp = (b0 < TB && b0 < nlines - l0) ? ((min(TB, nlines - l0) - 1 - b0) / c.Mx + 1) * LZ : 0;
+
         cpx<T> v[NF][PPT];
         // every field's tile is requested from HBM up front, so that one load
         // latency covers the whole tile; the forward transforms then run
@@ -633,59 +716,150 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                 const int e = tid + i * NT;
                 v[fi][i] = cpx<T>{T(0), T(0)};
                 if (EL % NT == 0 || e < EL) {
-                    const ZElem z = zelem<LZ, TB>(c, w, e);
-                    if (z.ok) v[fi][i] = src[z.kk];
+                    int lds; size_t off;
+                    if (hbm_i(i, lds, off)) v[fi][i] = src[off];
                 }
             }
         }
+        // element tid + i*NT of the tile in line order (t fastest) for the
+        // k-space stage
+        auto line_i = [&](int i, int& t, int& b) {
+            if constexpr (NT % LZ == 0) {
+                t = tid % LZ;
+                b = tid / LZ + i * (NT / LZ);
+            } else if constexpr (LZ % NT == 0) {
+                b = i / (LZ / NT);
+                t = (i % (LZ / NT)) * NT + tid;
+            } else {
+                const int e = tid + i * NT;
+                b = e / LZ;
+                t = e - b * LZ;
+            }
+        };
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi) {
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) buf[zelem<LZ, TB>(c, w, e).lds] = v[fi][i];
+                if (EL % NT == 0 || e < EL) {
+                    int lds; size_t off;
+                    hbm_i(i, lds, off);
+                    buf[lds] = v[fi][i];
+                }
             }
             __syncthreads();
-            fft_lds<T, LZ, TL, NT, -1, TL>(buf, tws, tid);
+            fft_lds<T, LZ, TB, NT, -1, ST>(buf, tws, tid);
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) v[fi][i] = buf[zelem<LZ, TB>(c, w, e).lds];
+                if (EL % NT == 0 || e < EL) {
+                    int t, b;
+                    line_i(i, t, b);
+                    const cpx<T> x = buf[t * ST + b];
+                    const int ord = lord[b];
+                    if (ord < 0) {
+                        v[fi][i] = x;
+                    } else {
+                        // kx = 0 pencil: R pencils (ky = 0, Ly/2) are split into P and Q here
+                        const int ky = lky[b];
+                        const bool rp = (ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ;
+                        side[(ord * LZ + t) * NF + fi] = rp ? split_pq(x, buf[(LZ - t) * ST + b], 2 * t < LZ) : x;
+                    }
+                }
             }
             __syncthreads();
         }
 
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        // points off the kx = 0 pencils: one evaluation each, in place
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
             const int e = tid + i * NT;
             if (!(EL % NT == 0 || e < EL)) continue;
-            const ZElem z = zelem<LZ, TB>(c, w, e);
-            if (!z.ok) continue;
-            cpx<T> pv[NF];
+            int t, b;
+            line_i(i, t, b);
+            const int kx = lkx[b];
+            if (kx <= 0) continue;
+            const int ky = lky[b];
+            const int kk = l0 + b + t * nlines;
+            cpx<KT> X[NF], O[NF];
 #pragma unroll
-            for (int fi = 0; fi < NF; ++fi) pv[fi] = v[fi][i];
-            kspace_point<KT, GROUP>(c, a, sc, pv, z, acc0, acc1, acc2);
+            for (int q = 0; q < NF; ++q) X[q] = {KT(v[q][i].x), KT(v[q][i].y)};
+            keval<KT, GROUP>(c, a, sc, kx, ky, t, kk, -1, X, O, 1.0, acc0, acc1, acc2);
 #pragma unroll
-            for (int fi = 0; fi < NF; ++fi) v[fi][i] = pv[fi];
+            for (int q = 0; q < NF; ++q) v[q][i] = cpx<T>{T(O[q].x), T(O[q].y)};
         }
+        // points of the kx = 0 pencils, from the side buffer: the first
+        // evaluation of every point, then the second where the plan has one,
+        // its first share kept in the second half of the side buffer
+        cpx<T>* first = side + nsp * NF;
+        __syncthreads();
+#pragma unroll 1
+        for (int j = tid; j < nsp; j += NT) {
+            const int ord = j / LZ, t = j - ord * LZ;
+            const KPlan pl = kplan<GROUP == ZG_DD>(c, lky[b0 + ord * c.Mx], t);
+            cpx<KT> X[NF], O[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) X[q] = ev_in(cpx<KT>{KT(side[j * NF + q].x), KT(side[j * NF + q].y)}, pl.m0);
+            const int kkp = pl.yp >= 0 ? kindex32(c, 0, pl.yp, pl.zp) : -1;
+            keval<KT, GROUP>(c, a, sc, pl.x0, pl.y0, pl.z0, kindex32(c, pl.x0, pl.y0, pl.z0), kkp, X, O, pl.wm,
+                             acc0, acc1, acc2);
+            const KT half = pl.nev == 2 ? KT(0.5) : KT(1);
+            cpx<T>* dst = pl.nev == 2 ? first + j * NF : side + j * NF;
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                const cpx<KT> r = ev_out(cpx<KT>{KT(0), KT(0)}, O[q], pl.m0, half);
+                dst[q] = cpx<T>{T(r.x), T(r.y)};
+            }
+        }
+#pragma unroll 1
+        for (int j = tid; j < nsp; j += NT) {
+            const int ord = j / LZ, t = j - ord * LZ;
+            const KPlan pl = kplan<GROUP == ZG_DD>(c, lky[b0 + ord * c.Mx], t);
+            if (pl.nev < 2) continue;
+            cpx<KT> X[NF], O[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) X[q] = ev_in(cpx<KT>{KT(side[j * NF + q].x), KT(side[j * NF + q].y)}, pl.m1);
+            keval<KT, GROUP>(c, a, sc, pl.x1, pl.y1, pl.z1, kindex32(c, pl.x1, pl.y1, pl.z1), -1, X, O, pl.wm,
+                             acc0, acc1, acc2);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                const cpx<KT> r = ev_out(cpx<KT>{KT(first[j * NF + q].x), KT(first[j * NF + q].y)}, O[q], pl.m1,
+                                         KT(0.5));
+                side[j * NF + q] = cpx<T>{T(r.x), T(r.y)};
+            }
+        }
+        __syncthreads();
         // inverse transforms
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi) {
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) buf[zelem<LZ, TB>(c, w, e).lds] = v[fi][i];
+                if (EL % NT == 0 || e < EL) {
+                    int t, b;
+                    line_i(i, t, b);
+                    const int ord = lord[b];
+                    cpx<T> x = v[fi][i];
+                    if (ord >= 0) {
+                        // R pencils: join P(t) (t < Lz/2) and Q(t) (held at Lz - t) back
+                        const int ky = lky[b];
+                        x = side[(ord * LZ + t) * NF + fi];
+                        if ((ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ)
+                            x = join_pq(x, side[(ord * LZ + LZ - t) * NF + fi], 2 * t < LZ);
+                    }
+                    buf[t * ST + b] = x;
+                }
             }
             __syncthreads();
-            fft_lds<T, LZ, TL, NT, +1, TL>(buf, tws, tid);
+            fft_lds<T, LZ, TB, NT, +1, ST>(buf, tws, tid);
             cpx<T>* __restrict__ dst = sp.s[F0 + fi];
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
                 if (EL % NT == 0 || e < EL) {
-                    const ZElem z = zelem<LZ, TB>(c, w, e);
-                    if (z.ok) dst[z.kk] = buf[z.lds];
+                    int lds; size_t off;
+                    if (hbm_i(i, lds, off)) dst[off] = buf[lds];
                 }
             }
             __syncthreads();
@@ -740,9 +914,9 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
     const int row0 = blockIdx.x * RB;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
-    // half-spectrum points of this block's rows: all NF fields are requested
-    // from HBM up front, so that one load latency covers the block
-    constexpr int HP = RB * (MX + 1);
+    // packed half-spectrum points of this block's rows: all NF fields are
+    // requested from HBM up front, so that one load latency covers the block
+    constexpr int HP = RB * MX;
     constexpr int HPT = (HP + NT - 1) / NT;
     cpx<T> h[NF][HPT];
 #pragma unroll
@@ -752,13 +926,8 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
             const int e = tid + i * NT;
             h[fi][i] = cpx<T>{T(0), T(0)};
             if (HP % NT != 0 && e >= HP) continue;
-            int r, kx;
-            if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
-            else { r = e - RB * MX; kx = MX; }
-            if (row0 + r < nrows) {
-                const size_t o = (kx < MX) ? (size_t)(row0 + r) * MX + kx : (size_t)c.Kh + (size_t)(row0 + r);
-                h[fi][i] = sp.s[fi][o];
-            }
+            const int r = e / MX;
+            if (row0 + r < nrows) h[fi][i] = sp.s[fi][(size_t)row0 * MX + e];
         }
     }
 
@@ -766,19 +935,21 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
 #pragma unroll
     for (int f0 = 0; f0 < NF; f0 += 2) {
         const bool two = (f0 + 1 < NF);
-        // Z = A + i B with both half spectra completed Hermitian; Im of
-        // kx = 0 and kx = Lx/2 is dropped, as a half-complex c2r does.
+        // Z = A + i B with both half spectra completed Hermitian; kx = 0
+        // holds the real A[0] + i A[Lx/2].
 #pragma unroll
         for (int i = 0; i < HPT; ++i) {
             const int e = tid + i * NT;
             if (HP % NT != 0 && e >= HP) continue;
-            int r, kx;
-            if (e < RB * MX) { r = e / MX; kx = e - r * MX; }
-            else { r = e - RB * MX; kx = MX; }
-            cpx<T> A = h[f0][i], B = two ? h[f0 + 1][i] : cpx<T>{T(0), T(0)};
-            if (kx == 0 || kx == MX) { A.y = T(0); B.y = T(0); }
-            buf[kx * ST + r] = cpx<T>{A.x - B.y, A.y + B.x};
-            if (kx != 0 && kx != MX) buf[(LX - kx) * ST + r] = cpx<T>{A.x + B.y, B.x - A.y};
+            const int r = e / MX, kx = e - r * MX;
+            const cpx<T> A = h[f0][i], B = two ? h[f0 + 1][i] : cpx<T>{T(0), T(0)};
+            if (kx == 0) {
+                buf[r] = cpx<T>{A.x, B.x};
+                buf[MX * ST + r] = cpx<T>{A.y, B.y};
+            } else {
+                buf[kx * ST + r] = cpx<T>{A.x - B.y, A.y + B.x};
+                buf[(LX - kx) * ST + r] = cpx<T>{A.x + B.y, B.x - A.y};
+            }
         }
         __syncthreads();
         fft_lds<T, LX, RB, NT, +1, ST>(buf, tws, tid);
@@ -1250,6 +1421,36 @@ __global__ __launch_bounds__(NT) void k_ewald_real(Consts c, EwaldArgs e, double
     for (int m = 0; m < 6; ++m) out[(size_t)m * c.N + cc] = T[m];
 }
 
+// Value at the point (kx, ky, kz), 0 <= kx <= Mx, of a transformed packed
+// spectrum: pencils with kx = 0 carry P (the kx = 0 plane), Q (kx = Lx/2)
+// and, at ky = 0 and Ly/2, R = P + iQ with P and Q Hermitian along kz.
+template <typename T>
+__device__ __forceinline__ cpx<double> spec_at(const Consts& c, const cpx<T>* s, int kx, int ky, int kz)
+{
+    auto at = [&](int x, int y, int z) {
+        const cpx<T> v = s[((size_t)z * c.Ly + y) * c.Mx + x];
+        return cpx<double>{(double)v.x, (double)v.y};
+    };
+    if (kx != 0 && kx != c.Mx) return at(kx, ky, kz);
+    const int kzp = kz == 0 ? 0 : c.Lz - kz;
+    cpx<double> P, Q;
+    if (ky != 0 && 2 * ky != c.Ly) {
+        if (2 * ky < c.Ly) {
+            P = at(0, ky, kz);
+            Q = at(0, c.Ly - ky, kz);
+        } else {
+            const cpx<double> p2 = at(0, c.Ly - ky, kzp), q2 = at(0, ky, kzp);
+            P = {p2.x, -p2.y};
+            Q = {q2.x, -q2.y};
+        }
+    } else {
+        const cpx<double> r = at(0, ky, kz), rp = at(0, ky, kzp);
+        P = {(r.x + rp.x) * 0.5, (r.y - rp.y) * 0.5};
+        Q = {(r.y + rp.y) * 0.5, (rp.x - r.x) * 0.5};
+    }
+    return kx == 0 ? P : Q;
+}
+
 // L(k) = Re FFT(real part) + reciprocal sum, times 14.399644/eps_inf; stored
 // as 6 words per point, in FP64 and in the spectral type.
 template <typename T>
@@ -1276,7 +1477,7 @@ __global__ __launch_bounds__(NT) void k_ewald_recip(Consts c, EwaldArgs e, Spect
     const double k1 = (double)ix / c.Lx;
     const int ia[6] = {0, 1, 2, 1, 2, 0}, ib[6] = {0, 1, 2, 2, 0, 1};
     for (int m = 0; m < 6; ++m) {
-        double acc = sp.s[m][kk].x;
+        double acc = spec_at(c, sp.s[m], ix, iy, iz).x;
         for (int nz = -e.max_z; nz <= e.max_z; ++nz) {
             const double kG3 = e.two_pi_over_a * (k3 + nz);
             for (int ny = -e.max_y; ny <= e.max_y; ++ny) {
@@ -1295,16 +1496,65 @@ __global__ __launch_bounds__(NT) void k_ewald_recip(Consts c, EwaldArgs e, Spect
     }
 }
 
-// w reconstructed from the optimised w(k): slots 0..2 <- -i acouK
-// (coord_module.F:457-469).
+// Point (kx, ky, kz), 0 <= kx <= Mx, of the transformed packed spectrum s.
+__device__ __forceinline__ void kdecode(const Consts& c, size_t kk, int& kx, int& ky, int& kz)
+{
+    if (kk < c.Mh) {
+        kx = (int)(kk % c.Mx);
+        const size_t r = kk / c.Mx;
+        ky = (int)(r % c.Ly);
+        kz = (int)(r / c.Ly);
+    } else {
+        const size_t r = kk - c.Mh;
+        kx = c.Mx;
+        ky = (int)(r / c.Lz);
+        kz = (int)(r % c.Lz);
+    }
+}
+
+// The transformed packed spectrum s on the k array (kindex order).
 template <typename T>
-__global__ void k_prep_w(Consts c, Spectra<T> sp, const cpx<double>* acouK)
+__global__ void k_unpack(Consts c, const cpx<T>* __restrict__ s, cpx<double>* __restrict__ out)
 {
     const size_t kk = (size_t)blockIdx.x * NT + threadIdx.x;
     if (kk >= (size_t)c.Kh) return;
+    int kx, ky, kz;
+    kdecode(c, kk, kx, ky, kz);
+    out[kk] = spec_at(c, s, kx, ky, kz);
+}
+
+// w reconstructed from the optimised w(k) (coord_module.F:457-469): slots
+// 0..2 <- the packed Hermitian part of G = -i acouK, which is what a c2r of
+// the half spectrum -i acouK returns.
+template <typename T>
+__global__ void k_prep_w(Consts c, Spectra<T> sp, const cpx<double>* acouK)
+{
+    const size_t p = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (p >= c.Mh) return;
+    const int x = (int)(p % c.Mx);
+    const size_t r = p / c.Mx;
+    const int y = (int)(r % c.Ly), z = (int)(r / c.Ly);
+    const int zp = z == 0 ? 0 : c.Lz - z;
     for (int q = 0; q < 3; ++q) {
-        const cpx<double> a = acouK[(size_t)q * c.Kh + kk];
-        sp.s[q][kk] = cpx<T>{(T)a.y, (T)(-a.x)};
+        const cpx<double>* aK = acouK + (size_t)q * c.Kh;
+        auto G = [&](int kx, int ky, int kz) {
+            const cpx<double> a = aK[kindex(c.Mx, c.Ly, c.Lz, c.Mh, kx, ky, kz)];
+            return cpx<double>{a.y, -a.x};
+        };
+        // (G(k) + conj G(partner)) / 2
+        auto H = [&](int kx, int ky, int kz, int py, int pz) {
+            const cpx<double> g = G(kx, ky, kz), h = G(kx, py, pz);
+            return cpx<double>{(g.x + h.x) * 0.5, (g.y - h.y) * 0.5};
+        };
+        cpx<double> v;
+        if (x != 0) v = G(x, y, z);
+        else if (y != 0 && 2 * y < c.Ly) v = H(0, y, z, c.Ly - y, zp);
+        else if (2 * y > c.Ly) v = H(c.Mx, c.Ly - y, z, y, zp);
+        else {
+            const cpx<double> hp = H(0, y, z, y, zp), hq = H(c.Mx, y, z, y, zp);
+            v = {hp.x - hq.y, hp.y + hq.x};
+        }
+        sp.s[q][p] = cpx<T>{(T)v.x, (T)v.y};
     }
 }
 
