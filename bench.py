@@ -43,19 +43,22 @@ ENERGY_TERMS = ("short_range", "long_range", "dipole_E_field", "unharmonic", "ho
 def kernel_bytes(name: str, N: int, Kh: int, nf: int, ws: int, materialize_frac: float) -> float:
     """Algorithmic HBM bytes of one launch (compulsory reads + writes).
 
-    ws: bytes of one spectral word (8 FP64, 4 FP32); a complex point is 2 ws;
-    state fields are FP64 in both modes."""
+    A packed half spectrum holds N/2 complex points (kx = 0 and kx = Lx/2
+    share column 0); ws: bytes of one spectral word (8 FP64, 4 FP32), a
+    complex point is 2 ws; state fields are FP64 in both modes; the L(k)
+    table has Kh = (Lx/2+1) Ly Lz points of 6 words."""
     cw = 2 * ws
-    if name == "xfwd_drift_r2c":      # read u, pk, code; write u, nf half spectra
-        return 3 * 8 * N * 3 + N + nf * cw * Kh
+    M = N // 2
+    if name == "xfwd_drift_r2c":      # read u, pk, code; write u, nf spectra
+        return 3 * 8 * N * 3 + N + nf * cw * M
     if name in ("y_forward", "y_inverse"):
-        return 2 * nf * cw * Kh
+        return 2 * nf * cw * M
     if name == "z_dipole_dipole":     # 3 spectra in/out + 6-word L(k) table
-        return 2 * 3 * cw * Kh + 6 * ws * Kh
+        return 2 * 3 * cw * M + 6 * ws * Kh
     if name == "z_strain":            # 6 spectra in/out
-        return 2 * (nf - 3) * cw * Kh
+        return 2 * (nf - 3) * cw * M
     if name == "xinv_c2r_local_kick":  # read nf spectra, u, pk, code; write pk (+p, F when materialised)
-        return nf * cw * Kh + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
+        return nf * cw * M + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
     return 0.0
 
 

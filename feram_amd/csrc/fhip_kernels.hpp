@@ -105,19 +105,39 @@ __host__ __device__ __forceinline__ double kcomp(int i, int L, double a0, bool f
     return f * M2PI / a0;
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red)
+// Workgroup sums of NS values: a butterfly across the 64 lanes of each wave,
+// then the NT/64 wave sums in wave order, so that every run adds in the same
+// order.  red holds at least NS * NT/64 doubles.
+template <int NS>
+__device__ __forceinline__ void block_sums(double (&v)[NS], double* red)
 {
-    const int tid = threadIdx.x;
-    red[tid] = v;
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) red[q * NW + wv] = v[q];
+    }
     __syncthreads();
 #pragma unroll
-    for (int o = NT / 2; o > 0; o >>= 1) {
-        if (tid < o) red[tid] = red[tid] + red[tid + o];
-        __syncthreads();
+    for (int q = 0; q < NS; ++q) {
+        double r = red[q * NW];
+#pragma unroll
+        for (int u = 1; u < NW; ++u) r += red[q * NW + u];
+        v[q] = r;
     }
-    double r = red[0];
     __syncthreads();
-    return r;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+    double a[1] = {v};
+    block_sums<1>(a, red);
+    return a[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -866,17 +886,19 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
         }
         const int nb = a.maxblocks;
         if constexpr (GROUP == ZG_DD) {
-            double r0 = block_sum(acc0, red), r1 = block_sum(acc1, red);
+            double r[2] = {acc0, acc1};
+            block_sums<2>(r, red);
             if (tid == 0) {
-                a.partials[P_ESR * nb + blockIdx.x] = r0;
-                a.partials[P_ELR * nb + blockIdx.x] = r1;
+                a.partials[P_ESR * nb + blockIdx.x] = r[0];
+                a.partials[P_ELR * nb + blockIdx.x] = r[1];
             }
         } else {
-            double r0 = block_sum(acc0, red), r1 = block_sum(acc1, red), r2 = block_sum(acc2, red);
+            double r[3] = {acc0, acc1, acc2};
+            block_sums<3>(r, red);
             if (tid == 0) {
-                a.partials[P_EIS * nb + blockIdx.x] = r0;
-                a.partials[P_EIC * nb + blockIdx.x] = r1;
-                a.partials[P_EIM * nb + blockIdx.x] = r2;
+                a.partials[P_EIS * nb + blockIdx.x] = r[0];
+                a.partials[P_EIC * nb + blockIdx.x] = r[1];
+                a.partials[P_EIM * nb + blockIdx.x] = r[2];
             }
         }
     }
@@ -1082,12 +1104,15 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
     }
     if (a.mode == XI_GENERIC) return;
     const int nb = a.maxblocks;
-    double r;
-    r = block_sum(eunh, red); if (tid == 0) a.partials[P_EUNH * nb + blockIdx.x] = r;
-    r = block_sum(ede, red);  if (tid == 0) a.partials[P_EDE * nb + blockIdx.x] = r;
-    r = block_sum(ke1, red);  if (tid == 0) a.partials[P_KE1 * nb + blockIdx.x] = r;
-    r = block_sum(ke2, red);  if (tid == 0) a.partials[P_KE2 * nb + blockIdx.x] = r;
-    r = block_sum(kea, red);  if (tid == 0) a.partials[P_KEA * nb + blockIdx.x] = r;
+    double r[5] = {eunh, ede, ke1, ke2, kea};
+    block_sums<5>(r, red);
+    if (tid == 0) {
+        a.partials[P_EUNH * nb + blockIdx.x] = r[0];
+        a.partials[P_EDE * nb + blockIdx.x] = r[1];
+        a.partials[P_KE1 * nb + blockIdx.x] = r[2];
+        a.partials[P_KE2 * nb + blockIdx.x] = r[3];
+        a.partials[P_KEA * nb + blockIdx.x] = r[4];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1119,10 +1144,12 @@ __global__ __launch_bounds__(NT) void k_kick(Consts c, Fields f, const Scalars* 
             for (int q = 0; q < 3; ++q) kea += f.pw[q][cc] * f.pw[q][cc] * c.mwinv[s];
         }
     }
-    double r = block_sum(ke, red);
-    if (threadIdx.x == 0) partials[P_KEPK * maxblocks + blockIdx.x] = r;
-    r = block_sum(kea, red);
-    if (threadIdx.x == 0) partials[P_KEA * maxblocks + blockIdx.x] = r;
+    double r[2] = {ke, kea};
+    block_sums<2>(r, red);
+    if (threadIdx.x == 0) {
+        partials[P_KEPK * maxblocks + blockIdx.x] = r[0];
+        partials[P_KEA * maxblocks + blockIdx.x] = r[1];
+    }
 }
 
 // Deterministic fold of the nb per-block partials of NS slots by one
@@ -1131,18 +1158,27 @@ template <int NS>
 __device__ __forceinline__ void fold_slots(const double* part, int maxblocks, int nb, const int (&slots)[NS],
                                            double (&out)[NS], double* red)
 {
+    constexpr int U = 8;                 // loads in flight per slot and thread
     double s[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) s[q] = 0.0;
-    for (int i = threadIdx.x; i < nb; i += NT) {
-        double v[NS];
+    for (int i0 = threadIdx.x; i0 < nb; i0 += U * NT) {
+        double v[U][NS];
 #pragma unroll
-        for (int q = 0; q < NS; ++q) v[q] = part[slots[q] * maxblocks + i];
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * NT;
 #pragma unroll
-        for (int q = 0; q < NS; ++q) s[q] += v[q];
+            for (int q = 0; q < NS; ++q) v[u][q] = i < nb ? part[slots[q] * maxblocks + i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) s[q] += v[u][q];
+        }
     }
+    block_sums<NS>(s, red);
 #pragma unroll
-    for (int q = 0; q < NS; ++q) out[q] = block_sum(s[q], red);
+    for (int q = 0; q < NS; ++q) out[q] = s[q];
 }
 
 // Sum of the nine potential energies (energy_module.F:73-87).
