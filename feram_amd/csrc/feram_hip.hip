@@ -10,6 +10,8 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <map>
+#include <mutex>
 
 using namespace fhip;
 
@@ -151,20 +153,21 @@ hipEvent_t take_event(fhip_ctx *ctx)
 
 // Launch helper: brackets the launch with events when measuring.
 template <typename F>
-int launch(fhip_ctx *ctx, int id, F &&fn)
+int launch(fhip_ctx *ctx, int id, F &&fn, hipStream_t st = nullptr)
 {
+    if (!st) st = ctx->stream;
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->prof) {
         a = take_event(ctx);
         b = take_event(ctx);
-        if (a) hipEventRecord(a, ctx->stream);
+        if (a) hipEventRecord(a, st);
     }
     fn();
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return fail(ctx, FHIP_ERR_HIP, std::string("launch ") + kKernelNames[id] + ": " + hipGetErrorString(e));
     if (ctx->prof && a && b) {
-        hipEventRecord(b, ctx->stream);
+        hipEventRecord(b, st);
         ctx->recs.push_back({id, a, b});
     }
     return FHIP_OK;
@@ -192,6 +195,25 @@ int collect_events(fhip_ctx *ctx)
         int rc_ = (expr);              \
         if (rc_ != FHIP_OK) return rc_; \
     } while (0)
+
+// Static plus dynamic LDS of a kernel above 64 KB needs the kernel's dynamic
+// limit raised (gfx950 has 160 KB per CU); done once per kernel and size.
+int ensure_lds(fhip_ctx *ctx, const void *fn, size_t dyn)
+{
+    static std::mutex mu;
+    static std::map<const void *, size_t> done;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = done.find(fn);
+    if (it != done.end() && it->second >= dyn) return FHIP_OK;
+    hipFuncAttributes fa{};
+    HIPCHK(hipFuncGetAttributes(&fa, fn));
+    if (fa.sharedSizeBytes + dyn > 160 * 1024)
+        return fail(ctx, FHIP_ERR_UNSUPPORTED, "kernel needs more LDS than a CU holds");
+    if (fa.sharedSizeBytes + dyn > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    done[fn] = dyn;
+    return FHIP_OK;
+}
 
 // ---- size dispatch ---------------------------------------------------------
 template <typename T>
@@ -239,20 +261,20 @@ int run_xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
 }
 
 template <typename T>
-int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf)
+int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf, hipStream_t st = nullptr)
 {
     Consts c = ctx->c;
     Spectra<T> sp = tset<T>(ctx).sp;
     const cpx<T> *tw = tset<T>(ctx).twy;
     dim3 grid(ctx->yblocks[tix<T>()], nf);
-    hipStream_t s = ctx->stream;
+    hipStream_t s = st ? st : ctx->stream;
     const int id = sign < 0 ? K_YFWD : K_YINV;
     switch (ctx->Ly) {
 #define FHIP_CASE(n)                                                                                   \
     case n:                                                                                            \
         if (sign < 0)                                                                                  \
-            return launch(ctx, id, [&] { k_ypass<T, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }); \
-        return launch(ctx, id, [&] { k_ypass<T, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); });
+            return launch(ctx, id, [&] { k_ypass<T, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }, s); \
+        return launch(ctx, id, [&] { k_ypass<T, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }, s);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -260,14 +282,14 @@ int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf)
 }
 
 template <typename T, int G>
-int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a)
+int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t st = nullptr)
 {
     Consts c = ctx->c;
     Spectra<T> sp = tset<T>(ctx).sp;
     Scalars *sc = ctx->dsc;
     const cpx<T> *tw = tset<T>(ctx).twz;
     const int nb = ctx->zblocks[tix<T>()];
-    hipStream_t s = ctx->stream;
+    hipStream_t s = st ? st : ctx->stream;
     const int id = (G == ZG_DD) ? K_ZDD : (G == ZG_PLAIN ? K_SETUP : K_ZSTRAIN);
     // side buffer of the kx = 0 pencils: two NF-field values per point of up
     // to (TB - 1) / Mx + 1 pencils per tile
@@ -277,14 +299,12 @@ int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a)
     const size_t shm = (G == ZG_PLAIN) ? 0 : 2 * (size_t)((tz - 1) / ctx->Mx + 1) * ctx->Lz * nf * sizeof(cpx<T>);
     if (shm > 96 * 1024)
         return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
+    auto lds_ok = [&](const void *fn) { return ensure_lds(ctx, fn, shm); };
     switch (ctx->Lz) {
 #define FHIP_CASE(n)                                                                           \
     case n:                                                                                    \
-        if (shm > 64 * 1024 &&                                                                 \
-            hipFuncSetAttribute((const void *)k_zpass<T, n, G>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)shm) != hipSuccess)                                       \
-            return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: side buffer exceeds the LDS limit"); \
-        return launch(ctx, id, [&] { k_zpass<T, n, G><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
+        RC(lds_ok((const void *)k_zpass<T, n, G>));                                            \
+        return launch(ctx, id, [&] { k_zpass<T, n, G><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); }, s);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -348,11 +368,11 @@ int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
         ScalarConsts k = ctx->sk;
         Scalars *sc = ctx->dsc;
         double *part = ctx->partials;
-        int mb = ctx->maxblocks, xb = ctx->xblocks;
+        int mb = ctx->maxblocks, xb = ctx->xblocks, zb = ctx->zblocks[tix<T>()];
         int mode = (kind == SK_NP) ? FIN_NP : (kind == SK_LF ? FIN_LF : (kind == SK_VS ? FIN_VS : FIN_FORCE));
         int ac = ctx->c.acoustic;
         hipStream_t s = ctx->stream;
-        RC(launch(ctx, K_FINISH, [&] { k_finish<<<1, NT, 0, s>>>(k, sc, part, mb, xb, mode, ac, next_step); }));
+        RC(launch(ctx, K_FINISH, [&] { k_finish<<<1, NT, 0, s>>>(k, sc, part, mb, xb, zb, mode, ac, next_step); }));
     }
     return FHIP_OK;
 }

@@ -639,6 +639,7 @@ template <> struct kspace_type<float> { using type = float; };
 // and the side buffer of the kx = 0 pencils, two NF-field values per point,
 // that the host sizes as dynamic LDS.
 constexpr int ZTILE_MAX = 64;
+constexpr int ZSTRAIN_FB = 2;   // strain fields transformed together (DD and acoustic groups: 3)
 template <typename T, int LZ>
 struct ZTile {
     static constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value < ZTILE_MAX
@@ -655,8 +656,13 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
     constexpr int TB = ZTile<T, LZ>::TB, ST = ZTile<T, LZ>::ST, EL = ZTile<T, LZ>::EL, PPT = ZTile<T, LZ>::PPT;
     constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
     constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
+    // fields transformed together: FB tiles side by side in LDS, so that each
+    // Stockham pass has FB times the butterflies between two barriers
+    constexpr int FB = (GROUP == ZG_PLAIN) ? 1 : (GROUP == ZG_STRAIN ? ZSTRAIN_FB : 3);
+    static_assert(NF % FB == 0, "field batches must divide the fields");
+    constexpr int STB = FB * TB + 1;
     using KT = typename kspace_type<T>::type;
-    __shared__ cpx<T> buf[LZ * ST];
+    __shared__ cpx<T> buf[LZ * STB];
     __shared__ cpx<T> tws[LZ];
     __shared__ double red[NT];
     __shared__ int lkx[TB], lky[TB], lord[TB];    // per pencil: kx, ky, ordinal among the kx = 0 pencils (or -1)
@@ -671,7 +677,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
     // element e of the tile in HBM order (b fastest, coalesced)
     auto hbm_at = [&](int e, int& lds, size_t& off) -> bool {
         const int t = e / TB, b = e - t * TB, l = w * TB + b;
-        lds = t * ST + b;
+        lds = t * STB + b;
         off = (size_t)l + (size_t)t * zstride;
         return l < nlines;
     };
@@ -687,7 +693,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
             b = e - t * TB;
         }
         const int l = w * TB + b;
-        lds = t * ST + b;
+        lds = t * STB + b;
         off = (size_t)l + (size_t)t * zstride;
         return l < nlines;
     };
@@ -701,8 +707,8 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
                 buf[lds] = ok ? s[off] : cpx<T>{T(0), T(0)};
             }
             __syncthreads();
-            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1, ST>(buf, tws, tid);
-            else fft_lds<T, LZ, TB, NT, +1, ST>(buf, tws, tid);
+            if (a.plain_sign < 0) fft_lds<T, LZ, TB, NT, -1, STB>(buf, tws, tid);
+            else fft_lds<T, LZ, TB, NT, +1, STB>(buf, tws, tid);
             for (int e = tid; e < EL; e += NT) {
                 int lds; size_t off;
                 if (hbm_at(e, lds, off)) s[off] = buf[lds];
@@ -757,33 +763,41 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
             }
         };
 #pragma unroll
-        for (int fi = 0; fi < NF; ++fi) {
+        for (int fb = 0; fb < NF; fb += FB) {
 #pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) {
-                    int lds; size_t off;
-                    hbm_i(i, lds, off);
-                    buf[lds] = v[fi][i];
+            for (int f = 0; f < FB; ++f) {
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    const int e = tid + i * NT;
+                    if (EL % NT == 0 || e < EL) {
+                        int lds; size_t off;
+                        hbm_i(i, lds, off);
+                        buf[lds + f * TB] = v[fb + f][i];
+                    }
                 }
             }
             __syncthreads();
-            fft_lds<T, LZ, TB, NT, -1, ST>(buf, tws, tid);
+            fft_lds<T, LZ, FB * TB, NT, -1, STB>(buf, tws, tid);
 #pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) {
-                    int t, b;
-                    line_i(i, t, b);
-                    const cpx<T> x = buf[t * ST + b];
-                    const int ord = lord[b];
-                    if (ord < 0) {
-                        v[fi][i] = x;
-                    } else {
-                        // kx = 0 pencil: R pencils (ky = 0, Ly/2) are split into P and Q here
-                        const int ky = lky[b];
-                        const bool rp = (ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ;
-                        side[(ord * LZ + t) * NF + fi] = rp ? split_pq(x, buf[(LZ - t) * ST + b], 2 * t < LZ) : x;
+            for (int f = 0; f < FB; ++f) {
+                const int fi = fb + f;
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    const int e = tid + i * NT;
+                    if (EL % NT == 0 || e < EL) {
+                        int t, b;
+                        line_i(i, t, b);
+                        const cpx<T> x = buf[t * STB + f * TB + b];
+                        const int ord = lord[b];
+                        if (ord < 0) {
+                            v[fi][i] = x;
+                        } else {
+                            // kx = 0 pencil: R pencils (ky = 0, Ly/2) are split into P and Q here
+                            const int ky = lky[b];
+                            const bool rp = (ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ;
+                            side[(ord * LZ + t) * NF + fi] =
+                                rp ? split_pq(x, buf[(LZ - t) * STB + f * TB + b], 2 * t < LZ) : x;
+                        }
                     }
                 }
             }
@@ -852,34 +866,41 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
         __syncthreads();
         // inverse transforms
 #pragma unroll
-        for (int fi = 0; fi < NF; ++fi) {
+        for (int fb = 0; fb < NF; fb += FB) {
 #pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) {
-                    int t, b;
-                    line_i(i, t, b);
-                    const int ord = lord[b];
-                    cpx<T> x = v[fi][i];
-                    if (ord >= 0) {
-                        // R pencils: join P(t) (t < Lz/2) and Q(t) (held at Lz - t) back
-                        const int ky = lky[b];
-                        x = side[(ord * LZ + t) * NF + fi];
-                        if ((ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ)
-                            x = join_pq(x, side[(ord * LZ + LZ - t) * NF + fi], 2 * t < LZ);
+            for (int f = 0; f < FB; ++f) {
+                const int fi = fb + f;
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    const int e = tid + i * NT;
+                    if (EL % NT == 0 || e < EL) {
+                        int t, b;
+                        line_i(i, t, b);
+                        const int ord = lord[b];
+                        cpx<T> x = v[fi][i];
+                        if (ord >= 0) {
+                            // R pencils: join P(t) (t < Lz/2) and Q(t) (held at Lz - t) back
+                            const int ky = lky[b];
+                            x = side[(ord * LZ + t) * NF + fi];
+                            if ((ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ)
+                                x = join_pq(x, side[(ord * LZ + LZ - t) * NF + fi], 2 * t < LZ);
+                        }
+                        buf[t * STB + f * TB + b] = x;
                     }
-                    buf[t * ST + b] = x;
                 }
             }
             __syncthreads();
-            fft_lds<T, LZ, TB, NT, +1, ST>(buf, tws, tid);
-            cpx<T>* __restrict__ dst = sp.s[F0 + fi];
+            fft_lds<T, LZ, FB * TB, NT, +1, STB>(buf, tws, tid);
 #pragma unroll
-            for (int i = 0; i < PPT; ++i) {
-                const int e = tid + i * NT;
-                if (EL % NT == 0 || e < EL) {
-                    int lds; size_t off;
-                    if (hbm_i(i, lds, off)) dst[off] = buf[lds];
+            for (int f = 0; f < FB; ++f) {
+                cpx<T>* __restrict__ dst = sp.s[F0 + fb + f];
+#pragma unroll
+                for (int i = 0; i < PPT; ++i) {
+                    const int e = tid + i * NT;
+                    if (EL % NT == 0 || e < EL) {
+                        int lds; size_t off;
+                        if (hbm_i(i, lds, off)) dst[off] = buf[lds + f * TB];
+                    }
                 }
             }
             __syncthreads();
@@ -1181,6 +1202,22 @@ __device__ __forceinline__ void fold_slots(const double* part, int maxblocks, in
     for (int q = 0; q < NS; ++q) out[q] = s[q];
 }
 
+// The scalar stage works on an LDS copy of the device scalars: one parallel
+// load, the serial arithmetic of one thread at LDS latency, one parallel
+// store.
+__device__ __forceinline__ void scalars_in(Scalars* ls, const Scalars* sc)
+{
+    constexpr int W = (int)(sizeof(Scalars) / sizeof(double));
+    static_assert(sizeof(Scalars) % sizeof(double) == 0, "Scalars holds doubles only");
+    for (int i = threadIdx.x; i < W; i += NT) reinterpret_cast<double*>(ls)[i] = reinterpret_cast<const double*>(sc)[i];
+}
+__device__ __forceinline__ void scalars_out(Scalars* sc, const Scalars* ls)
+{
+    constexpr int W = (int)(sizeof(Scalars) / sizeof(double));
+    __syncthreads();
+    for (int i = threadIdx.x; i < W; i += NT) reinterpret_cast<double*>(sc)[i] = reinterpret_cast<const double*>(ls)[i];
+}
+
 // Sum of the nine potential energies (energy_module.F:73-87).
 __device__ __forceinline__ double potential(const double* en)
 {
@@ -1230,21 +1267,29 @@ __device__ __forceinline__ void np_begin_scalar(const ScalarConsts& k, Scalars* 
 
 // Energies of the two z-pass groups, homogeneous strain and its coupling
 // (optimize-homo-strain.F:27-85, dVddi-homo-coupling.F:21-26).
-__global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* sc, const double* partials,
+__device__ __forceinline__ void homo_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[3]);
+
+__global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* gsc, const double* partials,
                                              int maxblocks, int zblocks)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
-    const int slots[5] = {P_ESR, P_ELR, P_EIS, P_EIC, P_EIM};
-    double r[5];
-    fold_slots<5>(partials, maxblocks, zblocks, slots, r, red);
-    if (threadIdx.x != 0) return;
+    __shared__ Scalars lsc;
+    scalars_in(&lsc, gsc);
+    const int slots[3] = {P_EIS, P_EIC, P_EIM};
+    double r[3];
+    fold_slots<3>(partials, maxblocks, zblocks, slots, r, red);
+    if (threadIdx.x == 0) homo_scalar(k, &lsc, r);
+    scalars_out(gsc, &lsc);
+}
+
+__device__ __forceinline__ void homo_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[3])
+{
+#pragma clang fp contract(off)
     double* en = sc->en;
-    en[E_SR] = r[0] * k.Ninv / 2;
-    en[E_LR] = r[1] * k.Ninv / 2;
-    en[E_IS] = r[2] * k.Ninv;
-    en[E_IC] = r[3] * 2 * k.Ninv;
-    en[E_IM] = k.modulation_energy ? r[4] * 2 * k.Ninv : 0.0;
+    en[E_IS] = r[0] * k.Ninv;
+    en[E_IC] = r[1] * 2 * k.Ninv;
+    en[E_IM] = k.modulation_energy ? r[2] * 2 * k.Ninv : 0.0;
 
     double c2uu[6];
     for (int i = 0; i < 6; ++i) {
@@ -1308,18 +1353,35 @@ enum FinishMode { FIN_FORCE = 0, FIN_NP = 1, FIN_LF = 2, FIN_VS = 3 };
 // of the integrator that follows the force evaluation.  With FIN_NP and
 // next_step set, the first half of the next Nose-Poincare step (eq. 38 for
 // pi and s, the drift factor) follows in the same launch.
-__global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* sc, const double* partials,
-                                               int maxblocks, int xblocks, int mode, int acoustic,
+__device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[5],
+                                              const double (&rz)[2], int mode, int acoustic, int next_step);
+
+__global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* gsc, const double* partials,
+                                               int maxblocks, int xblocks, int zblocks, int mode, int acoustic,
                                                int next_step)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
+    __shared__ Scalars lsc;
+    scalars_in(&lsc, gsc);
     const int slots[5] = {P_EUNH, P_EDE, P_KE1, P_KE2, P_KEA};
     double r[5];
     fold_slots<5>(partials, maxblocks, xblocks, slots, r, red);
-    if (threadIdx.x != 0) return;
+    const int zslots[2] = {P_ESR, P_ELR};
+    double rz[2];
+    fold_slots<2>(partials, maxblocks, zblocks, zslots, rz, red);
+    if (threadIdx.x == 0) finish_scalar(k, &lsc, r, rz, mode, acoustic, next_step);
+    scalars_out(gsc, &lsc);
+}
+
+__device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[5],
+                                              const double (&rz)[2], int mode, int acoustic, int next_step)
+{
+#pragma clang fp contract(off)
     const double eunh = r[0], ede = r[1], ke1 = r[2], ke2 = r[3], kea = r[4];
     double* en = sc->en;
+    en[E_SR] = rz[0] * k.Ninv / 2;
+    en[E_LR] = rz[1] * k.Ninv / 2;
     en[E_UNH] = eunh;
     en[E_DE] = -ede;
     const double nsp = (double)k.n_sum_p, Nd = (double)k.N;
@@ -1350,17 +1412,21 @@ __global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* sc, cons
 
 // First half of the first Nose-Poincare step of a call: KE from the kick
 // kernel's partials, then eq. 38.
-__global__ __launch_bounds__(NT) void k_np_begin(ScalarConsts k, Scalars* sc, const double* partials,
+__global__ __launch_bounds__(NT) void k_np_begin(ScalarConsts k, Scalars* gsc, const double* partials,
                                                  int maxblocks, int kblocks)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
+    __shared__ Scalars lsc;
+    scalars_in(&lsc, gsc);
     const int slots[1] = {P_KEPK};
     double r[1];
     fold_slots<1>(partials, maxblocks, kblocks, slots, r, red);
-    if (threadIdx.x != 0) return;
-    sc->ke_pk = r[0] / (double)k.n_sum_p * (double)k.N / 2;
-    np_begin_scalar(k, sc);
+    if (threadIdx.x == 0) {
+        lsc.ke_pk = r[0] / (double)k.n_sum_p * (double)k.N / 2;
+        np_begin_scalar(k, &lsc);
+    }
+    scalars_out(gsc, &lsc);
 }
 
 // Kinetic energies of the current momenta for the energy report
