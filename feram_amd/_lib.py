@@ -17,7 +17,7 @@ ERRORS = {-1: "FHIP_ERR_ARG", -2: "FHIP_ERR_UNSUPPORTED", -3: "FHIP_ERR_HIP",
           -4: "FHIP_ERR_ALLOC", -5: "FHIP_ERR_STATE"}
 
 FIELDS = {"dipoR": 0, "dipoP": 1, "dVddi": 2, "acouR": 3, "acouP": 4, "dVdacR": 5,
-          "E_field_local": 6, "acouR_optimised": 7}
+          "E_field_local": 6, "acouR_optimised": 7, "dipoRavg": 8}
 
 # Every symbol include/feram_hip.h declares.
 EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_range",
@@ -26,7 +26,8 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_get_soa", "fhip_get_nose", "fhip_set_nose", "fhip_get_strain",
            "fhip_set_E_field_add", "fhip_force", "fhip_md_prologue", "fhip_nose_poincare",
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_kernel_stats",
-           "fhip_synchronize", "fhip_device_bytes")
+           "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
+           "fhip_avg_get")
 
 
 class FhipParams(C.Structure):
@@ -60,6 +61,22 @@ class FhipEnergy(C.Structure):
 
 
 _lib = None
+
+
+class FhipAverage(C.Structure):
+    """fhip_average: Average_type sums (average_module.F:17-30)."""
+    _fields_ = [("energy", FhipEnergy), ("e2", C.c_double), ("acou_kinetic_true", C.c_double),
+                ("dipo_kinetic_true", C.c_double), ("strain", C.c_double * 6), ("u", C.c_double * 3),
+                ("uu", C.c_double * 6), ("p", C.c_double * 3), ("pp", C.c_double * 6),
+                ("u_tmp", C.c_double * 3), ("uu_tmp", C.c_double * 3), ("p_tmp", C.c_double * 3),
+                ("pp_tmp", C.c_double * 3), ("n_add", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        d = {"energy": self.energy.as_dict()}
+        for name, _ in self._fields_[1:]:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
 
 
 def load():
@@ -98,11 +115,16 @@ def load():
     L.fhip_synchronize.argtypes = [vp]
     L.fhip_device_bytes.argtypes = [vp]
     L.fhip_device_bytes.restype = C.c_int64
+    L.fhip_avg_reset.argtypes = [vp, ip]
+    L.fhip_avg_add.argtypes = [vp]
+    L.fhip_avg_auto.argtypes = [vp, ip]
+    L.fhip_avg_get.argtypes = [vp, C.POINTER(FhipAverage), vp, vp]
     for n in ("fhip_set_long_range", "fhip_get_long_range", "fhip_set_modulation", "fhip_set_cells", "fhip_put",
               "fhip_get", "fhip_put_soa", "fhip_get_soa", "fhip_get_nose", "fhip_set_nose",
               "fhip_get_strain", "fhip_set_E_field_add", "fhip_force", "fhip_md_prologue",
               "fhip_nose_poincare", "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile",
-              "fhip_kernel_stats", "fhip_synchronize"):
+              "fhip_kernel_stats", "fhip_synchronize", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
+              "fhip_avg_get"):
         getattr(L, n).restype = ip
     _lib = L
     return L

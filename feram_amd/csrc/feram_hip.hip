@@ -52,11 +52,12 @@ int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX); }
 
 enum KernelId {
     K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
-    K_KE, K_SCALE, K_SETUP, K_NKERNEL
+    K_KE, K_SCALE, K_SETUP, K_AVG, K_NKERNEL
 };
 const char *kKernelNames[K_NKERNEL] = {
     "xfwd_drift_r2c", "y_forward", "z_dipole_dipole", "z_strain", "homo_strain", "y_inverse",
-    "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup"};
+    "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup",
+    "average"};
 
 }  // namespace
 
@@ -87,6 +88,10 @@ struct fhip_ctx {
     double *Ltab = nullptr;           // FP64 L(k), 6 words per point (master copy)
     cpx<double> *acouK = nullptr;
     cpx<double> *modk = nullptr;      // FFT of the modulation field on the k array
+    AvgAcc *davg = nullptr;           // averages (add_to_Average sums)
+    unsigned long long *dist1 = nullptr;
+    int *dist3 = nullptr;
+    bool avg_auto = false, avg_hist = false, avg_ravg = false;
     cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
     double *kvx = nullptr, *kvy = nullptr, *kvz = nullptr;
     size_t Ks = 0;                    // spectral slot stride: Mh packed points
@@ -330,6 +335,27 @@ ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
     return a;
 }
 
+// ---- averages (add_to_Average) ----------------------------------------------
+int avg_add(fhip_ctx *ctx)
+{
+    if (!ctx->davg) return fail(ctx, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    double *part = ctx->partials;
+    int mb = ctx->maxblocks, kb = ctx->kblocks;
+    unsigned long long *d1 = ctx->dist1;
+    int *d3 = ctx->dist3;
+    const int ravg = ctx->avg_ravg ? 1 : 0;
+    hipStream_t s = ctx->stream;
+    if (ctx->avg_hist)
+        RC(launch(ctx, K_AVG, [&] { k_avg_cells<1><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); }));
+    else
+        RC(launch(ctx, K_AVG, [&] { k_avg_cells<0><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); }));
+    Scalars *sc = ctx->dsc;
+    AvgAcc *acc = ctx->davg;
+    return launch(ctx, K_AVG, [&] { k_avg_fold<<<1, NT, 0, s>>>(sc, acc, part, mb, kb); });
+}
+
 // ---- one force evaluation (mirror_force_mirror) inside a step ---------------
 int nforward(const fhip_ctx *ctx) { return ctx->c.acoustic ? 12 : 9; }
 
@@ -435,6 +461,14 @@ size_t padded_index(const fhip_ctx *ctx, int comp, int ncomp, int x, int y, int 
     return (size_t)comp + ncomp * ((size_t)x + dx * ((size_t)y + dy * (size_t)z));
 }
 
+int alloc_uavg(fhip_ctx *ctx)
+{
+    double *blk = nullptr;
+    RC(dalloc(ctx, &blk, 3 * (size_t)ctx->N));
+    for (int q = 0; q < 3; ++q) ctx->f.uavg[q] = blk + q * (size_t)ctx->N;
+    return FHIP_OK;
+}
+
 double **field_ptrs(fhip_ctx *ctx, int field)
 {
     switch (field) {
@@ -445,6 +479,7 @@ double **field_ptrs(fhip_ctx *ctx, int field)
     case FHIP_ACOU_P: return ctx->f.pw;
     case FHIP_DVDAC: return ctx->f.dVdw;
     case FHIP_E_LOCAL: return ctx->f.eloc;
+    case FHIP_DIPO_R_AVG: return ctx->f.uavg;
     default: return nullptr;
     }
 }
@@ -912,6 +947,7 @@ static int put_impl(fhip_ctx *ctx, int field, const double *host, bool soa)
     if (!ctx || !host) return fail(ctx, FHIP_ERR_ARG, "null argument");
     if (field == FHIP_DVDDI || field == FHIP_DVDAC || field == FHIP_ACOU_R_OPT)
         return fail(ctx, FHIP_ERR_ARG, "field is read only");
+    if (field == FHIP_DIPO_R_AVG && !ctx->f.uavg[0]) RC(alloc_uavg(ctx));
     if (field == FHIP_E_LOCAL && !ctx->f.eloc[0]) {
         double *blk = nullptr;
         RC(dalloc(ctx, &blk, 3 * (size_t)ctx->N));
@@ -1071,6 +1107,7 @@ int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e)
     for (int i = 0; i < nsteps; ++i) {
         const bool last = (i == nsteps - 1);
         RC(force_pipeline(ctx, SK_NP, last, last ? 0 : 1));
+        if (ctx->avg_auto) RC(avg_add(ctx));
     }
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
@@ -1085,7 +1122,10 @@ int fhip_leapfrog(fhip_ctx *ctx, int i_step, int nsteps, fhip_energy *e)
     if (i_step == 0)
         return fail(ctx, FHIP_ERR_UNSUPPORTED,
                     "the i_step == 0 half kick is not reachable from molecular_dynamics (molecular-dynamics.F:64)");
-    for (int i = 0; i < nsteps; ++i) RC(force_pipeline(ctx, SK_LF, i == nsteps - 1));
+    for (int i = 0; i < nsteps; ++i) {
+        RC(force_pipeline(ctx, SK_LF, i == nsteps - 1));
+        if (ctx->avg_auto) RC(avg_add(ctx));
+    }
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
     return FHIP_OK;
@@ -1096,7 +1136,10 @@ int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e)
     if (!ctx) return FHIP_ERR_ARG;
     if (nsteps < 1) return fail(ctx, FHIP_ERR_ARG, "nsteps must be >= 1");
     if (ctx->p.method != FHIP_VS) return fail(ctx, FHIP_ERR_STATE, "context was created for another method");
-    for (int i = 0; i < nsteps; ++i) RC(force_pipeline(ctx, SK_VS, i == nsteps - 1));
+    for (int i = 0; i < nsteps; ++i) {
+        RC(force_pipeline(ctx, SK_VS, i == nsteps - 1));
+        if (ctx->avg_auto) RC(avg_add(ctx));
+    }
     {
         Consts c = ctx->c;
         Fields f = ctx->f;
@@ -1109,6 +1152,72 @@ int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e)
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
     return FHIP_OK;
+}
+
+int fhip_avg_reset(fhip_ctx *ctx, int options)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    const bool histograms = (options & FHIP_AVG_HISTOGRAMS) != 0;
+    const bool ravg = (options & FHIP_AVG_DIPO_R_AVG) != 0;
+    if (!ctx->davg) RC(dalloc(ctx, &ctx->davg, 1));
+    if (ravg && !ctx->f.uavg[0]) RC(alloc_uavg(ctx));
+    ctx->avg_ravg = ravg;
+    if (ravg)
+        for (int q = 0; q < 3; ++q) HIPCHK(hipMemsetAsync(ctx->f.uavg[q], 0, sizeof(double) * ctx->N, ctx->stream));
+    if (histograms && !ctx->dist1) {
+        RC(dalloc(ctx, &ctx->dist1, 3 * (size_t)DIST_W));
+        RC(dalloc(ctx, &ctx->dist3, (size_t)DIST_W * DIST_W * DIST_W));
+    }
+    ctx->avg_hist = histograms;
+    HIPCHK(hipMemsetAsync(ctx->davg, 0, sizeof(AvgAcc), ctx->stream));
+    if (ctx->dist1) {
+        HIPCHK(hipMemsetAsync(ctx->dist1, 0, sizeof(unsigned long long) * 3 * DIST_W, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->dist3, 0, sizeof(int) * (size_t)DIST_W * DIST_W * DIST_W, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return FHIP_OK;
+}
+
+int fhip_avg_add(fhip_ctx *ctx)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    return avg_add(ctx);
+}
+
+int fhip_avg_auto(fhip_ctx *ctx, int enable)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (enable && !ctx->davg) return fail(ctx, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
+    ctx->avg_auto = enable != 0;
+    return FHIP_OK;
+}
+
+int fhip_avg_get(fhip_ctx *ctx, fhip_average *out, int64_t *distribution, int32_t *distribution3d)
+{
+    if (!ctx || !out) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    if (!ctx->davg) return fail(ctx, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
+    if ((distribution || distribution3d) && !ctx->avg_hist)
+        return fail(ctx, FHIP_ERR_STATE, "histograms were not requested in fhip_avg_reset");
+    AvgAcc a{};
+    HIPCHK(hipMemcpyAsync(&a, ctx->davg, sizeof(AvgAcc), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> d1;
+    if (distribution) {
+        d1.resize(3 * (size_t)DIST_W);
+        HIPCHK(hipMemcpyAsync(d1.data(), ctx->dist1, sizeof(unsigned long long) * d1.size(), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    if (distribution3d)
+        HIPCHK(hipMemcpyAsync(distribution3d, ctx->dist3, sizeof(int) * (size_t)DIST_W * DIST_W * DIST_W,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    static_assert(sizeof(AvgAcc) == sizeof(fhip_average), "fhip_average mirrors AvgAcc");
+    std::memcpy(out, &a, sizeof(AvgAcc) - sizeof(double));
+    out->n_add = (int64_t)a.n_add;
+    // distribution(3, -max_k:max_k) with the component fastest (average_module.F:28)
+    if (distribution)
+        for (int k = 0; k < DIST_W; ++k)
+            for (int q = 0; q < 3; ++q) distribution[q + 3 * k] = (int64_t)d1[q * DIST_W + k];
+    return collect_events(ctx);
 }
 
 int fhip_profile(fhip_ctx *ctx, int enable)

@@ -27,7 +27,9 @@ namespace fhip {
 constexpr int NT = 256;          // threads per workgroup in every kernel
 constexpr int MAX_SLOTS = 12;    // spectral slots
 enum PartialSlot {
-    P_ESR = 0, P_ELR, P_EIS, P_EIC, P_EIM, P_EUNH, P_EDE, P_KE1, P_KE2, P_KEA, P_KEPK, P_NSLOT
+    P_ESR = 0, P_ELR, P_EIS, P_EIC, P_EIM, P_EUNH, P_EDE, P_KE1, P_KE2, P_KEA, P_KEPK,
+    P_AVG,                       // 12 slots: sums of u, u^2, Z* u, (Z* u)^2 over the counted cells
+    P_NSLOT = P_AVG + 12
 };
 
 // Constants shared by all kernels (by value in the kernel arguments).
@@ -58,7 +60,9 @@ struct Scalars {
     double en[13];                   // Energy_type order
     double ke_pk;                    // kinetic energy after the first half kick
     double E_add[3];                 // uniform part of the electric field
-    double pad[3];
+    double en_rep[13];               // energies and s at the end of the last step, kept when
+    double s_rep;                    //   the next step's eq. 38 has already started
+    double pad[2];
 };
 enum EnergyIdx {
     E_ACOU_KIN = 0, E_DIPO_KIN, E_SR, E_LR, E_DE, E_UNH, E_HS, E_HC, E_IS, E_IC, E_IM, E_TOTAL, E_HNP
@@ -73,6 +77,7 @@ struct Fields {
     double* pw[3];
     double* dVdw[3];     // materialised dV/dw
     double* eloc[3];
+    double* uavg[3];     // c%dipoRavg: sum of u over the averaging steps
     const unsigned char* code;   // bits 0-1 species, bit 2 frozen (1/m = 0), bit 3 counted (sum_p)
 };
 
@@ -1407,6 +1412,8 @@ __device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc
         }
     }
     en[E_TOTAL] = en[E_DIPO_KIN] / (sc->s * sc->s) + en[E_ACOU_KIN] + potential(en);
+    for (int i = 0; i < 13; ++i) sc->en_rep[i] = en[i];
+    sc->s_rep = sc->s;
     if (mode == FIN_NP && next_step) np_begin_scalar(k, sc);
 }
 
@@ -1463,6 +1470,110 @@ __global__ void k_reset_scale(Scalars* sc)
 {
     sc->scale_p = 1.0;
     sc->scale_pw = 1.0;
+}
+
+// ---------------------------------------------------------------------------
+// Averages of the MD steps (average_module.F:57-163): per step, the sums over
+// the counted cells of u, u^2, Z* u and (Z* u)^2, the histograms of u, and
+// the accumulation of energies, strain and the products of the sums.
+// ---------------------------------------------------------------------------
+constexpr int DIST_K = 99;                 // max_k (average_module.F:15)
+constexpr int DIST_W = 2 * DIST_K + 1;
+constexpr double DIST_STEP_INV = 200.0;    // 1/step, step = 0.005 (average_module.F:13-14)
+
+// Layout shared with fhip_average (all doubles, n_add last).
+struct AvgAcc {
+    double energy[13];
+    double e2, acou_kinetic_true, dipo_kinetic_true;
+    double strain[6];
+    double u[3], uu[6], p[3], pp[6];
+    double u_tmp[3], uu_tmp[3], p_tmp[3], pp_tmp[3];
+    double n_add;
+};
+
+template <int HIST>
+__global__ __launch_bounds__(NT) void k_avg_cells(Consts c, Fields f, double* partials, int maxblocks,
+                                                  unsigned long long* dist1, int* dist3, int ravg)
+{
+    __shared__ double red[NT];
+    const size_t cc = (size_t)blockIdx.x * NT + threadIdx.x;
+    double v[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) v[q] = 0.0;
+    if (ravg && cc < (size_t)c.N) {                             // molecular-dynamics.F:144-148
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f.uavg[q][cc] = f.uavg[q][cc] + f.u[q][cc];
+    }
+    if (cc < (size_t)c.N && (f.code[cc] & 8)) {
+        double u[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) u[q] = f.u[q][cc];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double zu = c.Zstar * u[q];                   // Z_u_R (dipoFFT.F:35-43)
+            v[q] = u[q];                                        // average_module.F:94-95
+            v[3 + q] = u[q] * u[q];
+            v[6 + q] = zu;                                      // average_module.F:115-116
+            v[9 + q] = zu * zu;
+        }
+        if (HIST) {                                             // average_module.F:130-160
+            long k[3];
+            bool in3 = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                k[q] = lround(u[q] * DIST_STEP_INV);            // Fortran nint: half away from zero
+                const bool in = k[q] >= -DIST_K && k[q] <= DIST_K;
+                if (in) atomicAdd(&dist1[q * DIST_W + (int)k[q] + DIST_K], 1ull);
+                in3 = in3 && in;
+            }
+            if (in3)
+                atomicAdd(&dist3[((int)k[0] + DIST_K) + DIST_W * (((int)k[1] + DIST_K) + DIST_W * ((int)k[2] + DIST_K))], 1);
+        }
+    }
+    block_sums<12>(v, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 12; ++q) partials[(P_AVG + q) * maxblocks + blockIdx.x] = v[q];
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_avg_fold(const Scalars* __restrict__ sc, AvgAcc* acc,
+                                                 const double* partials, int maxblocks, int kblocks)
+{
+#pragma clang fp contract(off)
+    __shared__ double red[NT];
+    int slots[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) slots[q] = P_AVG + q;
+    double r[12];
+    fold_slots<12>(partials, maxblocks, kblocks, slots, r, red);
+    if (threadIdx.x != 0) return;
+    AvgAcc a = *acc;
+    const double* en = sc->en_rep;
+    for (int i = 0; i < 13; ++i) a.energy[i] = a.energy[i] + en[i];                 // average_module.F:76
+    a.e2 = a.e2 + en[E_TOTAL] * en[E_TOTAL];                                      // :77
+    a.dipo_kinetic_true = a.dipo_kinetic_true + en[E_DIPO_KIN] / (sc->s_rep * sc->s_rep);   // :79
+    for (int i = 0; i < 6; ++i) a.strain[i] = a.strain[i] + sc->strain[i];        // :80
+    const double* ut = r;
+    const double* pt = r + 6;
+    for (int q = 0; q < 3; ++q) {
+        a.u[q] = a.u[q] + ut[q];                                                  // :101
+        a.uu[q] = a.uu[q] + ut[q] * ut[q];                                        // :102-104
+        a.p[q] = a.p[q] + pt[q];                                                  // :122
+        a.pp[q] = a.pp[q] + pt[q] * pt[q];                                        // :123-125
+        a.u_tmp[q] = ut[q];
+        a.uu_tmp[q] = r[3 + q];
+        a.p_tmp[q] = pt[q];
+        a.pp_tmp[q] = r[9 + q];
+    }
+    a.uu[3] = a.uu[3] + ut[1] * ut[2];                                            // :105-107
+    a.uu[4] = a.uu[4] + ut[2] * ut[0];
+    a.uu[5] = a.uu[5] + ut[0] * ut[1];
+    a.pp[3] = a.pp[3] + pt[1] * pt[2];                                            // :126-128
+    a.pp[4] = a.pp[4] + pt[2] * pt[0];
+    a.pp[5] = a.pp[5] + pt[0] * pt[1];
+    a.n_add = a.n_add + 1.0;
+    *acc = a;
 }
 
 // ---------------------------------------------------------------------------

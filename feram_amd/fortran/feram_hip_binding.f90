@@ -43,6 +43,16 @@ module feram_hip_binding
           &            inho_modulation, total, H_Nose_Poincare
   end type fhip_energy
 
+  ! Average_type sums (average_module.F:17-30), as add_to_Average builds them
+  type, bind(C) :: fhip_average
+     type(fhip_energy) :: energy
+     real(c_double) :: e2, acou_kinetic_true, dipo_kinetic_true
+     real(c_double) :: strain(6)
+     real(c_double) :: u(3), uu(6), p(3), pp(6)
+     real(c_double) :: u_tmp(3), uu_tmp(3), p_tmp(3), pp_tmp(3)
+     integer(c_int64_t) :: n_add
+  end type fhip_average
+
   interface
      integer(c_int) function fhip_create(p, ctx) bind(C, name='fhip_create')
        import :: c_int, c_ptr, fhip_params
@@ -136,6 +146,27 @@ module feram_hip_binding
        integer(c_int), value :: nsteps
        type(fhip_energy), intent(out) :: e
      end function fhip_velocity_scaling
+     ! reset_Average / add_to_Average / the averaging loop / the sums (average_module.F:32-163)
+     integer(c_int) function fhip_avg_reset(ctx, options) bind(C, name='fhip_avg_reset')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: options
+     end function fhip_avg_reset
+     integer(c_int) function fhip_avg_add(ctx) bind(C, name='fhip_avg_add')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function fhip_avg_add
+     integer(c_int) function fhip_avg_auto(ctx, enable) bind(C, name='fhip_avg_auto')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: enable
+     end function fhip_avg_auto
+     integer(c_int) function fhip_avg_get(ctx, avg, distribution, distribution3d) bind(C, name='fhip_avg_get')
+       import :: c_int, c_ptr, fhip_average
+       type(c_ptr), value :: ctx
+       type(fhip_average), intent(out) :: avg
+       type(c_ptr), value :: distribution, distribution3d
+     end function fhip_avg_get
      integer(c_int) function fhip_synchronize(ctx) bind(C, name='fhip_synchronize')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

@@ -199,6 +199,35 @@ class System:
         return self.velocity_scaling(nsteps)
 
     # ---- measurement ---------------------------------------------------------
+    # ---- averages (add_to_Average, average_module.F:32-163) -----------------
+    def avg_reset(self, histograms: bool = False, dipoRavg: bool = False) -> None:
+        opts = (1 if histograms else 0) | (2 if dipoRavg else 0)
+        self._check(self._L.fhip_avg_reset(self.ctx, opts), "avg_reset")
+
+    def avg_add(self) -> None:
+        self._check(self._L.fhip_avg_add(self.ctx), "avg_add")
+
+    def avg_auto(self, enable: bool = True) -> None:
+        self._check(self._L.fhip_avg_auto(self.ctx, 1 if enable else 0), "avg_auto")
+
+    def avg_get(self, histograms: bool = False) -> dict:
+        """The Average_type sums; with histograms, also distribution (3, 199)
+        and distribution3d (199, 199, 199) indexed [kz][ky][kx] + 99."""
+        from ._lib import FhipAverage
+        a = FhipAverage()
+        d1 = d3 = None
+        if histograms:
+            d1 = np.zeros((199, 3), dtype=np.int64)          # distribution(3, -99:99), component fastest
+            d3 = np.zeros((199, 199, 199), dtype=np.int32)   # distribution3d(kx, ky, kz), kx fastest
+        self._check(self._L.fhip_avg_get(self.ctx, C.byref(a),
+                                         d1.ctypes.data_as(C.c_void_p) if d1 is not None else None,
+                                         d3.ctypes.data_as(C.c_void_p) if d3 is not None else None), "avg_get")
+        out = a.as_dict()
+        if histograms:
+            out["distribution"] = d1.T.copy()       # [component][k + 99]
+            out["distribution3d"] = d3
+        return out
+
     def profile(self, enable: bool = True) -> None:
         self._check(self._L.fhip_profile(self.ctx, 1 if enable else 0), "profile")
 

@@ -85,8 +85,10 @@ enum {
     FHIP_ACOU_P = 4,   /* c%acouP   p_w                                 */
     FHIP_DVDAC  = 5,   /* dV/dw in the dVdac_ptr buffer (read only)      */
     FHIP_E_LOCAL = 6,  /* c%E_field_local; the uniform part is set with fhip_set_E_field_add */
-    FHIP_ACOU_R_OPT = 7 /* w reconstructed from the optimised w(k) as write_system does
+    FHIP_ACOU_R_OPT = 7, /* w reconstructed from the optimised w(k) as write_system does
                            (coord_module.F:457-469) (read only)           */
+    FHIP_DIPO_R_AVG = 8  /* c%dipoRavg, the sum of u over the averaging steps
+                            (molecular-dynamics.F:144-148)                */
 };
 
 typedef struct fhip_ctx fhip_ctx;
@@ -134,6 +136,39 @@ int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e);
 int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e);
 int fhip_leapfrog(fhip_ctx *ctx, int i_step, int nsteps, fhip_energy *e);
 int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e);
+
+/* Average_type (average_module.F:17-30) as the sums add_to_Average builds
+ * (average_module.F:57-128), before divide_and_write_Average divides them. */
+typedef struct fhip_average {
+    fhip_energy energy;           /* sum of e (:76) */
+    double e2;                    /* sum of e%total**2 (:77) */
+    double acou_kinetic_true;     /* stays 0: its accumulation is commented out in the reference (:78) */
+    double dipo_kinetic_true;     /* sum of e%dipo_kinetic / s_Nose**2 (:79) */
+    double strain[6];             /* sum of c%strain (:80) */
+    double u[3], uu[6];           /* sum of U and of U_a U_b (xx yy zz yz zx xy), U = sum of u over
+                                     the sum_p cells (:87-107) */
+    double p[3], pp[6];           /* the same for P = sum of Z_u_R (:109-128) */
+    double u_tmp[3], uu_tmp[3];   /* U and the sum of u**2 of the last call (the detail lines :165-176) */
+    double p_tmp[3], pp_tmp[3];   /* P and the sum of Z_u_R**2 of the last call */
+    int64_t n_add;                /* number of add_to_Average calls */
+} fhip_average;
+
+/* reset_Average (average_module.F:32-55).  Options: FHIP_AVG_HISTOGRAMS also
+ * clears and fills distribution(3,-99:99) and distribution3d(-99:99,-99:99,-99:99);
+ * FHIP_AVG_DIPO_R_AVG also clears c%dipoRavg (field FHIP_DIPO_R_AVG) and adds
+ * u to it at every add, as molecular-dynamics.F:144-148 does. */
+enum { FHIP_AVG_HISTOGRAMS = 1, FHIP_AVG_DIPO_R_AVG = 2 };
+int fhip_avg_reset(fhip_ctx *ctx, int options);
+/* add_to_Average(p,c,e,avg) (average_module.F:57-163) with the state and
+ * energies of the last step or force evaluation, on the device. */
+int fhip_avg_add(fhip_ctx *ctx);
+/* enable != 0: fhip_nose_poincare / fhip_leapfrog / fhip_velocity_scaling call
+ * add_to_Average after every step, as molecular-dynamics.F:141-148 does in
+ * the averaging phase, without leaving the device. */
+int fhip_avg_auto(fhip_ctx *ctx, int enable);
+/* The sums; distribution (3*199 int64, component fastest) and distribution3d
+ * (199^3 int32, kx fastest) may be NULL. */
+int fhip_avg_get(fhip_ctx *ctx, fhip_average *out, int64_t *distribution, int32_t *distribution3d);
 
 /* Measurement: when enabled, HIP events bracket every kernel launch on the
  * context's stream and accumulate per-kernel time.  fhip_kernel_stats fills

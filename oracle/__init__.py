@@ -52,6 +52,22 @@ class FoEnergy(C.Structure):
         return {n: getattr(self, n) for n in ENERGY_FIELDS}
 
 
+class FoAverage(C.Structure):
+    """fo_average: Average_type sums (average_module.F:17-30)."""
+    _fields_ = [("energy", FoEnergy), ("e2", C.c_double), ("acou_kinetic_true", C.c_double),
+                ("dipo_kinetic_true", C.c_double), ("strain", C.c_double * 6), ("u", C.c_double * 3),
+                ("uu", C.c_double * 6), ("p", C.c_double * 3), ("pp", C.c_double * 6),
+                ("u_tmp", C.c_double * 3), ("uu_tmp", C.c_double * 3), ("p_tmp", C.c_double * 3),
+                ("pp_tmp", C.c_double * 3), ("n_add", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        d = {"energy": self.energy.as_dict()}
+        for name, _ in self._fields_[1:]:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
 def build(force: bool = False) -> str:
     src = os.path.join(HERE, "feram_oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
@@ -98,6 +114,8 @@ def lib():
             getattr(L, n).restype = C.c_double
             getattr(L, n).argtypes = [vp]
         L.fo_sum_potential_energies.restype = C.c_double
+        L.fo_add_to_average.argtypes = [vp, C.POINTER(FoEnergy), C.POINTER(FoAverage), vp, vp]
+        L.fo_add_to_average.restype = None
         L.fo_sum_potential_energies.argtypes = [C.POINTER(FoEnergy)]
         L.fo_init_normal.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int,
                                      C.c_int, C.POINTER(C.c_double)]
@@ -141,7 +159,7 @@ def params_struct(p) -> FoParams:
 class Oracle:
     """One oracle context: the reference's ``Coord_type`` plus its tables."""
 
-    REAL3 = ("dipoR", "dipoP", "dVddi", "acouR", "acouP", "dVdacR", "E_field", "dVddi_long")
+    REAL3 = ("dipoR", "dipoP", "dVddi", "acouR", "acouP", "dVdacR", "E_field", "dVddi_long", "Z_u_R")
 
     def __init__(self, p, modulation_R=None, tables=True):
         self.p = p
@@ -258,6 +276,14 @@ class Oracle:
         if m == "lf":
             return self.leapfrog(e, 1)
         return self.velocity_scaling(e)
+
+    def add_to_average(self, e, acc, dist1=None, dist3=None):
+        """add_to_Average (average_module.F:57-163) into acc (FoAverage); dist1
+        int64 (199, 3) [k + 99][component], dist3 int32 (199, 199, 199) [kz][ky][kx]."""
+        self._L.fo_add_to_average(self.ctx, C.byref(e), C.byref(acc),
+                                  dist1.ctypes.data_as(C.c_void_p) if dist1 is not None else None,
+                                  dist3.ctypes.data_as(C.c_void_p) if dist3 is not None else None)
+        return acc
 
     def kinetic_energy_dipo(self):
         return self._L.fo_kinetic_energy_dipo(self.ctx)

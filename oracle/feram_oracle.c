@@ -371,7 +371,7 @@ double *fo_field(fo_ctx *o, const char *n)
     F(dipoR) F(dipoP) F(dVddi) F(acouR) F(acouP) F(dVdacR) F(E_field) F(Z_starR)
     F(mass_dipo_inv) F(mass_acou_inv) F(long_range) F(short_range) F(inhomo_K)
     F(inhomo_K_G) F(inhomo_coup2_K) F(acouK) F(didiK) F(dipoK) F(c_k_modulation_k)
-    F(inhoR) F(dVddi_long) F(didiR) F(modulation_k)
+    F(inhoR) F(dVddi_long) F(didiR) F(modulation_k) F(Z_u_R)
 #undef F
     return NULL;
 }
@@ -1316,4 +1316,58 @@ void fo_init_normal(fo_ctx *o, const double avg[3], const double dev[3], int see
 void fo_uni64_fill(double *out, long n)
 {
     for (long i = 0; i < n; ++i) out[i] = fo_uni64();
+}
+
+/* ------------------------------------------------------------------------- */
+/* add_to_Average (average_module.F:57-163).                                 */
+/* ------------------------------------------------------------------------- */
+void fo_add_to_average(fo_ctx *o, const fo_energy *e, fo_average *a, int64_t *dist1, int32_t *dist3)
+{
+    const int N = o->N, max_k = 99, W = 2 * max_k + 1;
+    const double step_inv = 1.0 / 0.005;                  /* average_module.F:13-14 */
+    const double *ee = &e->acou_kinetic;
+    double *ae = &a->energy.acou_kinetic;
+    for (int i = 0; i < 13; ++i) ae[i] = ae[i] + ee[i];   /* :76 */
+    a->e2 = a->e2 + e->total * e->total;                   /* :77 */
+    a->dipo_kinetic_true = a->dipo_kinetic_true + e->dipo_kinetic / (o->nose[0] * o->nose[0]);   /* :79 */
+    for (int i = 0; i < 6; ++i) a->strain[i] = a->strain[i] + o->strain[i];                    /* :80 */
+    double ut[3] = {0, 0, 0}, uut[3] = {0, 0, 0}, pt[3] = {0, 0, 0}, ppt[3] = {0, 0, 0};
+    for (int c = 0; c < N; ++c) {                          /* :89-100, z outer, x inner */
+        if (!o->sum_p[c]) continue;
+        for (int q = 0; q < 3; ++q) {
+            ut[q] = ut[q] + o->dipoR[3 * c + q];
+            uut[q] = uut[q] + o->dipoR[3 * c + q] * o->dipoR[3 * c + q];
+        }
+    }
+    for (int q = 0; q < 3; ++q) { a->u[q] = a->u[q] + ut[q]; a->uu[q] = a->uu[q] + ut[q] * ut[q]; }  /* :101-104 */
+    a->uu[3] = a->uu[3] + ut[1] * ut[2];
+    a->uu[4] = a->uu[4] + ut[2] * ut[0];
+    a->uu[5] = a->uu[5] + ut[0] * ut[1];
+    for (int c = 0; c < N; ++c) {                          /* :111-121 */
+        if (!o->sum_p[c]) continue;
+        for (int q = 0; q < 3; ++q) {
+            pt[q] = pt[q] + o->Z_u_R[3 * c + q];
+            ppt[q] = ppt[q] + o->Z_u_R[3 * c + q] * o->Z_u_R[3 * c + q];
+        }
+    }
+    for (int q = 0; q < 3; ++q) { a->p[q] = a->p[q] + pt[q]; a->pp[q] = a->pp[q] + pt[q] * pt[q]; }  /* :122-125 */
+    a->pp[3] = a->pp[3] + pt[1] * pt[2];
+    a->pp[4] = a->pp[4] + pt[2] * pt[0];
+    a->pp[5] = a->pp[5] + pt[0] * pt[1];
+    if (dist1 || dist3) {                                  /* :130-160 */
+        for (int c = 0; c < N; ++c) {
+            if (!o->sum_p[c]) continue;
+            long k[3];
+            int in3 = 1;
+            for (int q = 0; q < 3; ++q) {
+                k[q] = lround(o->dipoR[3 * c + q] * step_inv);   /* nint */
+                const int in = labs(k[q]) <= max_k;
+                if (in && dist1) dist1[q + 3 * (k[q] + max_k)] += 1;
+                in3 = in3 && in;
+            }
+            if (in3 && dist3) dist3[(k[0] + max_k) + W * ((k[1] + max_k) + W * (k[2] + max_k))] += 1;
+        }
+    }
+    for (int q = 0; q < 3; ++q) { a->u_tmp[q] = ut[q]; a->uu_tmp[q] = uut[q]; a->p_tmp[q] = pt[q]; a->pp_tmp[q] = ppt[q]; }
+    a->n_add += 1;
 }

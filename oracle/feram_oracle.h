@@ -20,6 +20,7 @@
  */
 #ifndef FERAM_ORACLE_H
 #define FERAM_ORACLE_H
+#include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -95,6 +96,20 @@ void fo_velocity_scaling(fo_ctx *o, fo_energy *e);         /* velocity_scaling.F
 double fo_kinetic_energy_dipo(fo_ctx *o);                  /* kinetic_energy_dipo.F:1-29 */
 double fo_kinetic_energy_acou(fo_ctx *o);                  /* kinetic_energy_acou.F:1-28 */
 double fo_sum_potential_energies(const fo_energy *e);      /* energy_module.F:73-87 */
+
+/* Average_type sums (average_module.F:17-30) and add_to_Average
+ * (average_module.F:57-163).  dist1 is distribution(3,-99:99) with the
+ * component fastest, dist3 distribution3d(-99:99,-99:99,-99:99) with kx
+ * fastest; either may be NULL. */
+typedef struct fo_average {
+    fo_energy energy;
+    double e2, acou_kinetic_true, dipo_kinetic_true;
+    double strain[6];
+    double u[3], uu[6], p[3], pp[6];
+    double u_tmp[3], uu_tmp[3], p_tmp[3], pp_tmp[3];
+    int64_t n_add;
+} fo_average;
+void fo_add_to_average(fo_ctx *o, const fo_energy *e, fo_average *a, int64_t *dist1, int32_t *dist3);
 void fo_nose_poincare_hamiltonian(fo_ctx *o, fo_energy *e);/* energy_module.F:128-148 */
 /* initialize_dipoR tail (initialize-dipoR.F:59-69) followed by the H0 set-up
  * of molecular_dynamics (molecular-dynamics.F:44-55). */

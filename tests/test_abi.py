@@ -40,16 +40,17 @@ def test_struct_layout_matches_c(tmp_path):
     from feram_amd import _lib
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "feram_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(fhip_params), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(fhip_params), '
                    'sizeof(fhip_energy), offsetof(fhip_params, dt), offsetof(fhip_params, j), '
-                   'offsetof(fhip_params, external_E_field));return 0;}\n')
+                   'offsetof(fhip_params, external_E_field), sizeof(fhip_average), '
+                   'offsetof(fhip_average, pp), offsetof(fhip_average, n_add));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                           check=True).stdout.split()]
-    P, E = _lib.FhipParams, _lib.FhipEnergy
+    P, E, A = _lib.FhipParams, _lib.FhipEnergy, _lib.FhipAverage
     assert got == [ctypes.sizeof(P), ctypes.sizeof(E), P.dt.offset, P.j.offset,
-                   P.external_E_field.offset]
+                   P.external_E_field.offset, ctypes.sizeof(A), A.pp.offset, A.n_add.offset]
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
