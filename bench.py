@@ -129,6 +129,94 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
     return res, e0, F0
 
 
+def sweep_replicas(n: int, precision: int, seed: int = 20241203):
+    """The batched-ensemble workload: n BaTiO3 bulk replicas with edges drawn
+    from {12, 16, 24, 32} (seeded, uneven mix), T ~ U[150, 450] K, static
+    E_z ~ U[0, 100] kV/cm (0 to 1e-3 V/A), NP-MD, u0 ~ U[-0.05, 0.05] A."""
+    from feram_amd.params import batio3_zvr
+    from feram_amd.system import System
+    rng = np.random.default_rng(seed)
+    edges = rng.choice([12, 16, 24, 32], size=n)
+    temps = rng.uniform(150.0, 450.0, size=n)
+    fields = rng.uniform(0.0, 1e-3, size=n)
+    systems = []
+    for L, T, Ez in zip(edges, temps, fields):
+        L = int(L)
+        p = batio3_zvr(L, method="md", kelvin=float(T))
+        p.Q_Nose = 0.1 * p.N / 16 ** 3          # heating example's 0.1 at 16^3, scaled with N
+        p.external_E_field = [0.0, 0.0, float(Ez)]
+        p.E_field_always_zero = False
+        g = System(p, precision=precision)
+        g.put("dipoR", rng.uniform(-0.05, 0.05, size=(L, L, L, 3)))
+        g.md_prologue()
+        systems.append(g)
+    return systems, edges, temps, fields
+
+
+def bench_replicas(args, sync_extra):
+    from feram_amd.system import Batch
+    n = args.replicas
+    modes = {}
+    for name, prec in (("fp64", 0), ("fp32", 1)):
+        if args.precision not in ("both", name):
+            continue
+        systems, edges, temps, fields = sweep_replicas(n, prec)
+        cells = int(sum(int(L) ** 3 for L in edges))
+        res = {"replicas": n, "cells": cells,
+               "edges": {str(L): int(np.sum(edges == L)) for L in (12, 16, 24, 32)}}
+        with Batch(systems) as bt:
+            res["size_classes"] = bt.classes
+            for graph in (True, False):
+                bt.use_graph(graph)
+                bt.step(args.warmup)
+                sync_extra()
+                print(f"[bench] {n} replicas {name} graph={graph}: timed {args.steps}", file=sys.stderr, flush=True)
+                t0 = time.perf_counter()
+                e = bt.step(args.steps)
+                sync_extra()
+                dt = time.perf_counter() - t0
+                if not all(np.isfinite(v) for x in e for v in x.values()):
+                    raise RuntimeError("non-finite energies after the timed steps")
+                key = "graph" if graph else "direct"
+                res[key] = {"replica_steps_per_s": n * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+                            "cell_updates_per_s": cells * args.steps / dt}
+                print(f"[bench] {key}: {res[key]['replica_steps_per_s']:.0f} replica-steps/s",
+                      file=sys.stderr, flush=True)
+        # the same replicas one context at a time (the per-context call per replica)
+        ks = max(1, args.steps // 4)
+        for g in systems:
+            g.nose_poincare(2)
+        sync_extra()
+        t0 = time.perf_counter()
+        for g in systems:
+            g.nose_poincare(ks)
+        sync_extra()
+        dt = time.perf_counter() - t0
+        res["sequential"] = {"replica_steps_per_s": n * ks / dt, "steps_per_replica": ks}
+        res["batch_speedup_vs_sequential"] = res["graph"]["replica_steps_per_s"] / res["sequential"]["replica_steps_per_s"]
+        res["replicas_per_hour_at_5000_steps"] = 3600.0 * res["graph"]["replica_steps_per_s"] / 5000.0
+        for g in systems:
+            g.close()
+        modes[name] = res
+    head_name = "fp32" if "fp32" in modes else "fp64"
+    head = modes[head_name]
+    out = {
+        "metric": f"replica NP-MD steps/s, {n}-replica batched sweep ({head_name}; both modes alongside)",
+        "value": head["graph"]["replica_steps_per_s"],
+        "unit": "replica-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["graph"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head_name == "fp32" else "f64", "data": "synthetic",
+        "config": {"workload": f"{n} BaTiO3 bulk replicas, edges from {{12,16,24,32}} seeded, "
+                               "T ~ U[150,450] K, E_z ~ U[0,100] kV/cm, NP-MD dt=2 fs, batched "
+                               "(one launch per stage and size class, graph-replayed steps)",
+                   "replicas": n},
+        "modes": modes,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +227,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
+    ap.add_argument("--workload", choices=("md128", "replicas"), default="md128",
+                    help="md128: the single-supercell NP-MD step (default); replicas: the "
+                         "batched replica sweep")
+    ap.add_argument("--replicas", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -158,6 +250,20 @@ def main():
               "(replica ensembles shard with no exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        have_torch = False
+
+    def sync_extra():
+        if have_torch:
+            torch.cuda.synchronize()
+
+    if args.workload == "replicas":
+        bench_replicas(args, sync_extra)
+        return
+
     from feram_amd.params import batio3_zvr
 
     L = args.size
@@ -168,16 +274,6 @@ def main():
     p.Q_Nose = args.q_nose if args.q_nose > 0 else 1.0 * p.N / 64 ** 3
     rng = np.random.default_rng(20241203)
     u0 = rng.uniform(-0.05, 0.05, size=(L, L, L, 3))
-
-    try:
-        import torch
-        have_torch = torch.cuda.is_available()
-    except Exception:
-        have_torch = False
-
-    def sync_extra():
-        if have_torch:
-            torch.cuda.synchronize()
 
     modes = {}
     refs = {}

@@ -12,7 +12,7 @@ from .params import Param, batio3_zvr  # noqa: F401
 def __getattr__(name):
     # System loads the HIP library lazily, so that the parameter layer imports
     # on machines without the built library.
-    if name in ("System", "FhipError"):
+    if name in ("System", "Batch", "FhipError"):
         from . import system
         return getattr(system, name)
     raise AttributeError(name)

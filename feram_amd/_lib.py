@@ -27,7 +27,8 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_set_E_field_add", "fhip_force", "fhip_md_prologue", "fhip_nose_poincare",
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_kernel_stats",
            "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
-           "fhip_avg_get")
+           "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
+           "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size", "fhip_batch_classes")
 
 
 class FhipParams(C.Structure):
@@ -119,6 +120,18 @@ def load():
     L.fhip_avg_add.argtypes = [vp]
     L.fhip_avg_auto.argtypes = [vp, ip]
     L.fhip_avg_get.argtypes = [vp, C.POINTER(FhipAverage), vp, vp]
+    L.fhip_batch_create.argtypes = [C.POINTER(vp), ip, C.POINTER(vp)]
+    L.fhip_batch_destroy.argtypes = [vp]
+    L.fhip_batch_destroy.restype = None
+    L.fhip_batch_last_error.argtypes = [vp]
+    L.fhip_batch_last_error.restype = C.c_char_p
+    L.fhip_batch_step.argtypes = [vp, ip, C.POINTER(FhipEnergy)]
+    L.fhip_batch_graph.argtypes = [vp, ip]
+    L.fhip_batch_size.argtypes = [vp]
+    L.fhip_batch_classes.argtypes = [vp]
+    for n in ("fhip_batch_create", "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size",
+              "fhip_batch_classes"):
+        getattr(L, n).restype = ip
     for n in ("fhip_set_long_range", "fhip_get_long_range", "fhip_set_modulation", "fhip_set_cells", "fhip_put",
               "fhip_get", "fhip_put_soa", "fhip_get_soa", "fhip_get_nose", "fhip_set_nose",
               "fhip_get_strain", "fhip_set_E_field_add", "fhip_force", "fhip_md_prologue",

@@ -18,9 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libferam_hip.so")
-SOURCES = [os.path.join(CSRC, "feram_hip.hip")]
+SOURCES = [os.path.join(CSRC, "feram_hip.hip"), os.path.join(CSRC, "fhip_batch.hip")]
 HEADERS = [os.path.join(CSRC, "fhip_fft.hpp"), os.path.join(CSRC, "fhip_kernels.hpp"),
-           os.path.join(ROOT, "include", "feram_hip.h")]
+           os.path.join(CSRC, "fhip_host.hpp"), os.path.join(ROOT, "include", "feram_hip.h")]
 ARCH = os.environ.get("FHIP_ARCH", "gfx950")
 
 
@@ -31,32 +31,56 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP library cannot be built")
 
 
-def command(out: str = LIB) -> list:
-    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+def flags() -> list:
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Xarch_host", "-ffp-contract=off",
             "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
-            "-I", os.path.join(ROOT, "include"), "-o", out] + SOURCES
+            "-I", os.path.join(ROOT, "include")]
+
+
+def compile_commands(objdir: str) -> list:
+    """One hipcc -c per translation unit (run in parallel)."""
+    return [[hipcc()] + flags() + ["-c", src, "-o", os.path.join(objdir, os.path.basename(src) + ".o")]
+            for src in SOURCES]
+
+
+def link_command(objdir: str, out: str = LIB) -> list:
+    return [hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", out] + \
+        [os.path.join(objdir, os.path.basename(src) + ".o") for src in SOURCES]
 
 
 def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
-    cmd = command(tmp)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
     t0 = time.time()
+    procs = []
+    for cmd in compile_commands(objdir):
+        if verbose:
+            print("[feram_amd.build]", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+    errs = []
+    for pr in procs:
+        out, err = pr.communicate()
+        errs.append(err)
+        if pr.returncode != 0:
+            sys.stderr.write(out + err)
+            raise RuntimeError(f"hipcc failed with status {pr.returncode}")
+    cmd = link_command(objdir, tmp)
     if verbose:
         print("[feram_amd.build]", " ".join(cmd), flush=True)
     res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+        raise RuntimeError(f"hipcc link failed with status {res.returncode}")
     os.replace(tmp, LIB)
     if verbose:
-        warn = [l for l in res.stderr.splitlines() if "warning" in l and "packed-fp32" not in l]
+        warn = [l for e in errs for l in e.splitlines() if "warning" in l and "packed-fp32" not in l]
         for l in warn[:20]:
             print(l)
         print(f"[feram_amd.build] built {LIB} in {time.time() - t0:.1f} s", flush=True)
     return LIB
-
 
 if __name__ == "__main__":
     build()

@@ -2,8 +2,7 @@
 // include/feram_hip.h.  One HIP stream per context; every kernel of a step is
 // launched on it back to back, and the host only synchronises when it reads
 // energies or fields.
-#include "../../include/feram_hip.h"
-#include "fhip_kernels.hpp"
+#include "fhip_host.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -14,6 +13,7 @@
 #include <mutex>
 
 using namespace fhip;
+using namespace fhip_host;
 
 namespace {
 
@@ -21,8 +21,6 @@ constexpr double M_PI_D = 3.14159265358979323846;
 constexpr double HARTREE_BOHR = 14.399644;      // define.h:32
 constexpr double KELVIN_EV = 11604.45;          // define.h:33
 constexpr double ELEMENTARY_CHARGE = 1.60217653;
-
-#define FHIP_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
 bool size_supported(int n)
 {
@@ -36,97 +34,10 @@ bool size_supported(int n)
     }
 }
 
-int tile_rows(int n, int cap = 1024)   // must agree with tile_of<n, cap>::value
-{
-    int p = 1;
-    while (2 * p * n <= cap) p *= 2;
-    return p;
-}
-
-// Lines per y tile of spectral type T (line_tile_cap<T>() on the device) and
-// per z tile (ZTile<T, n>::TB).
-template <typename T>
-int line_tile_rows(int n) { return tile_rows(n, 16384 / (int)sizeof(cpx<T>)); }
-template <typename T>
-int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX); }
-
-enum KernelId {
-    K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
-    K_KE, K_SCALE, K_SETUP, K_AVG, K_NKERNEL
-};
 const char *kKernelNames[K_NKERNEL] = {
     "xfwd_drift_r2c", "y_forward", "z_dipole_dipole", "z_strain", "homo_strain", "y_inverse",
     "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup",
     "average"};
-
-}  // namespace
-
-template <typename T>
-struct TypedSet {
-    Spectra<T> sp{};
-    cpx<T> *twx = nullptr, *twy = nullptr, *twz = nullptr;
-    T *Lt = nullptr;                  // L(k) in the spectral type, 6 words per point
-};
-
-struct fhip_ctx {
-    fhip_params p{};
-    int Lx = 0, Ly = 0, Lz = 0, Kx = 0, Mx = 0, N = 0, Kh = 0, nslots = 0;
-    size_t Mh = 0;
-    int xblocks = 0, kblocks = 0, maxblocks = 0;
-    int yblocks[2] = {0, 0}, zblocks[2] = {0, 0};   // per spectral type: [0] FP64, [1] FP32
-    hipStream_t stream = nullptr;
-    std::string err;
-    Consts c{};
-    ScalarConsts sk{};
-    Fields f{};
-    int precision = FHIP_FP64;
-    TypedSet<double> dset;            // FP64 spectra (plain mode) and the FP64 set-up
-    TypedSet<float> fset;             // FP32 spectra (FP32 mode)
-    Scalars *dsc = nullptr;
-    Scalars *hsc = nullptr;           // pinned mirror
-    double *partials = nullptr;
-    double *Ltab = nullptr;           // FP64 L(k), 6 words per point (master copy)
-    cpx<double> *acouK = nullptr;
-    cpx<double> *modk = nullptr;      // FFT of the modulation field on the k array
-    AvgAcc *davg = nullptr;           // averages (add_to_Average sums)
-    unsigned long long *dist1 = nullptr;
-    int *dist3 = nullptr;
-    bool avg_auto = false, avg_hist = false, avg_ravg = false;
-    cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
-    double *kvx = nullptr, *kvy = nullptr, *kvz = nullptr;
-    size_t Ks = 0;                    // spectral slot stride: Mh packed points
-    unsigned char *code = nullptr;
-    std::vector<void *> allocs;
-    int64_t bytes = 0;
-    bool acouK_valid = false;
-    // measurement
-    bool prof = false;
-    std::vector<hipEvent_t> ev_pool;
-    struct Rec { int id; hipEvent_t a, b; };
-    std::vector<Rec> recs;
-    double kms[K_NKERNEL] = {0};
-    int64_t kcount[K_NKERNEL] = {0};
-};
-
-namespace {
-
-template <typename T> constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
-template <typename T> TypedSet<T> &tset(fhip_ctx *ctx);
-template <> TypedSet<double> &tset<double>(fhip_ctx *ctx) { return ctx->dset; }
-template <> TypedSet<float> &tset<float>(fhip_ctx *ctx) { return ctx->fset; }
-
-int fail(fhip_ctx *ctx, int code, const std::string &msg)
-{
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(ctx, FHIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 template <typename P>
 int dalloc(fhip_ctx *ctx, P **ptr, size_t count)
@@ -195,30 +106,7 @@ int collect_events(fhip_ctx *ctx)
     return FHIP_OK;
 }
 
-#define RC(expr)                       \
-    do {                               \
-        int rc_ = (expr);              \
-        if (rc_ != FHIP_OK) return rc_; \
-    } while (0)
 
-// Static plus dynamic LDS of a kernel above 64 KB needs the kernel's dynamic
-// limit raised (gfx950 has 160 KB per CU); done once per kernel and size.
-int ensure_lds(fhip_ctx *ctx, const void *fn, size_t dyn)
-{
-    static std::mutex mu;
-    static std::map<const void *, size_t> done;
-    std::lock_guard<std::mutex> lock(mu);
-    auto it = done.find(fn);
-    if (it != done.end() && it->second >= dyn) return FHIP_OK;
-    hipFuncAttributes fa{};
-    HIPCHK(hipFuncGetAttributes(&fa, fn));
-    if (fa.sharedSizeBytes + dyn > 160 * 1024)
-        return fail(ctx, FHIP_ERR_UNSUPPORTED, "kernel needs more LDS than a CU holds");
-    if (fa.sharedSizeBytes + dyn > 64 * 1024)
-        HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
-    done[fn] = dyn;
-    return FHIP_OK;
-}
 
 // ---- size dispatch ---------------------------------------------------------
 template <typename T>
@@ -316,24 +204,6 @@ int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t st = nullptr)
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lz not supported");
 }
 
-template <typename T>
-ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
-{
-    ZArgs<T> a{};
-    a.Ltab = tset<T>(ctx).Lt;
-    a.csx = ctx->csx;
-    a.csy = ctx->csy;
-    a.csz = ctx->csz;
-    a.kvx = ctx->kvx;
-    a.kvy = ctx->kvy;
-    a.kvz = ctx->kvz;
-    a.modk = ctx->modk;
-    a.acouK = want_acouK ? ctx->acouK : nullptr;
-    a.partials = ctx->partials;
-    a.maxblocks = ctx->maxblocks;
-    a.nplain = 0;
-    return a;
-}
 
 // ---- averages (add_to_Average) ----------------------------------------------
 int avg_add(fhip_ctx *ctx)
@@ -423,24 +293,6 @@ int push_scalars(fhip_ctx *ctx)
     return FHIP_OK;
 }
 
-void energies_out(const fhip_ctx *ctx, fhip_energy *e)
-{
-    if (!e) return;
-    const double *en = ctx->hsc->en;
-    e->acou_kinetic = en[E_ACOU_KIN];
-    e->dipo_kinetic = en[E_DIPO_KIN];
-    e->short_range = en[E_SR];
-    e->long_range = en[E_LR];
-    e->dipole_E_field = en[E_DE];
-    e->unharmonic = en[E_UNH];
-    e->homo_strain = en[E_HS];
-    e->homo_coupling = en[E_HC];
-    e->inho_strain = en[E_IS];
-    e->inho_coupling = en[E_IC];
-    e->inho_modulation = en[E_IM];
-    e->total = en[E_TOTAL];
-    e->H_Nose_Poincare = en[E_HNP];
-}
 
 int launch_kick(fhip_ctx *ctx, bool kick)
 {

@@ -1,0 +1,500 @@
+// fhip_batch.hip -- replica batches: many independent contexts (replicas of a
+// temperature / field sweep) stepped together.
+//
+// The replicas of one size class (edges, precision, acoustic degrees of
+// freedom) share every kernel instantiation and grid, so each stage of a step
+// is one launch covering all of them (blockIdx.z = replica): a 12^3 replica
+// alone fills a few dozen workgroups of the 256 CUs, twelve of them together
+// fill a few hundred.  Each class runs on its own stream, forked from and
+// joined to the batch stream, so the classes overlap on the device.  The
+// steps between the first and the last of a call are identical launch
+// sequences; they are captured once into a HIP graph and replayed.
+#include "fhip_host.hpp"
+
+#include <cstring>
+
+using namespace fhip;
+using namespace fhip_host;
+
+namespace {
+
+// Edges of the batched instantiations.  Supercells above 64 per edge fill the
+// device alone and are stepped with the per-context calls.
+#define FHIP_BATCH_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
+
+bool batch_size_ok(int n)
+{
+    switch (n) {
+#define FHIP_CASE(n) case n:
+        FHIP_BATCH_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+        return true;
+    default:
+        return false;
+    }
+}
+
+struct BClass {
+    int prec = FHIP_FP64;
+    int Lx = 0, Ly = 0, Lz = 0, acoustic = 0, nf = 9;
+    int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
+    size_t zshm_dd = 0, zshm_g = 0;     // dynamic LDS of the two z passes
+    std::vector<fhip_ctx *> members;
+    void *dreps = nullptr;              // Rep<T>[members.size()] on the device
+    std::vector<unsigned char> hreps;   // host image of the same
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool avg = false, hist = false;
+};
+
+}  // namespace
+
+struct fhip_batch {
+    std::vector<fhip_ctx *> ctxs;
+    std::vector<BClass> classes;
+    int method = FHIP_MD;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<int> gkey;              // averaging flags of the captured graph
+    std::string err;
+};
+
+namespace {
+
+int bfail(fhip_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg;
+    return code;
+}
+
+#define BHIP(expr)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return bfail(b, FHIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define BLAUNCH(what)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess)                                                            \
+            return bfail(b, FHIP_ERR_HIP, std::string("launch ") + what + ": " + hipGetErrorString(e_)); \
+        return FHIP_OK;                                                                  \
+    } while (0)
+
+int xf_mode(int method) { return method == FHIP_MD ? XF_NP : (method == FHIP_LF ? XF_LF : XF_VS); }
+int xi_mode(int method) { return method == FHIP_MD ? XI_NP : (method == FHIP_LF ? XI_LF : XI_VS); }
+int fin_mode(int method) { return method == FHIP_MD ? FIN_NP : (method == FHIP_LF ? FIN_LF : FIN_VS); }
+
+// Host image of a replica's arguments, read from the context as it is now.
+template <typename T>
+void fill_rep(Rep<T> &r, fhip_ctx *ctx, int method)
+{
+    std::memset(&r, 0, sizeof(r));
+    r.c = ctx->c;
+    r.k = ctx->sk;
+    r.f = ctx->f;
+    r.sp = tset<T>(ctx).sp;
+    r.za = zargs<T>(ctx, !ctx->c.acoustic);
+    r.xa.mode = xf_mode(method);
+    r.xa.nf = ctx->c.acoustic ? 12 : 9;
+    r.ia.mode = xi_mode(method);
+    r.ia.materialize = 0;
+    r.ia.partials = ctx->partials;
+    r.ia.maxblocks = ctx->maxblocks;
+    r.sc = ctx->dsc;
+    r.partials = ctx->partials;
+    r.twx = tset<T>(ctx).twx;
+    r.twy = tset<T>(ctx).twy;
+    r.twz = tset<T>(ctx).twz;
+    r.acc = ctx->davg;
+    r.dist1 = ctx->dist1;
+    r.dist3 = ctx->dist3;
+    r.maxblocks = ctx->maxblocks;
+    r.xblocks = ctx->xblocks;
+    r.zblocks = ctx->zblocks[tix<T>()];
+    r.kblocks = ctx->kblocks;
+    r.ravg = ctx->avg_ravg ? 1 : 0;
+}
+
+template <typename T>
+int upload_reps(fhip_batch *b, BClass &k)
+{
+    const size_t n = k.members.size();
+    k.hreps.resize(n * sizeof(Rep<T>));
+    Rep<T> *h = reinterpret_cast<Rep<T> *>(k.hreps.data());
+    for (size_t i = 0; i < n; ++i) fill_rep<T>(h[i], k.members[i], b->method);
+    if (!k.dreps) BHIP(hipMalloc(&k.dreps, k.hreps.size()));
+    BHIP(hipMemcpy(k.dreps, k.hreps.data(), k.hreps.size(), hipMemcpyHostToDevice));
+    return FHIP_OK;
+}
+
+// ---- per-stage launches of one class -----------------------------------------
+template <typename T>
+int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const dim3 g(k.xblocks, 1, (unsigned)k.members.size());
+    switch (k.Lx) {
+#define FHIP_CASE(n) \
+    case n: kb_xfwd<T, n><<<g, NT, 0, s>>>(r); BLAUNCH("xfwd");
+        FHIP_BATCH_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return bfail(b, FHIP_ERR_UNSUPPORTED, "Lx not batched");
+}
+
+template <typename T>
+int l_ypass(fhip_batch *b, BClass &k, hipStream_t s, int sign)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const dim3 g(k.yblocks, k.nf, (unsigned)k.members.size());
+    switch (k.Ly) {
+#define FHIP_CASE(n)                                                  \
+    case n:                                                           \
+        if (sign < 0) kb_ypass<T, n, -1><<<g, NT, 0, s>>>(r);         \
+        else kb_ypass<T, n, +1><<<g, NT, 0, s>>>(r);                  \
+        BLAUNCH("ypass");
+        FHIP_BATCH_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return bfail(b, FHIP_ERR_UNSUPPORTED, "Ly not batched");
+}
+
+template <typename T, int G>
+const void *zpass_fn(int Lz)
+{
+    switch (Lz) {
+#define FHIP_CASE(n) \
+    case n: return (const void *)kb_zpass<T, n, G>;
+        FHIP_BATCH_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return nullptr;
+}
+
+template <typename T, int G>
+int l_zpass(fhip_batch *b, BClass &k, hipStream_t s, int want_acouK)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const dim3 g(k.zblocks, 1, (unsigned)k.members.size());
+    const size_t shm = (G == ZG_DD) ? k.zshm_dd : k.zshm_g;
+    switch (k.Lz) {
+#define FHIP_CASE(n) \
+    case n: kb_zpass<T, n, G><<<g, NT, shm, s>>>(r, want_acouK); BLAUNCH("zpass");
+        FHIP_BATCH_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return bfail(b, FHIP_ERR_UNSUPPORTED, "Lz not batched");
+}
+
+template <typename T>
+int l_xinv(fhip_batch *b, BClass &k, hipStream_t s, int materialize)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const dim3 g(k.xblocks, 1, (unsigned)k.members.size());
+    switch (k.Lx) {
+#define FHIP_CASE(n)                                                          \
+    case n:                                                                   \
+        if (k.nf == 12) kb_xinv<T, n, 12><<<g, NT, 0, s>>>(r, materialize);   \
+        else kb_xinv<T, n, 9><<<g, NT, 0, s>>>(r, materialize);               \
+        BLAUNCH("xinv");
+        FHIP_BATCH_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return bfail(b, FHIP_ERR_UNSUPPORTED, "Lx not batched");
+}
+
+// One step of every replica of the class: the force pipeline of the
+// single-context path (xfwd, y, z, homo, y^-1, xinv, finish), then the
+// averages when they are on.
+template <typename T>
+int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int next_step)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const unsigned n = (unsigned)k.members.size();
+    RC(l_xfwd<T>(b, k, s));
+    RC(l_ypass<T>(b, k, s, -1));
+    const int want = (materialize && !k.acoustic) ? 1 : 0;
+    RC((l_zpass<T, ZG_DD>(b, k, s, want)));
+    if (k.acoustic) RC((l_zpass<T, ZG_ACOU>(b, k, s, want)));
+    else RC((l_zpass<T, ZG_STRAIN>(b, k, s, want)));
+    kb_homo<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
+    {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("launch homo: ") + hipGetErrorString(e));
+    }
+    RC(l_ypass<T>(b, k, s, +1));
+    RC(l_xinv<T>(b, k, s, materialize ? 1 : 0));
+    kb_finish<T><<<dim3(1, 1, n), NT, 0, s>>>(r, fin_mode(b->method), next_step);
+    if (k.avg) {
+        if (k.hist) kb_avg_cells<T, 1><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
+        else kb_avg_cells<T, 0><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
+        kb_avg_fold<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("launch finish: ") + hipGetErrorString(e));
+    return FHIP_OK;
+}
+
+// First half kick and eq. 38 of the first Nose-Poincare step of a call.
+template <typename T>
+int class_np_prelude(fhip_batch *b, BClass &k, hipStream_t s)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const unsigned n = (unsigned)k.members.size();
+    kb_kick<T, 1><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
+    kb_np_begin<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("launch np prelude: ") + hipGetErrorString(e));
+    return FHIP_OK;
+}
+
+// Pending velocity-scaling factors applied to the momenta.
+template <typename T>
+int class_vs_epilogue(fhip_batch *b, BClass &k, hipStream_t s)
+{
+    const Rep<T> *r = (const Rep<T> *)k.dreps;
+    const unsigned n = (unsigned)k.members.size();
+    kb_apply_scale<T><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
+    kb_reset_scale<T><<<dim3(1, 1, n), 1, 0, s>>>(r);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("launch vs epilogue: ") + hipGetErrorString(e));
+    return FHIP_OK;
+}
+
+enum Phase { PH_PRELUDE = 0, PH_STEP = 1, PH_LAST = 2, PH_EPILOGUE = 3 };
+
+int class_phase(fhip_batch *b, BClass &k, int phase)
+{
+    const bool f32 = k.prec == FHIP_FP32;
+    hipStream_t s = k.stream;
+    switch (phase) {
+    case PH_PRELUDE:
+        return f32 ? class_np_prelude<float>(b, k, s) : class_np_prelude<double>(b, k, s);
+    case PH_STEP:
+    case PH_LAST: {
+        const bool last = phase == PH_LAST;
+        const int next = (b->method == FHIP_MD && !last) ? 1 : 0;
+        return f32 ? class_step<float>(b, k, s, last, next) : class_step<double>(b, k, s, last, next);
+    }
+    default:
+        return f32 ? class_vs_epilogue<float>(b, k, s) : class_vs_epilogue<double>(b, k, s);
+    }
+}
+
+// One phase of every class: the class streams fork from the batch stream and
+// join back into it.
+int batch_phase(fhip_batch *b, int phase)
+{
+    BHIP(hipEventRecord(b->fork, b->stream));
+    for (auto &k : b->classes) {
+        BHIP(hipStreamWaitEvent(k.stream, b->fork, 0));
+        RC(class_phase(b, k, phase));
+        BHIP(hipEventRecord(k.done, k.stream));
+        BHIP(hipStreamWaitEvent(b->stream, k.done, 0));
+    }
+    return FHIP_OK;
+}
+
+std::vector<int> graph_key(const fhip_batch *b)
+{
+    std::vector<int> key;
+    for (const auto &k : b->classes) {
+        key.push_back(k.avg ? 1 : 0);
+        key.push_back(k.hist ? 1 : 0);
+    }
+    return key;
+}
+
+void drop_graph(fhip_batch *b)
+{
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    b->gkey.clear();
+}
+
+// Capture one middle step (every class) into a graph, once per averaging
+// configuration.
+int ensure_graph(fhip_batch *b)
+{
+    const std::vector<int> key = graph_key(b);
+    if (b->gexec && key == b->gkey) return FHIP_OK;
+    drop_graph(b);
+    BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_phase(b, PH_STEP);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != FHIP_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    BHIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    b->gkey = key;
+    return FHIP_OK;
+}
+
+int refresh(fhip_batch *b)
+{
+    for (auto &k : b->classes) {
+        bool avg = false, hist = false;
+        for (size_t i = 0; i < k.members.size(); ++i) {
+            fhip_ctx *c = k.members[i];
+            if (c->avg_auto && !c->davg) return bfail(b, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
+            if (i == 0) {
+                avg = c->avg_auto;
+                hist = c->avg_auto && c->avg_hist;
+            } else if (avg != c->avg_auto || (avg && hist != c->avg_hist)) {
+                return bfail(b, FHIP_ERR_UNSUPPORTED,
+                             "replicas of one size class must share the averaging settings (fhip_avg_auto, histograms)");
+            }
+        }
+        k.avg = avg;
+        k.hist = hist;
+        RC(k.prec == FHIP_FP32 ? upload_reps<float>(b, k) : upload_reps<double>(b, k));
+    }
+    return FHIP_OK;
+}
+
+template <typename T>
+int prepare_lds(fhip_batch *b, BClass &k)
+{
+    // side buffer of the kx = 0 pencils: two NF-field values per point of up
+    // to (TB - 1) / Mx + 1 pencils per tile, NF = 3 (dipole-dipole), 6
+    // (strain) or 9 (acoustic)
+    const int tz = ztile_rows<T>(k.Lz);
+    const size_t per_field = 2 * (size_t)((tz - 1) / (k.Lx / 2) + 1) * k.Lz * sizeof(cpx<T>);
+    k.zshm_dd = 3 * per_field;
+    k.zshm_g = (k.acoustic ? 9 : 6) * per_field;
+    if (k.zshm_g > 96 * 1024)
+        return bfail(b, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
+    fhip_ctx *ctx = nullptr;
+    int rc = ensure_lds(ctx, zpass_fn<T, ZG_DD>(k.Lz), k.zshm_dd);
+    if (rc == FHIP_OK)
+        rc = ensure_lds(ctx, k.acoustic ? zpass_fn<T, ZG_ACOU>(k.Lz) : zpass_fn<T, ZG_STRAIN>(k.Lz), k.zshm_g);
+    if (rc != FHIP_OK) return bfail(b, rc, "z pass: LDS attribute");
+    return FHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fhip_batch_last_error(const fhip_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+void fhip_batch_destroy(fhip_batch *b)
+{
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (auto &k : b->classes) {
+        if (k.stream) hipStreamSynchronize(k.stream);
+        if (k.dreps) hipFree(k.dreps);
+        if (k.done) hipEventDestroy(k.done);
+        if (k.stream) hipStreamDestroy(k.stream);
+    }
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
+{
+    if (!out) return FHIP_ERR_ARG;
+    *out = nullptr;
+    fhip_batch *b = new fhip_batch();
+    *out = b;
+    if (!ctxs || n < 1) return bfail(b, FHIP_ERR_ARG, "need at least one context");
+    for (int i = 0; i < n; ++i) {
+        fhip_ctx *c = ctxs[i];
+        if (!c) return bfail(b, FHIP_ERR_ARG, "null context");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == c) return bfail(b, FHIP_ERR_ARG, "a context appears twice");
+        if (i == 0) b->method = c->p.method;
+        else if (c->p.method != b->method)
+            return bfail(b, FHIP_ERR_UNSUPPORTED, "every replica of a batch uses the same method");
+        if (!batch_size_ok(c->Lx) || !batch_size_ok(c->Ly) || !batch_size_ok(c->Lz))
+            return bfail(b, FHIP_ERR_UNSUPPORTED,
+                         "batched supercell edges are 4 6 8 12 16 24 32 48 64 (larger ones fill the device alone)");
+        b->ctxs.push_back(c);
+        BClass *k = nullptr;
+        for (auto &q : b->classes)
+            if (q.Lx == c->Lx && q.Ly == c->Ly && q.Lz == c->Lz && q.prec == c->precision &&
+                q.acoustic == c->c.acoustic)
+                k = &q;
+        if (!k) {
+            b->classes.emplace_back();
+            k = &b->classes.back();
+            k->prec = c->precision;
+            k->Lx = c->Lx;
+            k->Ly = c->Ly;
+            k->Lz = c->Lz;
+            k->acoustic = c->c.acoustic;
+            k->nf = c->c.acoustic ? 12 : 9;
+            const int t = c->precision == FHIP_FP32 ? 1 : 0;
+            k->xblocks = c->xblocks;
+            k->yblocks = c->yblocks[t];
+            k->zblocks = c->zblocks[t];
+            k->kblocks = c->kblocks;
+        }
+        k->members.push_back(c);
+    }
+    BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BHIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (auto &k : b->classes) {
+        BHIP(hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking));
+        BHIP(hipEventCreateWithFlags(&k.done, hipEventDisableTiming));
+        RC(k.prec == FHIP_FP32 ? prepare_lds<float>(b, k) : prepare_lds<double>(b, k));
+    }
+    return FHIP_OK;
+}
+
+int fhip_batch_graph(fhip_batch *b, int enable)
+{
+    if (!b) return FHIP_ERR_ARG;
+    b->use_graph = enable != 0;
+    return FHIP_OK;
+}
+
+int fhip_batch_step(fhip_batch *b, int nsteps, fhip_energy *e)
+{
+    if (!b) return FHIP_ERR_ARG;
+    if (nsteps < 1) return bfail(b, FHIP_ERR_ARG, "nsteps must be >= 1");
+    // the contexts' own streams are idle after their (synchronous) calls;
+    // make sure of it before the batch streams touch their memory
+    for (fhip_ctx *c : b->ctxs) BHIP(hipStreamSynchronize(c->stream));
+    RC(refresh(b));
+    if (b->method == FHIP_MD) RC(batch_phase(b, PH_PRELUDE));
+    const int middle = nsteps - 1;
+    if (middle > 0) {
+        if (b->use_graph) {
+            RC(ensure_graph(b));
+            for (int i = 0; i < middle; ++i) BHIP(hipGraphLaunch(b->gexec, b->stream));
+        } else {
+            for (int i = 0; i < middle; ++i) RC(batch_phase(b, PH_STEP));
+        }
+    }
+    RC(batch_phase(b, PH_LAST));
+    if (b->method == FHIP_VS) RC(batch_phase(b, PH_EPILOGUE));
+    for (fhip_ctx *c : b->ctxs)
+        BHIP(hipMemcpyAsync(c->hsc, c->dsc, sizeof(Scalars), hipMemcpyDeviceToHost, b->stream));
+    BHIP(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->ctxs.size(); ++i) {
+        fhip_ctx *c = b->ctxs[i];
+        if (!c->c.acoustic) c->acouK_valid = true;
+        energies_out(c, e ? e + i : nullptr);
+    }
+    return FHIP_OK;
+}
+
+int fhip_batch_size(const fhip_batch *b) { return b ? (int)b->ctxs.size() : 0; }
+int fhip_batch_classes(const fhip_batch *b) { return b ? (int)b->classes.size() : 0; }
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// fhip_host.hpp -- host-side context record and helpers shared by the
+// translation units of the library (single-context steps in feram_hip.hip,
+// replica batches in fhip_batch.hip).
+#pragma once
+#include "../../include/feram_hip.h"
+#include "fhip_kernels.hpp"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace fhip;
+
+// Supercell edges with a compiled FFT length.
+#define FHIP_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+namespace fhip_host {
+
+inline int tile_rows(int n, int cap = 1024)   // must agree with tile_of<n, cap>::value
+{
+    int p = 1;
+    while (2 * p * n <= cap) p *= 2;
+    return p;
+}
+
+// Lines per y tile of spectral type T (line_tile_cap<T>() on the device) and
+// per z tile (ZTile<T, n>::TB).
+template <typename T>
+inline int line_tile_rows(int n) { return tile_rows(n, 16384 / (int)sizeof(cpx<T>)); }
+template <typename T>
+inline int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX); }
+
+enum KernelId {
+    K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
+    K_KE, K_SCALE, K_SETUP, K_AVG, K_NKERNEL
+};
+
+}  // namespace fhip_host
+
+template <typename T>
+struct TypedSet {
+    Spectra<T> sp{};
+    cpx<T> *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    T *Lt = nullptr;                  // L(k) in the spectral type, 6 words per point
+};
+
+struct fhip_ctx {
+    fhip_params p{};
+    int Lx = 0, Ly = 0, Lz = 0, Kx = 0, Mx = 0, N = 0, Kh = 0, nslots = 0;
+    size_t Mh = 0;
+    int xblocks = 0, kblocks = 0, maxblocks = 0;
+    int yblocks[2] = {0, 0}, zblocks[2] = {0, 0};   // per spectral type: [0] FP64, [1] FP32
+    hipStream_t stream = nullptr;
+    std::string err;
+    fhip::Consts c{};
+    fhip::ScalarConsts sk{};
+    fhip::Fields f{};
+    int precision = FHIP_FP64;
+    TypedSet<double> dset;            // FP64 spectra (plain mode) and the FP64 set-up
+    TypedSet<float> fset;             // FP32 spectra (FP32 mode)
+    Scalars *dsc = nullptr;
+    Scalars *hsc = nullptr;           // pinned mirror
+    double *partials = nullptr;
+    double *Ltab = nullptr;           // FP64 L(k), 6 words per point (master copy)
+    cpx<double> *acouK = nullptr;
+    cpx<double> *modk = nullptr;      // FFT of the modulation field on the k array
+    AvgAcc *davg = nullptr;           // averages (add_to_Average sums)
+    unsigned long long *dist1 = nullptr;
+    int *dist3 = nullptr;
+    bool avg_auto = false, avg_hist = false, avg_ravg = false;
+    cpx<double> *csx = nullptr, *csy = nullptr, *csz = nullptr;
+    double *kvx = nullptr, *kvy = nullptr, *kvz = nullptr;
+    size_t Ks = 0;                    // spectral slot stride: Mh packed points
+    unsigned char *code = nullptr;
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    bool acouK_valid = false;
+    // measurement
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int id; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    double kms[fhip_host::K_NKERNEL] = {0};
+    int64_t kcount[fhip_host::K_NKERNEL] = {0};
+};
+
+namespace fhip_host {
+
+template <typename T> constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
+template <typename T> TypedSet<T> &tset(fhip_ctx *ctx);
+template <> inline TypedSet<double> &tset<double>(fhip_ctx *ctx) { return ctx->dset; }
+template <> inline TypedSet<float> &tset<float>(fhip_ctx *ctx) { return ctx->fset; }
+
+inline int fail(fhip_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(ctx, FHIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define RC(expr)                       \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != FHIP_OK) return rc_; \
+    } while (0)
+
+// Static plus dynamic LDS of a kernel above 64 KB needs the kernel's dynamic
+// limit raised (gfx950 has 160 KB per CU); done once per kernel and size.
+inline int ensure_lds(fhip_ctx *ctx, const void *fn, size_t dyn)
+{
+    static std::mutex mu;
+    static std::map<const void *, size_t> done;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = done.find(fn);
+    if (it != done.end() && it->second >= dyn) return FHIP_OK;
+    hipFuncAttributes fa{};
+    HIPCHK(hipFuncGetAttributes(&fa, fn));
+    if (fa.sharedSizeBytes + dyn > 160 * 1024)
+        return fail(ctx, FHIP_ERR_UNSUPPORTED, "kernel needs more LDS than a CU holds");
+    if (fa.sharedSizeBytes + dyn > 64 * 1024)
+        HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    done[fn] = dyn;
+    return FHIP_OK;
+}
+
+inline void energies_out(const fhip_ctx *ctx, fhip_energy *e)
+{
+    if (!e) return;
+    const double *en = ctx->hsc->en;
+    e->acou_kinetic = en[E_ACOU_KIN];
+    e->dipo_kinetic = en[E_DIPO_KIN];
+    e->short_range = en[E_SR];
+    e->long_range = en[E_LR];
+    e->dipole_E_field = en[E_DE];
+    e->unharmonic = en[E_UNH];
+    e->homo_strain = en[E_HS];
+    e->homo_coupling = en[E_HC];
+    e->inho_strain = en[E_IS];
+    e->inho_coupling = en[E_IC];
+    e->inho_modulation = en[E_IM];
+    e->total = en[E_TOTAL];
+    e->H_Nose_Poincare = en[E_HNP];
+}
+
+// Arguments of a z pass of the context's spectral type T.
+template <typename T>
+inline ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
+{
+    ZArgs<T> a{};
+    a.Ltab = tset<T>(ctx).Lt;
+    a.csx = ctx->csx;
+    a.csy = ctx->csy;
+    a.csz = ctx->csz;
+    a.kvx = ctx->kvx;
+    a.kvy = ctx->kvy;
+    a.kvz = ctx->kvz;
+    a.modk = ctx->modk;
+    a.acouK = want_acouK ? ctx->acouK : nullptr;
+    a.partials = ctx->partials;
+    a.maxblocks = ctx->maxblocks;
+    a.nplain = 0;
+    return a;
+}
+
+}  // namespace fhip_host

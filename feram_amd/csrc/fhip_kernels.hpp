@@ -158,9 +158,9 @@ struct XfwdArgs {
 };
 
 template <typename T, int LX>
-__global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, XfwdArgs a,
+__device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XfwdArgs& a,
                                              const Scalars* __restrict__ sc,
-                                             const cpx<T>* __restrict__ twx)
+                                             const cpx<T>* twx, int bx)
 {
     constexpr int RB = tile_of<LX>::value;
     constexpr int ST = RB + 1;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
     __shared__ cpx<T> tws[LX];
     const int tid = threadIdx.x;
     const int nrows = c.Ly * c.Lz;
-    const int row0 = blockIdx.x * RB;
+    const int row0 = bx * RB;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
     double u[CPT][3], w[CPT][3];
@@ -291,6 +291,14 @@ __global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, 
     }
 }
 
+template <typename T, int LX>
+__global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, XfwdArgs a,
+                                             const Scalars* __restrict__ sc,
+                                             const cpx<T>* __restrict__ twx)
+{
+    xfwd_body<T, LX>(c, f, sp, a, sc, twx, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Line transform along y, in place, one field per blockIdx.y: TB columns
 // (z, kx) of stride Mx per tile.  The kx = 0 column carries C = P + iQ, the
@@ -320,17 +328,17 @@ __device__ __forceinline__ cpx<T> join_pq(cpx<T> v, cpx<T> m, bool lower)
 }
 
 template <typename T, int LY, int SIGN>
-__global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0,
-                                              const cpx<T>* __restrict__ twy)
+__device__ __forceinline__ void ypass_body(const Consts& c, const Spectra<T>& sp, int slot0,
+                                              const cpx<T>* twy, int bx, int by)
 {
     constexpr int TB = tile_of<LY, line_tile_cap<T>()>::value;
     __shared__ cpx<T> buf[LY * TB];
     __shared__ cpx<T> tws[LY];
     const int tid = threadIdx.x;
     for (int i = tid; i < LY; i += NT) tws[i] = twy[i];
-    const int l0 = blockIdx.x * TB;
+    const int l0 = bx * TB;
     const int nlines = c.Mx * c.Lz;
-    cpx<T>* s = sp.s[slot0 + blockIdx.y];
+    cpx<T>* s = sp.s[slot0 + by];
     for (int e = tid; e < LY * TB; e += NT) {
         const int t = e / TB, b = e - t * TB, l = l0 + b;
         if (l < nlines) {
@@ -353,6 +361,13 @@ __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0
             s[((size_t)z * LY + t) * c.Mx + kx] = v;
         }
     }
+}
+
+template <typename T, int LY, int SIGN>
+__global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0,
+                                              const cpx<T>* __restrict__ twy)
+{
+    ypass_body<T, LY, SIGN>(c, sp, slot0, twy, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -655,8 +670,8 @@ struct ZTile {
 };
 
 template <typename T, int LZ, int GROUP>
-__global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
-                                              const cpx<T>* __restrict__ twz)
+__device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp, const ZArgs<T>& a, Scalars* sc,
+                                              const cpx<T>* twz, int bx)
 {
     constexpr int TB = ZTile<T, LZ>::TB, ST = ZTile<T, LZ>::ST, EL = ZTile<T, LZ>::EL, PPT = ZTile<T, LZ>::PPT;
     constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
@@ -674,7 +689,7 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
     extern __shared__ char zdyn[];
     cpx<T>* side = reinterpret_cast<cpx<T>*>(zdyn);   // [ordinal*LZ + t][NF]
     const int tid = threadIdx.x;
-    const int w = blockIdx.x;
+    const int w = bx;
     const int nlines = c.Mx * c.Ly;
     const size_t zstride = (size_t)nlines;
     for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
@@ -915,19 +930,26 @@ __global__ __launch_bounds__(NT) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> 
             double r[2] = {acc0, acc1};
             block_sums<2>(r, red);
             if (tid == 0) {
-                a.partials[P_ESR * nb + blockIdx.x] = r[0];
-                a.partials[P_ELR * nb + blockIdx.x] = r[1];
+                a.partials[P_ESR * nb + bx] = r[0];
+                a.partials[P_ELR * nb + bx] = r[1];
             }
         } else {
             double r[3] = {acc0, acc1, acc2};
             block_sums<3>(r, red);
             if (tid == 0) {
-                a.partials[P_EIS * nb + blockIdx.x] = r[0];
-                a.partials[P_EIC * nb + blockIdx.x] = r[1];
-                a.partials[P_EIM * nb + blockIdx.x] = r[2];
+                a.partials[P_EIS * nb + bx] = r[0];
+                a.partials[P_EIC * nb + bx] = r[1];
+                a.partials[P_EIM * nb + bx] = r[2];
             }
         }
     }
+}
+
+template <typename T, int LZ, int GROUP>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GROUP == ZG_ACOU ? 1 : 2))) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
+                                              const cpx<T>* __restrict__ twz)
+{
+    zpass_body<T, LZ, GROUP>(c, sp, a, sc, twz, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -945,9 +967,9 @@ struct XinvArgs {
 };
 
 template <typename T, int LX, int NF>
-__global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, XinvArgs a,
+__device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XinvArgs& a,
                                              const Scalars* __restrict__ sc,
-                                             const cpx<T>* __restrict__ twx)
+                                             const cpx<T>* twx, int bx)
 {
     constexpr int RB = tile_of<LX>::value;
     constexpr int ST = RB + 1;
@@ -959,7 +981,7 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
     __shared__ double red[NT];
     const int tid = threadIdx.x;
     const int nrows = c.Ly * c.Lz;
-    const int row0 = blockIdx.x * RB;
+    const int row0 = bx * RB;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
     // packed half-spectrum points of this block's rows: all NF fields are
@@ -1133,23 +1155,31 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
     double r[5] = {eunh, ede, ke1, ke2, kea};
     block_sums<5>(r, red);
     if (tid == 0) {
-        a.partials[P_EUNH * nb + blockIdx.x] = r[0];
-        a.partials[P_EDE * nb + blockIdx.x] = r[1];
-        a.partials[P_KE1 * nb + blockIdx.x] = r[2];
-        a.partials[P_KE2 * nb + blockIdx.x] = r[3];
-        a.partials[P_KEA * nb + blockIdx.x] = r[4];
+        a.partials[P_EUNH * nb + bx] = r[0];
+        a.partials[P_EDE * nb + bx] = r[1];
+        a.partials[P_KE1 * nb + bx] = r[2];
+        a.partials[P_KE2 * nb + bx] = r[3];
+        a.partials[P_KEA * nb + bx] = r[4];
     }
+}
+
+template <typename T, int LX, int NF>
+__global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, XinvArgs a,
+                                             const Scalars* __restrict__ sc,
+                                             const cpx<T>* __restrict__ twx)
+{
+    xinv_body<T, LX, NF>(c, f, sp, a, sc, twx, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
 // Kinetic energy of p (optionally after the first Nose-Poincare half kick).
 // ---------------------------------------------------------------------------
 template <int KICK>
-__global__ __launch_bounds__(NT) void k_kick(Consts c, Fields f, const Scalars* __restrict__ sc,
-                                             double* partials, int maxblocks)
+__device__ __forceinline__ void kick_body(const Consts& c, const Fields& f, const Scalars* sc,
+                                             double* partials, int maxblocks, int bx)
 {
     __shared__ double red[NT];
-    const size_t cc = (size_t)blockIdx.x * NT + threadIdx.x;
+    const size_t cc = (size_t)bx * NT + threadIdx.x;
     double ke = 0.0, kea = 0.0;
     if (cc < (size_t)c.N) {
         const unsigned char code = f.code[cc];
@@ -1173,9 +1203,16 @@ __global__ __launch_bounds__(NT) void k_kick(Consts c, Fields f, const Scalars* 
     double r[2] = {ke, kea};
     block_sums<2>(r, red);
     if (threadIdx.x == 0) {
-        partials[P_KEPK * maxblocks + blockIdx.x] = r[0];
-        partials[P_KEA * maxblocks + blockIdx.x] = r[1];
+        partials[P_KEPK * maxblocks + bx] = r[0];
+        partials[P_KEA * maxblocks + bx] = r[1];
     }
+}
+
+template <int KICK>
+__global__ __launch_bounds__(NT) void k_kick(Consts c, Fields f, const Scalars* __restrict__ sc,
+                                             double* partials, int maxblocks)
+{
+    kick_body<KICK>(c, f, sc, partials, maxblocks, blockIdx.x);
 }
 
 // Deterministic fold of the nb per-block partials of NS slots by one
@@ -1274,8 +1311,8 @@ __device__ __forceinline__ void np_begin_scalar(const ScalarConsts& k, Scalars* 
 // (optimize-homo-strain.F:27-85, dVddi-homo-coupling.F:21-26).
 __device__ __forceinline__ void homo_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[3]);
 
-__global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* gsc, const double* partials,
-                                             int maxblocks, int zblocks)
+__device__ __forceinline__ void homo_body(const ScalarConsts& k, Scalars* gsc, const double* partials,
+                                             int maxblocks, int zblocks, int bx)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
@@ -1286,6 +1323,12 @@ __global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* gsc, const
     fold_slots<3>(partials, maxblocks, zblocks, slots, r, red);
     if (threadIdx.x == 0) homo_scalar(k, &lsc, r);
     scalars_out(gsc, &lsc);
+}
+
+static __global__ __launch_bounds__(NT) void k_homo(ScalarConsts k, Scalars* gsc, const double* partials,
+                                             int maxblocks, int zblocks)
+{
+    homo_body(k, gsc, partials, maxblocks, zblocks, blockIdx.x);
 }
 
 __device__ __forceinline__ void homo_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[3])
@@ -1361,9 +1404,9 @@ enum FinishMode { FIN_FORCE = 0, FIN_NP = 1, FIN_LF = 2, FIN_VS = 3 };
 __device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[5],
                                               const double (&rz)[2], int mode, int acoustic, int next_step);
 
-__global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* gsc, const double* partials,
+__device__ __forceinline__ void finish_body(const ScalarConsts& k, Scalars* gsc, const double* partials,
                                                int maxblocks, int xblocks, int zblocks, int mode, int acoustic,
-                                               int next_step)
+                                               int next_step, int bx)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
@@ -1377,6 +1420,13 @@ __global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* gsc, con
     fold_slots<2>(partials, maxblocks, zblocks, zslots, rz, red);
     if (threadIdx.x == 0) finish_scalar(k, &lsc, r, rz, mode, acoustic, next_step);
     scalars_out(gsc, &lsc);
+}
+
+static __global__ __launch_bounds__(NT) void k_finish(ScalarConsts k, Scalars* gsc, const double* partials,
+                                               int maxblocks, int xblocks, int zblocks, int mode, int acoustic,
+                                               int next_step)
+{
+    finish_body(k, gsc, partials, maxblocks, xblocks, zblocks, mode, acoustic, next_step, blockIdx.x);
 }
 
 __device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc, const double (&r)[5],
@@ -1419,8 +1469,8 @@ __device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc
 
 // First half of the first Nose-Poincare step of a call: KE from the kick
 // kernel's partials, then eq. 38.
-__global__ __launch_bounds__(NT) void k_np_begin(ScalarConsts k, Scalars* gsc, const double* partials,
-                                                 int maxblocks, int kblocks)
+__device__ __forceinline__ void np_begin_body(const ScalarConsts& k, Scalars* gsc, const double* partials,
+                                                 int maxblocks, int kblocks, int bx)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
@@ -1436,9 +1486,15 @@ __global__ __launch_bounds__(NT) void k_np_begin(ScalarConsts k, Scalars* gsc, c
     scalars_out(gsc, &lsc);
 }
 
+static __global__ __launch_bounds__(NT) void k_np_begin(ScalarConsts k, Scalars* gsc, const double* partials,
+                                                 int maxblocks, int kblocks)
+{
+    np_begin_body(k, gsc, partials, maxblocks, kblocks, blockIdx.x);
+}
+
 // Kinetic energies of the current momenta for the energy report
 // (kinetic_energy_dipo.F, kinetic_energy_acou.F).
-__global__ __launch_bounds__(NT) void k_ke_report(ScalarConsts k, Scalars* sc, const double* partials,
+static __global__ __launch_bounds__(NT) void k_ke_report(ScalarConsts k, Scalars* sc, const double* partials,
                                                   int maxblocks, int kblocks)
 {
 #pragma clang fp contract(off)
@@ -1452,9 +1508,9 @@ __global__ __launch_bounds__(NT) void k_ke_report(ScalarConsts k, Scalars* sc, c
 }
 
 // Apply the pending velocity-scaling factors so the host sees scaled momenta.
-__global__ __launch_bounds__(NT) void k_apply_scale(Consts c, Fields f, Scalars* sc)
+__device__ __forceinline__ void apply_scale_body(const Consts& c, const Fields& f, const Scalars* sc, int bx)
 {
-    const size_t cc = (size_t)blockIdx.x * NT + threadIdx.x;
+    const size_t cc = (size_t)bx * NT + threadIdx.x;
     if (cc >= (size_t)c.N) return;
     const double sp = sc->scale_p;
 #pragma unroll
@@ -1466,7 +1522,12 @@ __global__ __launch_bounds__(NT) void k_apply_scale(Consts c, Fields f, Scalars*
     }
 }
 
-__global__ void k_reset_scale(Scalars* sc)
+static __global__ __launch_bounds__(NT) void k_apply_scale(Consts c, Fields f, Scalars* sc)
+{
+    apply_scale_body(c, f, sc, blockIdx.x);
+}
+
+static __global__ void k_reset_scale(Scalars* sc)
 {
     sc->scale_p = 1.0;
     sc->scale_pw = 1.0;
@@ -1492,11 +1553,11 @@ struct AvgAcc {
 };
 
 template <int HIST>
-__global__ __launch_bounds__(NT) void k_avg_cells(Consts c, Fields f, double* partials, int maxblocks,
-                                                  unsigned long long* dist1, int* dist3, int ravg)
+__device__ __forceinline__ void avg_cells_body(const Consts& c, const Fields& f, double* partials, int maxblocks,
+                                                  unsigned long long* dist1, int* dist3, int ravg, int bx)
 {
     __shared__ double red[NT];
-    const size_t cc = (size_t)blockIdx.x * NT + threadIdx.x;
+    const size_t cc = (size_t)bx * NT + threadIdx.x;
     double v[12];
 #pragma unroll
     for (int q = 0; q < 12; ++q) v[q] = 0.0;
@@ -1533,12 +1594,19 @@ __global__ __launch_bounds__(NT) void k_avg_cells(Consts c, Fields f, double* pa
     block_sums<12>(v, red);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int q = 0; q < 12; ++q) partials[(P_AVG + q) * maxblocks + blockIdx.x] = v[q];
+        for (int q = 0; q < 12; ++q) partials[(P_AVG + q) * maxblocks + bx] = v[q];
     }
 }
 
-__global__ __launch_bounds__(NT) void k_avg_fold(const Scalars* __restrict__ sc, AvgAcc* acc,
-                                                 const double* partials, int maxblocks, int kblocks)
+template <int HIST>
+__global__ __launch_bounds__(NT) void k_avg_cells(Consts c, Fields f, double* partials, int maxblocks,
+                                                  unsigned long long* dist1, int* dist3, int ravg)
+{
+    avg_cells_body<HIST>(c, f, partials, maxblocks, dist1, dist3, ravg, blockIdx.x);
+}
+
+__device__ __forceinline__ void avg_fold_body(const Scalars* sc, AvgAcc* acc,
+                                                 const double* partials, int maxblocks, int kblocks, int bx)
 {
 #pragma clang fp contract(off)
     __shared__ double red[NT];
@@ -1576,6 +1644,12 @@ __global__ __launch_bounds__(NT) void k_avg_fold(const Scalars* __restrict__ sc,
     *acc = a;
 }
 
+static __global__ __launch_bounds__(NT) void k_avg_fold(const Scalars* __restrict__ sc, AvgAcc* acc,
+                                                 const double* partials, int maxblocks, int kblocks)
+{
+    avg_fold_body(sc, acc, partials, maxblocks, kblocks, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Set-up: Ewald real-space sum (dipole-dipole-long-range.F:47-91) and the
 // reciprocal-space sum plus scaling (dipole-dipole-long-range.F:107-150).
@@ -1600,7 +1674,7 @@ __device__ __forceinline__ double C_decay(double r, double rr, double kappa)
 }
 
 // Real-space part for the 6 upper-triangle components (xx yy zz yz zx xy).
-__global__ __launch_bounds__(NT) void k_ewald_real(Consts c, EwaldArgs e, double* out /*[6][N]*/)
+static __global__ __launch_bounds__(NT) void k_ewald_real(Consts c, EwaldArgs e, double* out /*[6][N]*/)
 {
 #pragma clang fp contract(off)
     const size_t cc = (size_t)blockIdx.x * NT + threadIdx.x;
@@ -1769,6 +1843,123 @@ __global__ void k_prep_w(Consts c, Spectra<T> sp, const cpx<double>* acouK)
         }
         sp.s[q][p] = cpx<T>{(T)v.x, (T)v.y};
     }
+}
+
+// ---------------------------------------------------------------------------
+// Replica batches.  Replicas of one size class (edges, precision, acoustic
+// degrees of freedom) share every kernel instantiation and grid, so each stage
+// of a step is one launch for all of them: blockIdx.z picks the replica, whose
+// arguments sit in a device array of Rep records; the stage bodies are the
+// single-replica ones.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Rep {
+    Consts c;
+    ScalarConsts k;
+    Fields f;
+    Spectra<T> sp;
+    ZArgs<T> za;                 // acouK set; a z pass given want_acouK = 0 drops it
+    XfwdArgs xa;
+    XinvArgs ia;                 // materialize set per launch
+    Scalars* sc;
+    double* partials;
+    const cpx<T>* twx;
+    const cpx<T>* twy;
+    const cpx<T>* twz;
+    AvgAcc* acc;
+    unsigned long long* dist1;
+    int* dist3;
+    int maxblocks, xblocks, zblocks, kblocks, ravg;
+};
+
+template <typename T, int LX>
+__global__ __launch_bounds__(NT) void kb_xfwd(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    xfwd_body<T, LX>(r.c, r.f, r.sp, r.xa, r.sc, r.twx, blockIdx.x);
+}
+
+template <typename T, int LY, int SIGN>
+__global__ __launch_bounds__(NT) void kb_ypass(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    ypass_body<T, LY, SIGN>(r.c, r.sp, 0, r.twy, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int LZ, int GROUP>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GROUP == ZG_ACOU ? 1 : 2)))
+void kb_zpass(const Rep<T>* __restrict__ reps, int want_acouK)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    ZArgs<T> a = r.za;
+    if (!want_acouK) a.acouK = nullptr;
+    zpass_body<T, LZ, GROUP>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x);
+}
+
+template <typename T, int LX, int NF>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF == 12 ? 2 : 3))) void kb_xinv(const Rep<T>* __restrict__ reps, int materialize)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    XinvArgs a = r.ia;
+    a.materialize = materialize;
+    xinv_body<T, LX, NF>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_homo(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    homo_body(r.k, r.sc, r.partials, r.maxblocks, r.zblocks, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_finish(const Rep<T>* __restrict__ reps, int mode, int next_step)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    finish_body(r.k, r.sc, r.partials, r.maxblocks, r.xblocks, r.zblocks, mode, r.c.acoustic, next_step, 0);
+}
+
+template <typename T, int KICK>
+__global__ __launch_bounds__(NT) void kb_kick(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    kick_body<KICK>(r.c, r.f, r.sc, r.partials, r.maxblocks, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_np_begin(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    np_begin_body(r.k, r.sc, r.partials, r.maxblocks, r.kblocks, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_apply_scale(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    apply_scale_body(r.c, r.f, r.sc, blockIdx.x);
+}
+
+template <typename T>
+__global__ void kb_reset_scale(const Rep<T>* __restrict__ reps)
+{
+    Scalars* sc = reps[blockIdx.z].sc;
+    sc->scale_p = 1.0;
+    sc->scale_pw = 1.0;
+}
+
+template <typename T, int HIST>
+__global__ __launch_bounds__(NT) void kb_avg_cells(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    avg_cells_body<HIST>(r.c, r.f, r.partials, r.maxblocks, r.dist1, r.dist3, r.ravg, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_avg_fold(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    avg_fold_body(r.sc, r.acc, r.partials, r.maxblocks, r.kblocks, 0);
 }
 
 }  // namespace fhip

@@ -167,6 +167,23 @@ module feram_hip_binding
        type(fhip_average), intent(out) :: avg
        type(c_ptr), value :: distribution, distribution3d
      end function fhip_avg_get
+     ! Replica batches: ctxs(1:n) are the contexts of a temperature / field sweep.
+     integer(c_int) function fhip_batch_create(ctxs, n, batch) bind(C, name='fhip_batch_create')
+       import :: c_int, c_ptr
+       type(c_ptr), intent(in) :: ctxs(*)
+       integer(c_int), value :: n
+       type(c_ptr), intent(out) :: batch
+     end function fhip_batch_create
+     subroutine fhip_batch_destroy(batch) bind(C, name='fhip_batch_destroy')
+       import :: c_ptr
+       type(c_ptr), value :: batch
+     end subroutine fhip_batch_destroy
+     integer(c_int) function fhip_batch_step(batch, nsteps, e) bind(C, name='fhip_batch_step')
+       import :: c_int, c_ptr, fhip_energy
+       type(c_ptr), value :: batch
+       integer(c_int), value :: nsteps
+       type(fhip_energy), intent(out) :: e(*)
+     end function fhip_batch_step
      integer(c_int) function fhip_synchronize(ctx) bind(C, name='fhip_synchronize')
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

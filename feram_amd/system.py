@@ -245,3 +245,58 @@ class System:
     @property
     def device_bytes(self) -> int:
         return int(self._L.fhip_device_bytes(self.ctx))
+
+
+class Batch:
+    """Replicas stepped together (``fhip_batch``): independent ``System``
+    objects -- e.g. one per temperature and field of a sweep -- whose steps run
+    as one launch per stage and size class, with the middle steps of a call
+    replayed from a captured HIP graph.  Each replica evolves exactly as its own
+    ``System.step`` would; the systems stay usable between batch calls."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        self._L = _lib.load()
+        arr = (C.c_void_p * len(self.systems))(*[s.ctx.value for s in self.systems])
+        b = C.c_void_p()
+        rc = self._L.fhip_batch_create(arr, len(self.systems), C.byref(b))
+        self.b = b
+        if rc != 0:
+            msg = self._L.fhip_batch_last_error(b).decode() if b.value else ""
+            self.close()
+            raise FhipError(f"fhip_batch_create: {_lib.ERRORS.get(rc, rc)}: {msg}")
+
+    def close(self):
+        if getattr(self, "b", None) is not None and self.b.value:
+            self._L.fhip_batch_destroy(self.b)
+        self.b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise FhipError(f"{what}: {_lib.ERRORS.get(rc, rc)}: {self._L.fhip_batch_last_error(self.b).decode()}")
+        return rc
+
+    def step(self, nsteps: int = 1) -> list:
+        """nsteps steps of every replica; the energies of each after the last."""
+        e = (_lib.FhipEnergy * len(self.systems))()
+        self._check(self._L.fhip_batch_step(self.b, int(nsteps), e), "batch_step")
+        return [x.as_dict() for x in e]
+
+    def use_graph(self, enable: bool = True) -> None:
+        self._check(self._L.fhip_batch_graph(self.b, 1 if enable else 0), "batch_graph")
+
+    @property
+    def classes(self) -> int:
+        return int(self._L.fhip_batch_classes(self.b))

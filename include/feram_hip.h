@@ -170,6 +170,34 @@ int fhip_avg_auto(fhip_ctx *ctx, int enable);
  * (199^3 int32, kx fastest) may be NULL. */
 int fhip_avg_get(fhip_ctx *ctx, fhip_average *out, int64_t *distribution, int32_t *distribution3d);
 
+/* Replica batches.  The reference runs one supercell per process and sweeps
+ * temperature or field by running the time loop of molecular_dynamics
+ * (molecular-dynamics.F:98-150) once per configuration; a batch steps many
+ * independent contexts (replicas) together on one device, each exactly as its
+ * own fhip_nose_poincare / fhip_leapfrog / fhip_velocity_scaling call would
+ * (same kernels, same arithmetic, bitwise the same state).  Replicas with the
+ * same edges, precision and acoustic setting form a size class whose stages
+ * run as one launch each; the steps between the first and the last of a call
+ * replay a captured HIP graph.
+ *
+ * Every context of a batch has the same method and edges of at most 64; a
+ * context belongs to at most one batch and must outlive it.  Per-context calls
+ * (put/get, averages, E_add, ...) remain valid between batch calls; their
+ * effects are picked up by the next fhip_batch_step.  Averaging settings
+ * (fhip_avg_auto, histograms) must agree within a size class. */
+typedef struct fhip_batch fhip_batch;
+int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **batch);
+void fhip_batch_destroy(fhip_batch *batch);
+const char *fhip_batch_last_error(const fhip_batch *batch);
+/* nsteps (>= 1) steps of every replica with its context's method; e (NULL or
+ * n entries, in the order of ctxs) receives each replica's energies after the
+ * last step, as the per-context step call would return them. */
+int fhip_batch_step(fhip_batch *batch, int nsteps, fhip_energy *e);
+/* enable = 0: launch every step directly instead of replaying the graph. */
+int fhip_batch_graph(fhip_batch *batch, int enable);
+int fhip_batch_size(const fhip_batch *batch);
+int fhip_batch_classes(const fhip_batch *batch);
+
 /* Measurement: when enabled, HIP events bracket every kernel launch on the
  * context's stream and accumulate per-kernel time.  fhip_kernel_stats fills
  * up to n entries (name, total ms, launches) and returns the entry count. */
