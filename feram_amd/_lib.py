@@ -28,7 +28,10 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_kernel_stats",
            "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
            "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
-           "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size", "fhip_batch_classes")
+           "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size", "fhip_batch_classes",
+           "fhip_mlp_nparams", "fhip_mlp_create", "fhip_mlp_destroy", "fhip_mlp_last_error",
+           "fhip_mlp_set_params", "fhip_mlp_get_params", "fhip_mlp_forward", "fhip_mlp_gradient",
+           "fhip_mlp_train")
 
 
 class FhipParams(C.Structure):
@@ -131,6 +134,21 @@ def load():
     L.fhip_batch_classes.argtypes = [vp]
     for n in ("fhip_batch_create", "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size",
               "fhip_batch_classes"):
+        getattr(L, n).restype = ip
+    fp = C.POINTER(C.c_float)
+    L.fhip_mlp_nparams.argtypes = []
+    L.fhip_mlp_create.argtypes = [C.c_uint64, C.POINTER(vp)]
+    L.fhip_mlp_destroy.argtypes = [vp]
+    L.fhip_mlp_destroy.restype = None
+    L.fhip_mlp_last_error.argtypes = [vp]
+    L.fhip_mlp_last_error.restype = C.c_char_p
+    L.fhip_mlp_set_params.argtypes = [vp, fp]
+    L.fhip_mlp_get_params.argtypes = [vp, fp]
+    L.fhip_mlp_forward.argtypes = [vp, ip, fp, fp]
+    L.fhip_mlp_gradient.argtypes = [vp, ip, fp, fp, fp, fp, fp]
+    L.fhip_mlp_train.argtypes = [vp, ip, fp, fp, fp, ip, C.c_float, fp]
+    for n in ("fhip_mlp_nparams", "fhip_mlp_create", "fhip_mlp_set_params", "fhip_mlp_get_params",
+              "fhip_mlp_forward", "fhip_mlp_gradient", "fhip_mlp_train"):
         getattr(L, n).restype = ip
     for n in ("fhip_set_long_range", "fhip_get_long_range", "fhip_set_modulation", "fhip_set_cells", "fhip_put",
               "fhip_get", "fhip_put_soa", "fhip_get_soa", "fhip_get_nose", "fhip_set_nose",

@@ -18,7 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libferam_hip.so")
-SOURCES = [os.path.join(CSRC, "feram_hip.hip"), os.path.join(CSRC, "fhip_batch.hip")]
+SOURCES = [os.path.join(CSRC, "feram_hip.hip"), os.path.join(CSRC, "fhip_batch.hip"),
+           os.path.join(CSRC, "fhip_mlp.hip")]
 HEADERS = [os.path.join(CSRC, "fhip_fft.hpp"), os.path.join(CSRC, "fhip_kernels.hpp"),
            os.path.join(CSRC, "fhip_host.hpp"), os.path.join(ROOT, "include", "feram_hip.h")]
 ARCH = os.environ.get("FHIP_ARCH", "gfx950")

@@ -1,0 +1,292 @@
+"""Learned warm start: equilibrium dipole statistics predicted from (T, E_z, p).
+
+The reference starts every run from ``init_dipo_avg`` / ``init_dipo_dev``
+(initialize-dipoR.F:40-58) or from the previous run's ``.restart`` file, as
+its heating scans do (03example-BaTiO3-bulk-phase-transition/.../heating.sh),
+and thermalises for tens of thousands of steps.  This module trains a small
+network (``Mlp``: fhip_mlp, BF16 MFMA kernels) on equilibrated states that the
+plain MD produces (``equilibrium_samples``, run as one replica batch) and
+uses it to start new runs near equilibrium (``WarmStart.initialise``).  The
+MD still decides when a run is equilibrated.
+
+Inputs: temperature [K], field E_z [V/A] (along z) and hydrostatic pressure
+[GPa].  Outputs (12): the sorted magnitudes of the mean dipole <u>
+(ascending, so the largest lies along z, the field axis), the per-component
+variance of u over the cells in the same order, and the homogeneous strain
+(xx yy zz yz zx xy) in that frame.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .params import Param
+
+NIN, NOUT = 3, 12
+# inputs are mapped to about [-1, 1] over the trained ranges
+T_MID, T_HALF = 300.0, 150.0
+E_MID, E_HALF = 5e-4, 5e-4
+P_MID, P_HALF = -5.0, 1.0
+
+
+class MlpError(RuntimeError):
+    pass
+
+
+def normalise_inputs(T, Ez, GPa) -> np.ndarray:
+    T, Ez, GPa = np.broadcast_arrays(np.asarray(T, float), np.asarray(Ez, float), np.asarray(GPa, float))
+    return np.stack([(T - T_MID) / T_HALF, (Ez - E_MID) / E_HALF, (GPa - P_MID) / P_HALF], axis=-1)
+
+
+class Mlp:
+    """The device network (fhip_mlp): 3-128-128-12, GELU, BF16 MFMA, FP32 master weights."""
+
+    def __init__(self, seed: int = 0):
+        self._L = _lib.load()
+        q = C.c_void_p()
+        rc = self._L.fhip_mlp_create(C.c_uint64(seed), C.byref(q))
+        self.q = q
+        if rc != 0:
+            msg = self._L.fhip_mlp_last_error(q).decode() if q.value else ""
+            self.close()
+            raise MlpError(f"fhip_mlp_create: {_lib.ERRORS.get(rc, rc)}: {msg}")
+        self.nparams = int(self._L.fhip_mlp_nparams())
+
+    def close(self):
+        if getattr(self, "q", None) is not None and self.q.value:
+            self._L.fhip_mlp_destroy(self.q)
+        self.q = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise MlpError(f"{what}: {_lib.ERRORS.get(rc, rc)}: {self._L.fhip_mlp_last_error(self.q).decode()}")
+
+    @staticmethod
+    def _f32(a, shape=None):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if shape is not None and a.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {a.shape}")
+        return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        self._check(self._L.fhip_mlp_get_params(self.q, out.ctypes.data_as(C.POINTER(C.c_float))), "get_params")
+        return out
+
+    @params.setter
+    def params(self, p) -> None:
+        a, ptr = self._f32(p, (self.nparams,))
+        self._check(self._L.fhip_mlp_set_params(self.q, ptr), "set_params")
+
+    def forward(self, x) -> np.ndarray:
+        x, px = self._f32(x)
+        n = x.shape[0]
+        y = np.zeros((n, NOUT), np.float32)
+        self._check(self._L.fhip_mlp_forward(self.q, n, px, y.ctypes.data_as(C.POINTER(C.c_float))), "forward")
+        return y
+
+    def gradient(self, x, y, w=None):
+        x, px = self._f32(x)
+        y, py = self._f32(y, (x.shape[0], NOUT))
+        w, pw = self._f32(np.ones(NOUT) if w is None else w, (NOUT,))
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        self._check(self._L.fhip_mlp_gradient(self.q, x.shape[0], px, py, pw, C.byref(loss),
+                                              g.ctypes.data_as(C.POINTER(C.c_float))), "gradient")
+        return float(loss.value), g
+
+    def train(self, x, y, w=None, iters: int = 1000, lr: float = 1e-3) -> np.ndarray:
+        x, px = self._f32(x)
+        y, py = self._f32(y, (x.shape[0], NOUT))
+        w, pw = self._f32(np.ones(NOUT) if w is None else w, (NOUT,))
+        hist = np.zeros(iters, np.float32)
+        self._check(self._L.fhip_mlp_train(self.q, x.shape[0], px, py, pw, int(iters), float(lr),
+                                           hist.ctypes.data_as(C.POINTER(C.c_float))), "train")
+        return hist
+
+
+# ---- canonical frame of an equilibrium state -------------------------------------
+_SHEAR = {(1, 2): 3, (2, 1): 3, (2, 0): 4, (0, 2): 4, (0, 1): 5, (1, 0): 5}
+
+
+def canonical(mean_u, var_u, strain):
+    """Rotate (permute and reflect the axes of) an equilibrium state so that
+    |<u>| ascends from x to z and every <u>_a >= 0; returns the 12 targets."""
+    mean_u, var_u, strain = np.asarray(mean_u), np.asarray(var_u), np.asarray(strain)
+    perm = np.argsort(np.abs(mean_u), kind="stable")          # new axis a <- old axis perm[a]
+    sign = np.where(mean_u >= 0, 1.0, -1.0)
+    m = np.abs(mean_u[perm])
+    v = var_u[perm]
+    e = np.empty(6)
+    e[:3] = strain[perm]
+    for (a, b), k in ((pair, _SHEAR[pair]) for pair in ((1, 2), (2, 0), (0, 1))):
+        pa, pb = perm[a], perm[b]
+        e[k] = strain[_SHEAR[(pa, pb)]] * sign[pa] * sign[pb]
+    return np.concatenate([m, v, e])
+
+
+def equilibrium_samples(conditions, L: int = 12, n_therm: int = 3000, n_avg: int = 1000,
+                        snapshots: int = 4, precision: int = 0, seed: int = 0, base: Param | None = None,
+                        batch_size: int = 256):
+    """Equilibrate one replica per condition (T [K], E_z [V/A], GPa) with plain
+    NP-MD from the reference's default start (init_dipo_avg 0.1, dev 0.02 per
+    component, p = 0), all replicas in one batch, and return the 12 canonical
+    targets per replica: <u> from the device averages over the last n_avg steps,
+    the variance of u over the cells averaged over `snapshots` states, and the
+    averaged homogeneous strain."""
+    from .params import batio3_zvr
+    from .system import Batch, System
+    conditions = np.asarray(conditions, dtype=float)
+    out = np.zeros((len(conditions), NOUT))
+    rng = np.random.default_rng(seed)
+    for c0 in range(0, len(conditions), batch_size):
+        chunk = conditions[c0:c0 + batch_size]
+        systems = []
+        for T, Ez, GPa in chunk:
+            p = batio3_zvr(L, method="md", kelvin=float(T)) if base is None else base.copy(kelvin=float(T))
+            p.GPa = [float(GPa)] * 3
+            p.external_E_field = [0.0, 0.0, float(Ez)]
+            p.E_field_always_zero = float(Ez) == 0.0
+            p.Q_Nose = 0.1 * p.N / 16 ** 3
+            g = System(p, precision=precision)
+            g.put("dipoR", 0.1 + 0.02 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
+            g.md_prologue()
+            systems.append(g)
+        with Batch(systems) as bt:
+            bt.step(n_therm)
+            for g in systems:
+                g.avg_reset()
+                g.avg_auto(True)
+            var = np.zeros((len(systems), 3))
+            per = max(1, n_avg // snapshots)
+            for _ in range(snapshots):
+                bt.step(per)
+                for i, g in enumerate(systems):
+                    var[i] += g.get("dipoR").reshape(-1, 3).var(axis=0)
+            var /= snapshots
+        for i, g in enumerate(systems):
+            a = g.avg_get()
+            n = a["n_add"] * g.N
+            mean_u = np.array(a["u"]) / n
+            strain = np.array(a["strain"]) / a["n_add"]
+            out[c0 + i] = canonical(mean_u, var[i], strain)
+            g.close()
+    return out
+
+
+class WarmStart:
+    """A trained network plus the target normalisation."""
+
+    def __init__(self, seed: int = 0):
+        self.net = Mlp(seed)
+        self.y_mean = np.zeros(NOUT)
+        self.y_std = np.ones(NOUT)
+
+    def fit(self, conditions, targets, iters: int = 3000, lr: float = 2e-3) -> np.ndarray:
+        x = normalise_inputs(*np.asarray(conditions, float).T)
+        targets = np.asarray(targets, float)
+        self.y_mean = targets.mean(axis=0)
+        self.y_std = np.maximum(targets.std(axis=0), 1e-12)
+        y = (targets - self.y_mean) / self.y_std
+        return self.net.train(x, y, iters=iters, lr=lr)
+
+    def predict(self, T, Ez, GPa) -> np.ndarray:
+        x = normalise_inputs(T, Ez, GPa).reshape(-1, NIN)
+        return self.net.forward(x) * self.y_std + self.y_mean
+
+    def initialise(self, system, T: float, Ez: float, GPa: float, seed: int = 0,
+                   momenta: bool = True) -> np.ndarray:
+        """u(R) = <u> + sqrt(var) * N(0, 1) per component in the canonical frame
+        (largest mean along z, all means >= 0); with momenta, p from the
+        Maxwell-Boltzmann distribution at T (p_a ~ N(0, m k T)).  Returns the
+        prediction."""
+        pred = self.predict(T, Ez, GPa)[0]
+        m, v = np.maximum(pred[:3], 0.0), np.maximum(pred[3:6], 0.0)
+        p = system.p
+        rng = np.random.default_rng(seed)
+        shape = (p.Lz, p.Ly, p.Lx, 3)
+        system.put("dipoR", m + np.sqrt(v) * rng.standard_normal(shape))
+        if momenta:
+            kT = T / 11604.45                                   # eV (define.h:33)
+            mass = np.asarray(p.mass_dipo, float)
+            system.put("dipoP", np.sqrt(mass * kT) * rng.standard_normal(shape))
+        return pred
+
+
+# ---- what the warm start gains ------------------------------------------------------
+POTENTIAL = ("short_range", "long_range", "dipole_E_field", "unharmonic", "homo_strain", "homo_coupling",
+             "inho_strain", "inho_coupling", "inho_modulation")
+
+
+def equilibration_index(series, tail: float = 0.5, window: int = 5, band: float = 4.0) -> int:
+    """First record after which the smoothed series stays within `band`
+    standard deviations of its mean over the last `tail` of the run."""
+    s = np.asarray(series, float)
+    k = np.ones(window) / window
+    sm = np.convolve(s, k, mode="valid")
+    t0 = int(len(sm) * (1 - tail))
+    mu, sd = sm[t0:].mean(), max(sm[t0:].std(), 1e-12 * max(abs(sm[t0:].mean()), 1.0))
+    out = np.abs(sm - mu) > band * sd
+    bad = np.nonzero(out)[0]
+    return 0 if len(bad) == 0 else int(bad[-1] + 1)
+
+
+def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, record: int = 10,
+                 Ez: float = 0.0, GPa: float = -5.0, precision: int = 0, seed: int = 1,
+                 base: Param | None = None) -> dict:
+    """Run every temperature twice in one batch -- from the reference's default
+    start (init_dipo_avg 0.1, dev 0.02, p = 0) and from the warm start -- record
+    the potential energy per cell every `record` steps, and return per
+    temperature the equilibration step of both runs and <u> (sorted
+    magnitudes) averaged over the second half of each run."""
+    from .params import batio3_zvr
+    from .system import Batch, System
+    rng = np.random.default_rng(seed)
+    systems, kinds = [], []
+    for T in temps:
+        for kind in ("cold", "warm"):
+            p = batio3_zvr(L, method="md", kelvin=float(T)) if base is None else base.copy(kelvin=float(T))
+            p.GPa = [float(GPa)] * 3
+            p.external_E_field = [0.0, 0.0, float(Ez)]
+            p.E_field_always_zero = float(Ez) == 0.0
+            p.Q_Nose = 0.1 * p.N / 16 ** 3
+            g = System(p, precision=precision)
+            if kind == "cold":
+                g.put("dipoR", 0.1 + 0.02 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
+            else:
+                ws.initialise(g, float(T), float(Ez), float(GPa), seed=int(rng.integers(1 << 31)))
+            g.md_prologue()
+            systems.append(g)
+            kinds.append(kind)
+    nrec = n_steps // record
+    series = np.zeros((len(systems), nrec))
+    with Batch(systems) as bt:
+        for r in range(nrec):
+            if r == nrec // 2:
+                for g in systems:
+                    g.avg_reset()
+                    g.avg_auto(True)
+            es = bt.step(record)
+            for i, e in enumerate(es):
+                series[i, r] = sum(e[t] for t in POTENTIAL) / systems[i].N
+    res = {"temps": list(map(float, temps)), "record": record, "n_steps": n_steps, "L": L,
+           "cold": {"tau": [], "u": []}, "warm": {"tau": [], "u": []}}
+    for i, g in enumerate(systems):
+        a = g.avg_get()
+        u = np.sort(np.abs(np.array(a["u"]) / (a["n_add"] * g.N)))
+        res[kinds[i]]["tau"].append(equilibration_index(series[i]) * record)
+        res[kinds[i]]["u"].append(u.tolist())
+        g.close()
+    res["series"] = series
+    tc, tw = sum(res["cold"]["tau"]), sum(res["warm"]["tau"])
+    res["gain"] = tc / max(tw, record)
+    return res
