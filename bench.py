@@ -217,6 +217,96 @@ def bench_replicas(args, sync_extra):
     print(json.dumps(out))
 
 
+def bench_warmstart(args, sync_extra):
+    """Learned warm start on a heating scan: train the network on equilibrated
+    replicas produced by batched plain MD, then run the scan from the
+    reference's default start and from the warm start in one batch."""
+    from feram_amd.warmstart import WarmStart, equilibrium_samples, heating_scan
+    prec = 1 if args.precision == "fp32" else 0
+    rng = np.random.default_rng(20241203)
+    n = args.train_replicas
+    cond = np.stack([rng.uniform(150, 500, n), np.zeros(n), np.full(n, -5.0)], axis=1)
+    cond[n // 2:, 1] = rng.uniform(0, 1e-3, n - n // 2)
+    cond[n // 2:, 2] = rng.uniform(-6, -4, n - n // 2)
+    print(f"[bench] warm start: {n} training replicas {args.train_size}^3", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    targets = equilibrium_samples(cond, L=args.train_size, n_therm=args.train_therm, n_avg=args.train_avg,
+                                  precision=prec, seed=5)
+    sync_extra()
+    t_data = time.perf_counter() - t0
+    ws = WarmStart(seed=2)
+    t0 = time.perf_counter()
+    hist = ws.fit(cond, targets, iters=args.train_iters, lr=2e-3)
+    t_fit = time.perf_counter() - t0
+    temps = np.linspace(250, 450, args.temps)
+    print(f"[bench] heating scan {args.temps} T at {args.size}^3, {args.scan_steps} steps", file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    res = heating_scan(ws, temps, L=args.size, n_steps=args.scan_steps, record=10, precision=prec, seed=7)
+    t_scan = time.perf_counter() - t0
+    res.pop("series")
+    print(f"[bench] gain {res['gain']:.2f}", file=sys.stderr, flush=True)
+    out = {
+        "metric": f"thermalisation-step gain of learned warm starts, {args.temps}-temperature "
+                  f"{args.size}^3 heating scan (sum of equilibration steps, default start / warm start)",
+        "value": res["gain"], "unit": "x", "n_gpus": 1, "steps": args.scan_steps, "warmup": 0,
+        "ms_per_step": 1e3 * t_scan / args.scan_steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if prec else "f64", "data": "synthetic",
+        "config": {"workload": f"BaTiO3 bulk {args.size}^3 NP-MD heating scan 250-450 K, E=0, -5 GPa; "
+                               f"network trained on {n} replicas {args.train_size}^3 "
+                               f"({args.train_therm}+{args.train_avg} steps each)",
+                   "train_replicas": n, "train_size": args.train_size, "scan_size": args.size},
+        "training": {"data_s": t_data, "fit_s": t_fit, "iters": args.train_iters,
+                     "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
+        "scan_s": t_scan,
+        "scan": res,
+    }
+    print(json.dumps(out))
+
+
+def bench_heating16(args, sync_extra):
+    """The reference's own 16^3 BaTiO3 heating scan (its example protocol:
+    170-350 K in 5 K steps, 40000 + 10000 NP-MD steps per temperature, each
+    started from the previous final state) for four seeds in one replica
+    batch, compared with the .avg file the reference ships."""
+    from feram_amd.sweeps import phase_of, temperature_chain
+    with open(os.path.join(ROOT, "tests", "golden", "bto16_phase_transition.json")) as f:
+        ref = {r[0]: r for r in json.load(f)["heating"]}
+    temps = np.arange(170, 351, 5)
+    seeds = tuple(range(1, args.seeds + 1))
+    prec = 1 if args.precision == "fp32" else 0
+    t0 = time.perf_counter()
+
+    def progress(i, T, u):
+        print(f"[bench] {T:.0f} K  t={time.perf_counter() - t0:.0f}s", file=sys.stderr, flush=True)
+
+    res = temperature_chain(temps, seeds=seeds, n_thermalize=args.scan_steps, n_average=args.avg_steps,
+                            precision=prec, progress=progress)
+    sync_extra()
+    dt = time.perf_counter() - t0
+    rows = []
+    for i, T in enumerate(res["temps"]):
+        r = ref[T]
+        rows.append({"T": T, "u_sorted": res["u_sorted_mean"][i].tolist(), "u_err": res["u_sorted_err"][i].tolist(),
+                     "phases": res["phases"][i], "ref_u_sorted": np.sort(np.abs(r[7:10])).tolist(),
+                     "ref_phase": phase_of(np.sort(np.abs(r[7:10])))})
+    agree = sum(1 for r in rows if all(ph == r["ref_phase"] for ph in r["phases"]))
+    nrep = len(temps) * len(seeds)
+    steps = args.scan_steps + args.avg_steps
+    out = {
+        "metric": "equilibrated replicas/hr, 16^3 BaTiO3 heating scan with the reference's example protocol",
+        "value": 3600.0 * nrep / dt, "unit": "replicas/hr", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * dt / (len(temps) * steps), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if prec else "f64", "data": "synthetic",
+        "config": {"workload": f"BaTiO3 bulk 16^3, -5 GPa, NP-MD, 170-350 K / 5 K, {args.scan_steps}+"
+                               f"{args.avg_steps} steps per temperature, chained, {len(seeds)} seeds batched",
+                   "replicas": nrep, "replica_steps_per_s": nrep * steps / dt},
+        "phase_agreement": f"{agree}/{len(rows)} temperatures with every seed in the reference's phase",
+        "scan": rows, "wall_s": dt,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,10 +317,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
-    ap.add_argument("--workload", choices=("md128", "replicas"), default="md128",
+    ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16"), default="md128",
                     help="md128: the single-supercell NP-MD step (default); replicas: the "
-                         "batched replica sweep")
+                         "batched replica sweep; warmstart: the learned warm start on a heating scan; "
+                         "heating16: the reference's 16^3 heating-scan example against its .avg output")
+    ap.add_argument("--seeds", type=int, default=4)
+    ap.add_argument("--avg-steps", type=int, default=10000)
     ap.add_argument("--replicas", type=int, default=48)
+    ap.add_argument("--temps", type=int, default=20)
+    ap.add_argument("--scan-steps", type=int, default=4000)
+    ap.add_argument("--train-replicas", type=int, default=1024)
+    ap.add_argument("--train-size", type=int, default=16)
+    ap.add_argument("--train-therm", type=int, default=4000)
+    ap.add_argument("--train-avg", type=int, default=2000)
+    ap.add_argument("--train-iters", type=int, default=3000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 launches) to calibrate "
                          "the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -262,6 +362,16 @@ def main():
 
     if args.workload == "replicas":
         bench_replicas(args, sync_extra)
+        return
+    if args.workload == "heating16":
+        if args.scan_steps == 4000:
+            args.scan_steps = 40000
+        bench_heating16(args, sync_extra)
+        return
+    if args.workload == "warmstart":
+        if args.size == 128:
+            args.size = 64
+        bench_warmstart(args, sync_extra)
         return
 
     from feram_amd.params import batio3_zvr

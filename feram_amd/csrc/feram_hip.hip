@@ -895,6 +895,16 @@ int fhip_set_E_field_add(fhip_ctx *ctx, const double E[3])
     return push_scalars(ctx);
 }
 
+int fhip_set_kelvin(fhip_ctx *ctx, double kelvin)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (!(kelvin >= 0.0)) return fail(ctx, FHIP_ERR_ARG, "kelvin must be >= 0");
+    ctx->p.kelvin = kelvin;
+    ctx->sk.T_eV = kelvin / KELVIN_EV;                       // param_module.F:226
+    ctx->sk.target_ke = kelvin * 1.5 * ctx->N / KELVIN_EV;   // param_module.F:468
+    return FHIP_OK;
+}
+
 int fhip_force(fhip_ctx *ctx, fhip_energy *e)
 {
     if (!ctx) return FHIP_ERR_ARG;

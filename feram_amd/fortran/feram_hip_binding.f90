@@ -9,16 +9,19 @@ module feram_hip_binding
   use, intrinsic :: iso_c_binding
   implicit none
   private
-  public :: fhip_params, fhip_energy
+  public :: fhip_params, fhip_energy, fhip_average
   public :: fhip_create, fhip_destroy, fhip_last_error, fhip_set_long_range, fhip_get_long_range, &
        &    fhip_set_modulation, fhip_set_cells, fhip_put, fhip_get, fhip_get_nose, fhip_set_nose, &
        &    fhip_get_strain, fhip_set_E_field_add, fhip_force, fhip_md_prologue, &
-       &    fhip_nose_poincare, fhip_leapfrog, fhip_velocity_scaling, fhip_synchronize
+       &    fhip_nose_poincare, fhip_leapfrog, fhip_velocity_scaling, fhip_synchronize, &
+       &    fhip_set_kelvin, fhip_avg_reset, fhip_avg_add, fhip_avg_auto, fhip_avg_get, &
+       &    fhip_batch_create, fhip_batch_destroy, fhip_batch_step
   public :: FHIP_DIPO_R, FHIP_DIPO_P, FHIP_DVDDI, FHIP_ACOU_R, FHIP_ACOU_P, FHIP_DVDAC, &
-       &    FHIP_E_LOCAL, FHIP_ACOU_R_OPT, fhip_error_message
+       &    FHIP_E_LOCAL, FHIP_ACOU_R_OPT, FHIP_DIPO_R_AVG, fhip_error_message
 
   integer(c_int), parameter :: FHIP_DIPO_R = 0, FHIP_DIPO_P = 1, FHIP_DVDDI = 2, FHIP_ACOU_R = 3, &
-       &                       FHIP_ACOU_P = 4, FHIP_DVDAC = 5, FHIP_E_LOCAL = 6, FHIP_ACOU_R_OPT = 7
+       &                       FHIP_ACOU_P = 4, FHIP_DVDAC = 5, FHIP_E_LOCAL = 6, FHIP_ACOU_R_OPT = 7, &
+       &                       FHIP_DIPO_R_AVG = 8
 
   ! include/feram_hip.h: struct fhip_params (same member order)
   type, bind(C) :: fhip_params
@@ -167,6 +170,11 @@ module feram_hip_binding
        type(fhip_average), intent(out) :: avg
        type(c_ptr), value :: distribution, distribution3d
      end function fhip_avg_get
+     integer(c_int) function fhip_set_kelvin(ctx, kelvin) bind(C, name='fhip_set_kelvin')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: kelvin
+     end function fhip_set_kelvin
      ! Replica batches: ctxs(1:n) are the contexts of a temperature / field sweep.
      integer(c_int) function fhip_batch_create(ctxs, n, batch) bind(C, name='fhip_batch_create')
        import :: c_int, c_ptr

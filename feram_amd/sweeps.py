@@ -1,0 +1,72 @@
+"""Temperature scans as the reference's example drivers run them, on replica batches.
+
+The reference's heating and cooling scans (03example-BaTiO3-bulk-phase-transition/
+16x16x16-40000-10000/heating.sh, cooling.sh) run one feram process per
+temperature; each starts from the previous temperature's final state (its
+.coord written with p / s_Nose, coord_module.F:474-479, read back as the
+.restart by initialize-dipoR.F:22-37), thermalises n_thermalize steps and
+averages n_average steps (molecular-dynamics.F:98-150).  ``temperature_chain``
+runs that protocol for several independent seeds at once: the seeds are
+replicas of one batch, so every temperature of every chain is one batched
+call, and the seed-to-seed spread gives the error bars of <u>(T).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .params import Param
+
+
+def phase_of(u_sorted, threshold: float = 0.04) -> str:
+    """Name of the ferroelectric phase from the sorted |<u>| (ascending):
+    R (three components polarised), O (two), T (one) or C (none)."""
+    n = int(np.sum(np.asarray(u_sorted) > threshold))
+    return "CTOR"[n]
+
+
+def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: int = 16,
+                      n_thermalize: int = 40000, n_average: int = 10000, precision: int = 0,
+                      init_avg: float = 0.1, init_dev: float = 0.02, progress=None) -> dict:
+    """Chained NP-MD scan over `temps` for every seed; returns per temperature
+    and seed <u> (3), its sorted magnitudes and the averaged homogeneous
+    strain (6), plus the seed means and standard errors of the sorted |<u>|."""
+    from .params import batio3_zvr
+    from .system import Batch, System
+    temps = [float(t) for t in temps]
+    systems = []
+    for sd in seeds:
+        p = batio3_zvr(L, method="md", kelvin=temps[0]) if base is None else base.copy(kelvin=temps[0])
+        g = System(p, precision=precision)
+        rng = np.random.default_rng(sd)
+        g.put("dipoR", init_avg + init_dev * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
+        systems.append(g)
+    u = np.zeros((len(temps), len(seeds), 3))
+    strain = np.zeros((len(temps), len(seeds), 6))
+    with Batch(systems) as bt:
+        for it, T in enumerate(temps):
+            for g in systems:
+                if it > 0:
+                    # the next run reads p / s_Nose from the previous .coord
+                    s = g.nose[0]
+                    g.put("dipoP", g.get("dipoP") / s)
+                g.set_kelvin(T)
+                g.md_prologue()
+            bt.step(n_thermalize)
+            for g in systems:
+                g.avg_reset()
+                g.avg_auto(True)
+            bt.step(n_average)
+            for j, g in enumerate(systems):
+                a = g.avg_get()
+                g.avg_auto(False)
+                u[it, j] = np.array(a["u"]) / (a["n_add"] * g.N)
+                strain[it, j] = np.array(a["strain"]) / a["n_add"]
+            if progress:
+                progress(it, T, u[it])
+    for g in systems:
+        g.close()
+    us = np.sort(np.abs(u), axis=2)
+    err = us.std(axis=1, ddof=1) / np.sqrt(len(seeds)) if len(seeds) > 1 else np.full(us.shape[::2], np.nan)
+    return {"temps": temps, "seeds": list(seeds), "u": u, "u_sorted": us, "strain": strain,
+            "u_sorted_mean": us.mean(axis=1), "u_sorted_err": err,
+            "phases": [[phase_of(us[i, j]) for j in range(len(seeds))] for i in range(len(temps))]}

@@ -164,6 +164,10 @@ class System:
         e = (C.c_double * 3)(*[float(v) for v in E])
         self._check(self._L.fhip_set_E_field_add(self.ctx, e), "set_E_field_add")
 
+    def set_kelvin(self, kelvin: float) -> None:
+        self._check(self._L.fhip_set_kelvin(self.ctx, float(kelvin)), "set_kelvin")
+        self.p = self.p.copy(kelvin=float(kelvin))
+
     # ---- the step ------------------------------------------------------------
     def mirror_force_mirror(self) -> dict:
         e = _lib.FhipEnergy()

@@ -242,12 +242,13 @@ def equilibration_index(series, tail: float = 0.5, window: int = 5, band: float 
 
 def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, record: int = 10,
                  Ez: float = 0.0, GPa: float = -5.0, precision: int = 0, seed: int = 1,
-                 base: Param | None = None) -> dict:
+                 base: Param | None = None, blocks: int = 5) -> dict:
     """Run every temperature twice in one batch -- from the reference's default
     start (init_dipo_avg 0.1, dev 0.02, p = 0) and from the warm start -- record
     the potential energy per cell every `record` steps, and return per
-    temperature the equilibration step of both runs and <u> (sorted
-    magnitudes) averaged over the second half of each run."""
+    temperature the equilibration step of both runs and the sorted magnitudes
+    of <u> over the second half of each run, as the mean and standard error of
+    `blocks` block averages."""
     from .params import batio3_zvr
     from .system import Batch, System
     rng = np.random.default_rng(seed)
@@ -268,25 +269,42 @@ def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, recor
             systems.append(g)
             kinds.append(kind)
     nrec = n_steps // record
+    half = nrec // 2
+    per_block = max(1, (nrec - half) // blocks)
     series = np.zeros((len(systems), nrec))
+    usum_prev = np.zeros((len(systems), 3))
+    nadd_prev = np.zeros(len(systems))
+    block_u = [[] for _ in systems]
     with Batch(systems) as bt:
         for r in range(nrec):
-            if r == nrec // 2:
+            if r == half:
                 for g in systems:
                     g.avg_reset()
                     g.avg_auto(True)
             es = bt.step(record)
             for i, e in enumerate(es):
                 series[i, r] = sum(e[t] for t in POTENTIAL) / systems[i].N
+            if r >= half and (r + 1 - half) % per_block == 0:
+                for i, g in enumerate(systems):
+                    a = g.avg_get()
+                    u = np.array(a["u"])
+                    dn = a["n_add"] - nadd_prev[i]
+                    if dn > 0:
+                        block_u[i].append(np.sort(np.abs((u - usum_prev[i]) / (dn * g.N))))
+                    usum_prev[i], nadd_prev[i] = u, a["n_add"]
     res = {"temps": list(map(float, temps)), "record": record, "n_steps": n_steps, "L": L,
-           "cold": {"tau": [], "u": []}, "warm": {"tau": [], "u": []}}
+           "cold": {"tau": [], "u": [], "u_err": []}, "warm": {"tau": [], "u": [], "u_err": []}}
     for i, g in enumerate(systems):
-        a = g.avg_get()
-        u = np.sort(np.abs(np.array(a["u"]) / (a["n_add"] * g.N)))
+        b = np.array(block_u[i])
         res[kinds[i]]["tau"].append(equilibration_index(series[i]) * record)
-        res[kinds[i]]["u"].append(u.tolist())
+        res[kinds[i]]["u"].append(b.mean(axis=0).tolist())
+        res[kinds[i]]["u_err"].append((b.std(axis=0, ddof=1) / np.sqrt(len(b))).tolist() if len(b) > 1
+                                      else [float("nan")] * 3)
         g.close()
     res["series"] = series
     tc, tw = sum(res["cold"]["tau"]), sum(res["warm"]["tau"])
     res["gain"] = tc / max(tw, record)
+    uc, uw = np.array(res["cold"]["u"]), np.array(res["warm"]["u"])
+    ec, ew = np.array(res["cold"]["u_err"]), np.array(res["warm"]["u_err"])
+    res["P_within_2sigma"] = (np.abs(uc - uw) <= 2 * np.sqrt(ec ** 2 + ew ** 2) + 1e-3).all(axis=1).tolist()
     return res

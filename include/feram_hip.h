@@ -129,6 +129,10 @@ int fhip_set_nose(fhip_ctx *ctx, double s, double pi, double H0);
 int fhip_get_strain(fhip_ctx *ctx, double strain[6]);
 /* Uniform part of E(R) = E_local(R) + E_add (molecular-dynamics.F:66-76). */
 int fhip_set_E_field_add(fhip_ctx *ctx, const double E_add[3]);
+/* Thermostat / velocity-scaling temperature p%kelvin, as a new run of a
+ * temperature scan reads it (param_module.F:224-226,468); the next
+ * fhip_md_prologue starts the run. */
+int fhip_set_kelvin(fhip_ctx *ctx, double kelvin);
 
 int fhip_force(fhip_ctx *ctx, fhip_energy *e);
 int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e);
