@@ -26,21 +26,31 @@ def phase_of(u_sorted, threshold: float = 0.04) -> str:
 
 def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: int = 16,
                       n_thermalize: int = 40000, n_average: int = 10000, precision: int = 0,
-                      init_avg: float = 0.1, init_dev: float = 0.02, progress=None) -> dict:
-    """Chained NP-MD scan over `temps` for every seed; returns per temperature
-    and seed <u> (3), its sorted magnitudes and the averaged homogeneous
-    strain (6), plus the seed means and standard errors of the sorted |<u>|."""
+                      init_avg: float = 0.1, init_dev: float = 0.02, progress=None,
+                      bases=None, variance: bool = False) -> dict:
+    """Chained NP-MD scan over `temps` for every seed (or, with `bases`, one
+    chain per given Param -- other fields, pressures -- seeded 1, 2, ...);
+    returns per temperature and chain <u> (3), its sorted magnitudes and the
+    averaged homogeneous strain (6), the chain means and standard errors of
+    the sorted |<u>|, and with `variance` the per-component variance of u over
+    the cells of the final state of each temperature."""
     from .params import batio3_zvr
     from .system import Batch, System
     temps = [float(t) for t in temps]
+    if bases is None:
+        p0 = batio3_zvr(L, method="md", kelvin=temps[0]) if base is None else base.copy(kelvin=temps[0])
+        bases = [p0] * len(seeds)
+    else:
+        seeds = tuple(range(1, len(bases) + 1))
     systems = []
-    for sd in seeds:
-        p = batio3_zvr(L, method="md", kelvin=temps[0]) if base is None else base.copy(kelvin=temps[0])
+    for sd, b in zip(seeds, bases):
+        p = b.copy(kelvin=temps[0])
         g = System(p, precision=precision)
         rng = np.random.default_rng(sd)
         g.put("dipoR", init_avg + init_dev * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
         systems.append(g)
     u = np.zeros((len(temps), len(seeds), 3))
+    var = np.zeros((len(temps), len(seeds), 3))
     strain = np.zeros((len(temps), len(seeds), 6))
     with Batch(systems) as bt:
         for it, T in enumerate(temps):
@@ -61,12 +71,17 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
                 g.avg_auto(False)
                 u[it, j] = np.array(a["u"]) / (a["n_add"] * g.N)
                 strain[it, j] = np.array(a["strain"]) / a["n_add"]
+                if variance:
+                    var[it, j] = g.get("dipoR").reshape(-1, 3).var(axis=0)
             if progress:
                 progress(it, T, u[it])
     for g in systems:
         g.close()
     us = np.sort(np.abs(u), axis=2)
     err = us.std(axis=1, ddof=1) / np.sqrt(len(seeds)) if len(seeds) > 1 else np.full(us.shape[::2], np.nan)
-    return {"temps": temps, "seeds": list(seeds), "u": u, "u_sorted": us, "strain": strain,
-            "u_sorted_mean": us.mean(axis=1), "u_sorted_err": err,
-            "phases": [[phase_of(us[i, j]) for j in range(len(seeds))] for i in range(len(temps))]}
+    out = {"temps": temps, "seeds": list(seeds), "u": u, "u_sorted": us, "strain": strain,
+           "u_sorted_mean": us.mean(axis=1), "u_sorted_err": err,
+           "phases": [[phase_of(us[i, j]) for j in range(len(seeds))] for i in range(len(temps))]}
+    if variance:
+        out["variance"] = var
+    return out

@@ -183,6 +183,35 @@ def equilibrium_samples(conditions, L: int = 12, n_therm: int = 3000, n_avg: int
     return out
 
 
+def chain_samples(temps, fields=(0.0, 2.5e-4, 5e-4, 1e-3), pressures=(-6.0, -5.0, -4.0), seeds: int = 2,
+                  L: int = 16, n_thermalize: int = 6000, n_average: int = 3000, precision: int = 0,
+                  progress=None):
+    """Training data on the heating branch: one chained scan over `temps` per
+    (field, pressure, seed), all chains in one batch (feram_amd.sweeps), and
+    the 12 canonical targets of every (chain, temperature).  Returns
+    (conditions [n][3] = T, E_z, GPa; targets [n][12])."""
+    from .params import batio3_zvr
+    from .sweeps import temperature_chain
+    bases, conds = [], []
+    for Ez in fields:
+        for GPa in pressures:
+            for _ in range(seeds):
+                p = batio3_zvr(L, method="md", kelvin=float(temps[0]))
+                p.GPa = [float(GPa)] * 3
+                p.external_E_field = [0.0, 0.0, float(Ez)]
+                p.E_field_always_zero = float(Ez) == 0.0
+                bases.append(p)
+                conds.append((float(Ez), float(GPa)))
+    res = temperature_chain(temps, bases=bases, n_thermalize=n_thermalize, n_average=n_average,
+                            precision=precision, variance=True, progress=progress)
+    X, Y = [], []
+    for i, T in enumerate(res["temps"]):
+        for j, (Ez, GPa) in enumerate(conds):
+            X.append((T, Ez, GPa))
+            Y.append(canonical(res["u"][i, j], res["variance"][i, j], res["strain"][i, j]))
+    return np.array(X), np.array(Y)
+
+
 class WarmStart:
     """A trained network plus the target normalisation."""
 
@@ -242,7 +271,7 @@ def equilibration_index(series, tail: float = 0.5, window: int = 5, band: float 
 
 def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, record: int = 10,
                  Ez: float = 0.0, GPa: float = -5.0, precision: int = 0, seed: int = 1,
-                 base: Param | None = None, blocks: int = 5) -> dict:
+                 base: Param | None = None, blocks: int = 5, Q_Nose: float | None = None) -> dict:
     """Run every temperature twice in one batch -- from the reference's default
     start (init_dipo_avg 0.1, dev 0.02, p = 0) and from the warm start -- record
     the potential energy per cell every `record` steps, and return per
@@ -259,7 +288,8 @@ def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, recor
             p.GPa = [float(GPa)] * 3
             p.external_E_field = [0.0, 0.0, float(Ez)]
             p.E_field_always_zero = float(Ez) == 0.0
-            p.Q_Nose = 0.1 * p.N / 16 ** 3
+            if Q_Nose is not None:
+                p.Q_Nose = Q_Nose
             g = System(p, precision=precision)
             if kind == "cold":
                 g.put("dipoR", 0.1 + 0.02 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
