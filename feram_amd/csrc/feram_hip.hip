@@ -653,6 +653,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
         ctx->hsc = (Scalars *)h;
         std::memset(ctx->hsc, 0, sizeof(Scalars));
         ctx->hsc->s = 1.0;
+        ctx->hsc->i_step = 1.0;         // molecular_dynamics counts steps from 1
         ctx->hsc->scale_p = 1.0;
         ctx->hsc->scale_pw = 1.0;
         for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = p->external_E_field[q];
@@ -893,6 +894,44 @@ int fhip_set_E_field_add(fhip_ctx *ctx, const double E[3])
     RC(pull_scalars(ctx));
     for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = E[q];
     return push_scalars(ctx);
+}
+
+int fhip_set_E_schedule(fhip_ctx *ctx, int type, const double amplitude[3], int period, int n_thermalize,
+                        int n_average, int64_t i_step)
+{
+    if (!ctx || !amplitude) return FHIP_ERR_ARG;
+    if (type < FHIP_E_CONSTANT || type > FHIP_E_HYSTERESIS) return fail(ctx, FHIP_ERR_ARG, "unknown E schedule");
+    if ((type == FHIP_E_TRIANGULAR_COS || type == FHIP_E_TRIANGULAR_SIN) && period < 1)
+        return fail(ctx, FHIP_ERR_ARG, "triangular waves need n_E_wave_period >= 1");
+    if ((type == FHIP_E_RAMPING_ON || type == FHIP_E_RAMPING_OFF) && (period < 1 || period > n_thermalize))
+        return fail(ctx, FHIP_ERR_ARG, "period must be smaller than n_thermalize (wave_module.F:41-44)");
+    if (type == FHIP_E_HYSTERESIS && n_average < 1) return fail(ctx, FHIP_ERR_ARG, "n_average must be >= 1");
+    if (i_step < 0) return fail(ctx, FHIP_ERR_ARG, "i_step must be >= 0");
+    if (!ctx->c.efield && (amplitude[0] != 0.0 || amplitude[1] != 0.0 || amplitude[2] != 0.0))
+        return fail(ctx, FHIP_ERR_STATE, "the context was created with E_field_always_zero");
+    RC(pull_scalars(ctx));
+    Scalars &h = *ctx->hsc;
+    for (int q = 0; q < 3; ++q) h.e_amp[q] = amplitude[q];
+    h.e_type = type;
+    h.e_period = period;
+    h.e_ntherm = n_thermalize;
+    h.e_navg = n_average;
+    h.i_step = (double)i_step;
+    if (type == FHIP_E_CONSTANT)
+        for (int q = 0; q < 3; ++q) h.E_add[q] = amplitude[q];
+    else
+        e_field_next(&h);
+    return push_scalars(ctx);
+}
+
+int fhip_get_step(fhip_ctx *ctx, int64_t *i_step, double E_add[3])
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    RC(pull_scalars(ctx));
+    if (i_step) *i_step = (int64_t)ctx->hsc->i_step;
+    if (E_add)
+        for (int q = 0; q < 3; ++q) E_add[q] = ctx->hsc->E_add[q];
+    return FHIP_OK;
 }
 
 int fhip_set_kelvin(fhip_ctx *ctx, double kelvin)

@@ -62,8 +62,61 @@ struct Scalars {
     double E_add[3];                 // uniform part of the electric field
     double en_rep[13];               // energies and s at the end of the last step, kept when
     double s_rep;                    //   the next step's eq. 38 has already started
-    double pad[2];
+    // E-field schedule (integers held exactly as doubles): E_add of step i_step
+    double e_amp[3];                 // amplitude (external_E_field)
+    double e_type;                   // ESchedule
+    double e_period, e_ntherm, e_navg;
+    double i_step;                   // index of the next step
 };
+
+// Time dependence of the uniform field.
+enum ESchedule { ES_CONSTANT = 0, ES_TRI_COS = 1, ES_TRI_SIN = 2, ES_RAMP_ON = 3, ES_RAMP_OFF = 4, ES_HYSTERESIS = 5 };
+
+// E_add at step i: wave() of wave_module.F:8-67 (triangular_cos/_sin,
+// ramping_on/_off) and the linear sweep of hysteresis_loop.F:47-49, with the
+// reference's integer and real operation order.  ES_CONSTANT leaves E alone.
+__host__ __device__ inline void e_field_at(int type, const double amp[3], long long period, long long ntherm,
+                                           long long navg, long long i_step, double out[3])
+{
+    if (type == ES_CONSTANT) return;
+    if (amp[0] == 0.0 && amp[1] == 0.0 && amp[2] == 0.0) {   // wave_module.F:19-24
+        for (int q = 0; q < 3; ++q) out[q] = amp[q];
+        return;
+    }
+    switch (type) {
+    case ES_TRI_COS:
+    case ES_TRI_SIN: {
+        long long i = i_step - ntherm;
+        if (i < 0) i = 0;
+        const long long m = (type == ES_TRI_COS) ? (i % period) : ((i + 3 * period / 4) % period);
+        long long d = m - period / 2;
+        if (d < 0) d = -d;
+        const long long tmp = d - period / 4;
+        for (int q = 0; q < 3; ++q) out[q] = amp[q] * (double)tmp / ((double)period / 4.0);
+        return;
+    }
+    case ES_RAMP_ON:
+        for (int q = 0; q < 3; ++q) out[q] = i_step < period ? amp[q] * (double)i_step / (double)period : amp[q];
+        return;
+    case ES_RAMP_OFF:
+        for (int q = 0; q < 3; ++q)
+            out[q] = i_step < period ? amp[q] * (double)(period - i_step) / (double)period : 0.0;
+        return;
+    case ES_HYSTERESIS:
+        for (int q = 0; q < 3; ++q)
+            out[q] = i_step >= ntherm ? amp[q] - 2.0 * amp[q] * (double)(i_step - ntherm) / (double)navg : amp[q];
+        return;
+    default:
+        for (int q = 0; q < 3; ++q) out[q] = amp[q];
+        return;
+    }
+}
+
+__host__ __device__ inline void e_field_next(Scalars* sc)
+{
+    e_field_at((int)sc->e_type, sc->e_amp, (long long)sc->e_period, (long long)sc->e_ntherm,
+               (long long)sc->e_navg, (long long)sc->i_step, sc->E_add);
+}
 enum EnergyIdx {
     E_ACOU_KIN = 0, E_DIPO_KIN, E_SR, E_LR, E_DE, E_UNH, E_HS, E_HC, E_IS, E_IC, E_IM, E_TOTAL, E_HNP
 };
@@ -1464,6 +1517,10 @@ __device__ __forceinline__ void finish_scalar(const ScalarConsts& k, Scalars* sc
     en[E_TOTAL] = en[E_DIPO_KIN] / (sc->s * sc->s) + en[E_ACOU_KIN] + potential(en);
     for (int i = 0; i < 13; ++i) sc->en_rep[i] = en[i];
     sc->s_rep = sc->s;
+    if (mode != FIN_FORCE) {            // the next step's field (molecular-dynamics.F:66-76)
+        sc->i_step = sc->i_step + 1.0;
+        e_field_next(sc);
+    }
     if (mode == FIN_NP && next_step) np_begin_scalar(k, sc);
 }
 

@@ -14,7 +14,7 @@ module feram_hip_binding
        &    fhip_set_modulation, fhip_set_cells, fhip_put, fhip_get, fhip_get_nose, fhip_set_nose, &
        &    fhip_get_strain, fhip_set_E_field_add, fhip_force, fhip_md_prologue, &
        &    fhip_nose_poincare, fhip_leapfrog, fhip_velocity_scaling, fhip_synchronize, &
-       &    fhip_set_kelvin, fhip_avg_reset, fhip_avg_add, fhip_avg_auto, fhip_avg_get, &
+       &    fhip_set_kelvin, fhip_set_E_schedule, fhip_get_step, fhip_avg_reset, fhip_avg_add, fhip_avg_auto, fhip_avg_get, &
        &    fhip_batch_create, fhip_batch_destroy, fhip_batch_step
   public :: FHIP_DIPO_R, FHIP_DIPO_P, FHIP_DVDDI, FHIP_ACOU_R, FHIP_ACOU_P, FHIP_DVDAC, &
        &    FHIP_E_LOCAL, FHIP_ACOU_R_OPT, FHIP_DIPO_R_AVG, fhip_error_message
@@ -170,6 +170,20 @@ module feram_hip_binding
        type(fhip_average), intent(out) :: avg
        type(c_ptr), value :: distribution, distribution3d
      end function fhip_avg_get
+     integer(c_int) function fhip_set_E_schedule(ctx, type, amplitude, period, n_thermalize, n_average, &
+          &                                      i_step) bind(C, name='fhip_set_E_schedule')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: type, period, n_thermalize, n_average
+       real(c_double), intent(in) :: amplitude(3)
+       integer(c_int64_t), value :: i_step
+     end function fhip_set_E_schedule
+     integer(c_int) function fhip_get_step(ctx, i_step, E_add) bind(C, name='fhip_get_step')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: i_step
+       real(c_double), intent(out) :: E_add(3)
+     end function fhip_get_step
      integer(c_int) function fhip_set_kelvin(ctx, kelvin) bind(C, name='fhip_set_kelvin')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

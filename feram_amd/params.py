@@ -51,6 +51,9 @@ class Param:
     n_average: int = 100000
     external_E_field: list = field(default_factory=lambda: [0.0, 0.0, 0.0])
     E_wave_type: str = ""
+    n_E_wave_period: int = 0              # param_module.F:149
+    n_hl_freq: int = 10000                # param_module.F:151
+    n_hysteresis_loop_continue: int = 1   # param_module.F:152
     P_kappa2: list = field(default_factory=lambda: _three(0.0))
     P_alpha: list = field(default_factory=lambda: _three(0.0))
     P_gamma: list = field(default_factory=lambda: _three(0.0))
@@ -204,8 +207,12 @@ class Param:
                 p.j = nums(7)
             elif name == "seed":
                 p.seed = [int(vals[0]), int(vals[1])]
-            elif name in ("n_E_wave_period", "n_coord_freq", "n_hl_freq",
-                          "n_hysteresis_loop_continue", "coord_directory", "slice_directory",
+            elif name in ("n_E_wave_period", "n_hl_freq", "n_hysteresis_loop_continue"):
+                setattr(p, name, int(first_float()))
+                if (name == "n_E_wave_period" and not p.E_wave_type.startswith("ramping")
+                        and p.n_E_wave_period % 4 != 0):   # param_module.F:284-287
+                    raise ValueError(f"line {lineno}: n_E_wave_period must be divided by 4.")
+            elif name in ("n_coord_freq", "coord_directory", "slice_directory",
                           "distribution_directory", "continue", "plot_dispersion",
                           "output_polarization", "DDB_a", "DDB_b", "DDB_c", "DDB_d", "DDB_e",
                           "DDB_f", "DDB_g"):

@@ -85,3 +85,46 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
     if variance:
         out["variance"] = var
     return out
+
+
+def hl_field(amplitude, n_thermalize: int, n_average: int, i_step: int) -> np.ndarray:
+    """E_field_add of step i_step of the 'hl' driver (hysteresis_loop.F:47-49;
+    the amplitude before n_thermalize, initialize_E_field.F:58-66)."""
+    A = np.asarray(amplitude, dtype=float)
+    if i_step < n_thermalize:
+        return A.copy()
+    return A - 2 * A * float(i_step - n_thermalize) / float(n_average)
+
+
+def hysteresis_loop(systems, n_thermalize: int, n_average: int, n_hl_freq: int, amplitude,
+                    i_start: int = 1, progress=None) -> list:
+    """The 'hl' driver (hysteresis_loop.F:41-96) for one or more supercells:
+    velocity-scaling steps i_start .. n_thermalize + n_average with the field
+    E_add = amplitude before n_thermalize and swept linearly to -amplitude
+    over n_average steps after it (evaluated on the device every step), and,
+    every n_hl_freq steps of the sweep, a .hl row per system: step, kelvin,
+    E_field_add, strain(6), <u> over the sum_p cells.  Several systems run as
+    one replica batch.  Returns one list of rows per system."""
+    from .system import Batch
+    systems = list(systems)
+    for g in systems:
+        if g.p.method != "vs":
+            raise ValueError("the hysteresis loop steps with velocity scaling (method 'vs')")
+        g.set_E_schedule("hysteresis", amplitude, 0, n_thermalize, n_average, i_start)
+    rows = [[] for _ in systems]
+    n_end = n_thermalize + n_average
+    outputs = [i for i in range(n_thermalize, n_end + 1, n_hl_freq) if i >= i_start]
+    with Batch(systems) as bt:
+        i = i_start
+        for out_step in outputs:
+            bt.step(out_step - i + 1)
+            i = out_step + 1
+            E = hl_field(amplitude, n_thermalize, n_average, out_step)
+            for j, g in enumerate(systems):
+                u = g.mean_u()
+                rows[j].append([out_step, g.p.kelvin] + E.tolist() + g.strain.tolist() + u.tolist())
+            if progress:
+                progress(out_step, [r[-1] for r in rows])
+        if i <= n_end:
+            bt.step(n_end - i + 1)
+    return rows

@@ -143,6 +143,13 @@ class System:
         rc = self._L.fhip_set_cells(self.ctx, pad(relaxor, np.int32), pad(sum_p, np.int32),
                                     pad(mass_dipo_inv, np.float64))
         self._check(rc, "set_cells")
+        self._sum_p = None if sum_p is None else np.asarray(sum_p).astype(bool)
+
+    def mean_u(self) -> np.ndarray:
+        """<u> over the sum_p cells (hysteresis_loop.F:85-89)."""
+        u = self.get("dipoR")
+        m = getattr(self, "_sum_p", None)
+        return u.reshape(-1, 3).mean(axis=0) if m is None else u[m].mean(axis=0)
 
     # ---- scalars -----------------------------------------------------------
     @property
@@ -163,6 +170,24 @@ class System:
     def set_E_field_add(self, E) -> None:
         e = (C.c_double * 3)(*[float(v) for v in E])
         self._check(self._L.fhip_set_E_field_add(self.ctx, e), "set_E_field_add")
+
+    E_SCHEDULES = {"constant": 0, "triangular_cos": 1, "triangular_sin": 2, "ramping_on": 3,
+                   "ramping_off": 4, "hysteresis": 5}
+
+    def set_E_schedule(self, kind: str, amplitude, period: int = 0, n_thermalize: int = 0,
+                       n_average: int = 0, i_step: int = 1) -> None:
+        """Device-side E_add(i_step): an E_wave_type of wave_module.F, the
+        hysteresis-loop sweep ('hysteresis') or 'constant'."""
+        a = (C.c_double * 3)(*[float(v) for v in amplitude])
+        self._check(self._L.fhip_set_E_schedule(self.ctx, self.E_SCHEDULES[kind], a, int(period),
+                                                int(n_thermalize), int(n_average), int(i_step)), "set_E_schedule")
+
+    def step_info(self):
+        """(index of the next step, the E_add it will use)."""
+        i = C.c_int64()
+        e = (C.c_double * 3)()
+        self._check(self._L.fhip_get_step(self.ctx, C.byref(i), e), "get_step")
+        return int(i.value), np.array(e[:])
 
     def set_kelvin(self, kelvin: float) -> None:
         self._check(self._L.fhip_set_kelvin(self.ctx, float(kelvin)), "set_kelvin")

@@ -129,6 +129,21 @@ int fhip_set_nose(fhip_ctx *ctx, double s, double pi, double H0);
 int fhip_get_strain(fhip_ctx *ctx, double strain[6]);
 /* Uniform part of E(R) = E_local(R) + E_add (molecular-dynamics.F:66-76). */
 int fhip_set_E_field_add(fhip_ctx *ctx, const double E_add[3]);
+/* Time-dependent uniform field, evaluated on the device at the start of every
+ * step of fhip_nose_poincare / fhip_leapfrog / fhip_velocity_scaling (and of
+ * batches): E_add(i_step) = wave(E_wave_type, i_step, n_thermalize,
+ * n_E_wave_period, external_E_field) as molecular-dynamics.F:66-76 calls
+ * wave_module.F:8-67, or the linear sweep of the hysteresis loop
+ * (hysteresis_loop.F:47-49; amplitude before n_thermalize).  i_step is the
+ * index of the next step (molecular_dynamics counts from 1, hysteresis_loop
+ * from 0); every step advances it.  FHIP_E_CONSTANT holds E_add = amplitude
+ * (as fhip_set_E_field_add). */
+enum { FHIP_E_CONSTANT = 0, FHIP_E_TRIANGULAR_COS = 1, FHIP_E_TRIANGULAR_SIN = 2, FHIP_E_RAMPING_ON = 3,
+       FHIP_E_RAMPING_OFF = 4, FHIP_E_HYSTERESIS = 5 };
+int fhip_set_E_schedule(fhip_ctx *ctx, int type, const double amplitude[3], int period, int n_thermalize,
+                        int n_average, int64_t i_step);
+/* The index of the next step and the E_add it will use. */
+int fhip_get_step(fhip_ctx *ctx, int64_t *i_step, double E_add[3]);
 /* Thermostat / velocity-scaling temperature p%kelvin, as a new run of a
  * temperature scan reads it (param_module.F:224-226,468); the next
  * fhip_md_prologue starts the run. */

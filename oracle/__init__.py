@@ -133,6 +133,9 @@ def lib():
         for n in ("fo_B_decay", "fo_C_decay"):
             getattr(L, n).restype = C.c_double
             getattr(L, n).argtypes = [C.c_double, C.c_double, C.c_double]
+        L.fo_wave.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_long, C.c_long, C.c_long, C.c_long,
+                              C.POINTER(C.c_double)]
+        L.fo_wave.restype = None
         _lib = L
     return _lib
 
@@ -337,3 +340,14 @@ def normal_dist_seed(s1, s2):
 
 def normal_dist(mu, sigma):
     return lib().fo_normal_dist(float(mu), float(sigma))
+
+
+WAVES = {"triangular_cos": 1, "triangular_sin": 2, "ramping_on": 3, "ramping_off": 4, "hysteresis": 5}
+
+
+def wave(kind: str, amplitude, period: int, n_thermalize: int, n_average: int, i_step: int) -> np.ndarray:
+    """fo_wave: the uniform field of step i_step (wave_module.F:8-67, hysteresis_loop.F:47-49)."""
+    a = (C.c_double * 3)(*[float(v) for v in amplitude])
+    out = (C.c_double * 3)()
+    lib().fo_wave(WAVES[kind], a, int(period), int(n_thermalize), int(n_average), int(i_step), out)
+    return np.array(out[:])

@@ -1371,3 +1371,37 @@ void fo_add_to_average(fo_ctx *o, const fo_energy *e, fo_average *a, int64_t *di
     for (int q = 0; q < 3; ++q) { a->u_tmp[q] = ut[q]; a->uu_tmp[q] = uut[q]; a->p_tmp[q] = pt[q]; a->pp_tmp[q] = ppt[q]; }
     a->n_add += 1;
 }
+
+/* wave() of wave_module.F:8-67 and the linear field sweep of
+ * hysteresis_loop.F:47-49 (type 5; the amplitude before n_thermalize),
+ * integer and real operations in the reference's order.  Types: 1
+ * triangular_cos, 2 triangular_sin, 3 ramping_on, 4 ramping_off, 5 the
+ * hysteresis sweep; anything else returns the amplitude (wave's default). */
+void fo_wave(int type, const double amp[3], long period, long n_thermalize, long n_average, long i_step,
+             double out[3])
+{
+    if (amp[0] == 0.0 && amp[1] == 0.0 && amp[2] == 0.0) {          /* :19-24 */
+        for (int q = 0; q < 3; ++q) out[q] = amp[q];
+        return;
+    }
+    if (type == 1 || type == 2) {                                   /* :27-39 */
+        long i = i_step - n_thermalize;
+        if (i < 0) i = 0;
+        long tmp;
+        if (type == 1) tmp = labs(i % period - period / 2) - period / 4;
+        else tmp = labs((i + 3 * period / 4) % period - period / 2) - period / 4;
+        for (int q = 0; q < 3; ++q) out[q] = amp[q] * (double)tmp / ((double)period / 4.0);
+    } else if (type == 3) {                                         /* :40-50 */
+        for (int q = 0; q < 3; ++q) out[q] = i_step < period ? amp[q] * (double)i_step / (double)period : amp[q];
+    } else if (type == 4) {                                         /* :51-61 */
+        for (int q = 0; q < 3; ++q)
+            out[q] = i_step < period ? amp[q] * (double)(period - i_step) / (double)period : 0.0;
+    } else if (type == 5) {                                         /* hysteresis_loop.F:47-49 */
+        for (int q = 0; q < 3; ++q)
+            out[q] = i_step >= n_thermalize
+                         ? amp[q] - 2 * amp[q] * (double)(i_step - n_thermalize) / (double)n_average
+                         : amp[q];
+    } else {
+        for (int q = 0; q < 3; ++q) out[q] = amp[q];
+    }
+}

@@ -134,6 +134,12 @@ void fo_c2r(fo_ctx *o, const double *in, int ncomp, int comp, double *out);
 /* 1-D complex DFT of length n (in place, interleaved re/im), sign -1 or +1. */
 void fo_fft1d(double *x, int n, int sign);
 
+/* Uniform field of step i_step: wave() (wave_module.F:8-67; types 1
+ * triangular_cos, 2 triangular_sin, 3 ramping_on, 4 ramping_off) or the
+ * hysteresis-loop sweep (type 5, hysteresis_loop.F:47-49). */
+void fo_wave(int type, const double amp[3], long period, long n_thermalize, long n_average, long i_step,
+             double out[3]);
+
 /* Erfc-based Ewald decay functions (decay_function_module.f:13-33). */
 double fo_B_decay(double r, double rr, double kappa);
 double fo_C_decay(double r, double rr, double kappa);
