@@ -5,8 +5,8 @@
 // consecutive in HBM load and store coalesced).  Each pass of radix R does the
 // N/R*TB butterflies of the tile: every thread loads its butterflies into
 // registers, the workgroup synchronises, and the results are written back to
-// their self-sorting (Stockham) positions.  Radices 4, 2, 3 and 5 cover the
-// supercell sizes feram runs (powers of two times 3 or 5).
+// their self-sorting (Stockham) positions.  Radices 16, 8, 4, 2, 3 and 5
+// cover the supercell sizes feram runs (powers of two times 3 or 5).
 //
 // Conventions follow FFTW's: forward is e^{-i}, inverse is e^{+i}, neither is
 // normalised.  Twiddles come from a per-size table tw[j] = e^{-2 pi i j / N}
@@ -52,6 +52,55 @@ __device__ __forceinline__ void dft(cpx<T>* v)
         v[2] = csub(s02, s13);
         v[1] = cadd(d02, d13);
         v[3] = csub(d02, d13);
+    } else if constexpr (R == 8) {
+        // 2 x 4: even and odd halves by radix 4, then one radix-2 stage with
+        // the twiddles w^k, w = e^{SIGN 2 pi i / 8}
+        const T r2 = T(0.70710678118654752440);
+        cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4, SIGN>(e);
+        dft<4, SIGN>(o);
+        const cpx<T> o1 = {r2 * (o[1].x - SIGN * o[1].y), r2 * (o[1].y + SIGN * o[1].x)};   // w^1 o1
+        const cpx<T> o2 = mul_si<SIGN>(o[2]);                                             // w^2 o2
+        const cpx<T> o3 = {r2 * (-o[3].x - SIGN * o[3].y), r2 * (-o[3].y + SIGN * o[3].x)}; // w^3 o3
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    } else if constexpr (R == 16) {
+        // 4 x 4: column DFTs over n1 of x[4 n1 + n2], twiddles w^{n2 k1}
+        // (w = e^{SIGN 2 pi i / 16}), row DFTs over n2; X[k1 + 4 k2]
+        const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
+        const T r2 = T(0.70710678118654752440);
+        cpx<T> y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+#pragma unroll
+            for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = v[4 * n1 + n2];
+            dft<4, SIGN>(y[n2]);
+        }
+        // w^e = cos(2 pi e / 16) + SIGN i sin(2 pi e / 16) for the exponents n2 * k1
+        auto tw = [&](cpx<T> z, T c, T s) { return cpx<T>{z.x * c - SIGN * z.y * s, z.y * c + SIGN * z.x * s}; };
+        y[1][1] = tw(y[1][1], c1, s1);      // e = 1
+        y[1][2] = tw(y[1][2], r2, r2);      // e = 2
+        y[1][3] = tw(y[1][3], s1, c1);      // e = 3
+        y[2][1] = tw(y[2][1], r2, r2);      // e = 2
+        y[2][2] = mul_si<SIGN>(y[2][2]);    // e = 4
+        y[2][3] = tw(y[2][3], -r2, r2);     // e = 6
+        y[3][1] = tw(y[3][1], s1, c1);      // e = 3
+        y[3][2] = tw(y[3][2], -r2, r2);     // e = 6
+        y[3][3] = tw(y[3][3], -c1, -s1);    // e = 9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            cpx<T> z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            dft<4, SIGN>(z);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = z[k2];
+        }
     } else if constexpr (R == 3) {
         const T h = T(0.5), s3 = T(0.86602540378443864676);
         cpx<T> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
@@ -77,12 +126,17 @@ __device__ __forceinline__ void dft(cpx<T>* v)
     }
 }
 
-template <int N>
+// Radix of the pass after NS: the largest of 16, 8, 4 that divides what is
+// left and still gives every thread of the workgroup a butterfly (fewer
+// passes, fewer trips through LDS), else 4, 2, 3, 5.
+template <int N, int TB = 1, int NT = 1>
 struct radix_of {
     template <int NS>
     static constexpr int at()
     {
         constexpr int rem = N / NS;
+        if constexpr (rem % 16 == 0 && (N / 16) * TB >= NT) return 16;
+        if constexpr (rem % 8 == 0 && (N / 8) * TB >= NT) return 8;
         return (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : (rem % 5 == 0) ? 5 : 0;
     }
 };
@@ -136,7 +190,7 @@ template <typename T, int N, int TB, int NT, int SIGN, int ST = TB, int NS = 1>
 __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
 {
     if constexpr (NS < N) {
-        constexpr int R = radix_of<N>::template at<NS>();
+        constexpr int R = radix_of<N, TB, NT>::template at<NS>();
         static_assert(R > 1, "FFT length must factor into 2, 3 and 5");
         stockham_pass<T, N, TB, NT, R, NS, SIGN, ST>(buf, tw, tid);
         fft_lds<T, N, TB, NT, SIGN, ST, NS * R>(buf, tw, tid);
