@@ -57,6 +57,8 @@ def kernel_bytes(name: str, N: int, Kh: int, nf: int, ws: int, materialize_frac:
         return 2 * 3 * cw * M + 6 * ws * Kh
     if name == "z_strain":            # 6 spectra in/out
         return 2 * (nf - 3) * cw * M
+    if name == "z_pair":              # both z groups in one launch
+        return 2 * 3 * cw * M + 6 * ws * Kh + 2 * (nf - 3) * cw * M
     if name == "xinv_c2r_local_kick":  # read nf spectra, u, pk, code; write pk (+p, F when materialised)
         return nf * cw * M + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
     return 0.0

@@ -37,7 +37,7 @@ bool size_supported(int n)
 const char *kKernelNames[K_NKERNEL] = {
     "xfwd_drift_r2c", "y_forward", "z_dipole_dipole", "z_strain", "homo_strain", "y_inverse",
     "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup",
-    "average"};
+    "average", "z_pair"};
 
 template <typename P>
 int dalloc(fhip_ctx *ctx, P **ptr, size_t count)
@@ -204,6 +204,34 @@ int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t st = nullptr)
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lz not supported");
 }
 
+// Both z groups of a step in one launch (k_zpair).
+template <typename T, int G2>
+int run_zpair(fhip_ctx *ctx, const ZArgs<T> &a)
+{
+    Consts c = ctx->c;
+    Spectra<T> sp = tset<T>(ctx).sp;
+    Scalars *sc = ctx->dsc;
+    const cpx<T> *tw = tset<T>(ctx).twz;
+    const int nb = ctx->zblocks[tix<T>()];
+    hipStream_t s = ctx->stream;
+    const int nf = (G2 == ZG_STRAIN) ? 6 : 9;      // the larger side buffer of the two groups
+    const int tz = ztile_rows<T>(ctx->Lz);
+    const ZArgs<T> za = a;
+    const size_t shm = 2 * (size_t)((tz - 1) / ctx->Mx + 1) * ctx->Lz * nf * sizeof(cpx<T>);
+    if (shm > 96 * 1024)
+        return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
+    auto lds_ok = [&](const void *fn) { return ensure_lds(ctx, fn, shm); };
+    switch (ctx->Lz) {
+#define FHIP_CASE(n)                                                                           \
+    case n:                                                                                    \
+        RC(lds_ok((const void *)k_zpair<T, n, G2>));                                           \
+        return launch(ctx, K_ZPAIR, [&] { k_zpair<T, n, G2><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
+        FHIP_SIZES(FHIP_CASE)
+#undef FHIP_CASE
+    }
+    return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lz not supported");
+}
+
 
 // ---- averages (add_to_Average) ----------------------------------------------
 int avg_add(fhip_ctx *ctx)
@@ -241,9 +269,8 @@ int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
     RC(run_xfwd<T>(ctx, xa));
     RC(run_ypass<T>(ctx, -1, 0, nf));
     ZArgs<T> za = zargs<T>(ctx, materialize && !ctx->c.acoustic);
-    RC((run_zpass_g<T, ZG_DD>(ctx, za)));
-    if (ctx->c.acoustic) RC((run_zpass_g<T, ZG_ACOU>(ctx, za)));
-    else RC((run_zpass_g<T, ZG_STRAIN>(ctx, za)));
+    if (ctx->c.acoustic) RC((run_zpair<T, ZG_ACOU>(ctx, za)));
+    else RC((run_zpair<T, ZG_STRAIN>(ctx, za)));
     if (materialize && !ctx->c.acoustic) ctx->acouK_valid = true;
     {
         ScalarConsts k = ctx->sk;

@@ -38,7 +38,7 @@ struct BClass {
     int prec = FHIP_FP64;
     int Lx = 0, Ly = 0, Lz = 0, acoustic = 0, nf = 9;
     int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
-    size_t zshm_dd = 0, zshm_g = 0;     // dynamic LDS of the two z passes
+    size_t zshm_g = 0;                  // dynamic LDS of the paired z pass
     std::vector<fhip_ctx *> members;
     void *dreps = nullptr;              // Rep<T>[members.size()] on the device
     std::vector<unsigned char> hreps;   // host image of the same
@@ -164,27 +164,27 @@ int l_ypass(fhip_batch *b, BClass &k, hipStream_t s, int sign)
     return bfail(b, FHIP_ERR_UNSUPPORTED, "Ly not batched");
 }
 
-template <typename T, int G>
-const void *zpass_fn(int Lz)
+template <typename T, int G2>
+const void *zpair_fn(int Lz)
 {
     switch (Lz) {
 #define FHIP_CASE(n) \
-    case n: return (const void *)kb_zpass<T, n, G>;
+    case n: return (const void *)kb_zpair<T, n, G2>;
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return nullptr;
 }
 
-template <typename T, int G>
-int l_zpass(fhip_batch *b, BClass &k, hipStream_t s, int want_acouK)
+template <typename T, int G2>
+int l_zpair(fhip_batch *b, BClass &k, hipStream_t s, int want_acouK)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.zblocks, 1, (unsigned)k.members.size());
-    const size_t shm = (G == ZG_DD) ? k.zshm_dd : k.zshm_g;
+    const size_t shm = k.zshm_g;
     switch (k.Lz) {
 #define FHIP_CASE(n) \
-    case n: kb_zpass<T, n, G><<<g, NT, shm, s>>>(r, want_acouK); BLAUNCH("zpass");
+    case n: kb_zpair<T, n, G2><<<g, NT, shm, s>>>(r, want_acouK); BLAUNCH("zpair");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -219,9 +219,8 @@ int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int ne
     RC(l_xfwd<T>(b, k, s));
     RC(l_ypass<T>(b, k, s, -1));
     const int want = (materialize && !k.acoustic) ? 1 : 0;
-    RC((l_zpass<T, ZG_DD>(b, k, s, want)));
-    if (k.acoustic) RC((l_zpass<T, ZG_ACOU>(b, k, s, want)));
-    else RC((l_zpass<T, ZG_STRAIN>(b, k, s, want)));
+    if (k.acoustic) RC((l_zpair<T, ZG_ACOU>(b, k, s, want)));
+    else RC((l_zpair<T, ZG_STRAIN>(b, k, s, want)));
     kb_homo<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
     {
         hipError_t e = hipGetLastError();
@@ -371,14 +370,11 @@ int prepare_lds(fhip_batch *b, BClass &k)
     // (strain) or 9 (acoustic)
     const int tz = ztile_rows<T>(k.Lz);
     const size_t per_field = 2 * (size_t)((tz - 1) / (k.Lx / 2) + 1) * k.Lz * sizeof(cpx<T>);
-    k.zshm_dd = 3 * per_field;
     k.zshm_g = (k.acoustic ? 9 : 6) * per_field;
     if (k.zshm_g > 96 * 1024)
         return bfail(b, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
     fhip_ctx *ctx = nullptr;
-    int rc = ensure_lds(ctx, zpass_fn<T, ZG_DD>(k.Lz), k.zshm_dd);
-    if (rc == FHIP_OK)
-        rc = ensure_lds(ctx, k.acoustic ? zpass_fn<T, ZG_ACOU>(k.Lz) : zpass_fn<T, ZG_STRAIN>(k.Lz), k.zshm_g);
+    int rc = ensure_lds(ctx, k.acoustic ? zpair_fn<T, ZG_ACOU>(k.Lz) : zpair_fn<T, ZG_STRAIN>(k.Lz), k.zshm_g);
     if (rc != FHIP_OK) return bfail(b, rc, "z pass: LDS attribute");
     return FHIP_OK;
 }

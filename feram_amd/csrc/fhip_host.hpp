@@ -34,7 +34,7 @@ inline int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX);
 
 enum KernelId {
     K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
-    K_KE, K_SCALE, K_SETUP, K_AVG, K_NKERNEL
+    K_KE, K_SCALE, K_SETUP, K_AVG, K_ZPAIR, K_NKERNEL
 };
 
 }  // namespace fhip_host

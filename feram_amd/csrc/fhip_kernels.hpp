@@ -722,23 +722,44 @@ struct ZTile {
     static constexpr int PPT = (EL + NT - 1) / NT;
 };
 
-template <typename T, int LZ, int GROUP>
+// Fields of a z group, fields transformed together (FB tiles side by side in
+// LDS, so that each Stockham pass has FB times the butterflies between two
+// barriers) and the LDS row stride that follows.
+template <int GROUP>
+struct ZGroupShape {
+    static constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
+    static constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
+    static constexpr int FB = (GROUP == ZG_PLAIN) ? 1 : (GROUP == ZG_STRAIN ? ZSTRAIN_FB : 3);
+};
+
+// LDS of a z-pass workgroup, declared by the kernel (one pool when two groups
+// run in the same workgroup).
+template <typename T, int LZ, int NBUF>
+struct ZLds {
+    cpx<T> buf[NBUF];
+    cpx<T> tws[LZ];
+    double red[NT];
+    int lkx[ZTile<T, LZ>::TB], lky[ZTile<T, LZ>::TB], lord[ZTile<T, LZ>::TB];
+};
+
+template <typename T, int LZ, int GROUP, typename L>
 __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp, const ZArgs<T>& a, Scalars* sc,
-                                              const cpx<T>* twz, int bx)
+                                              const cpx<T>* twz, int bx, L& lds)
 {
     constexpr int TB = ZTile<T, LZ>::TB, ST = ZTile<T, LZ>::ST, EL = ZTile<T, LZ>::EL, PPT = ZTile<T, LZ>::PPT;
-    constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
-    constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
-    // fields transformed together: FB tiles side by side in LDS, so that each
-    // Stockham pass has FB times the butterflies between two barriers
-    constexpr int FB = (GROUP == ZG_PLAIN) ? 1 : (GROUP == ZG_STRAIN ? ZSTRAIN_FB : 3);
+    constexpr int NF = ZGroupShape<GROUP>::NF;
+    constexpr int F0 = ZGroupShape<GROUP>::F0;
+    constexpr int FB = ZGroupShape<GROUP>::FB;
     static_assert(NF % FB == 0, "field batches must divide the fields");
     constexpr int STB = FB * TB + 1;
+    static_assert(sizeof(lds.buf) >= sizeof(cpx<T>) * LZ * STB, "LDS pool too small for the group");
     using KT = typename kspace_type<T>::type;
-    __shared__ cpx<T> buf[LZ * STB];
-    __shared__ cpx<T> tws[LZ];
-    __shared__ double red[NT];
-    __shared__ int lkx[TB], lky[TB], lord[TB];    // per pencil: kx, ky, ordinal among the kx = 0 pencils (or -1)
+    cpx<T>* buf = lds.buf;
+    cpx<T>* tws = lds.tws;
+    double* red = lds.red;
+    int* lkx = lds.lkx;   // per pencil: kx, ky, ordinal among the kx = 0 pencils (or -1)
+    int* lky = lds.lky;
+    int* lord = lds.lord;
     extern __shared__ char zdyn[];
     cpx<T>* side = reinterpret_cast<cpx<T>*>(zdyn);   // [ordinal*LZ + t][NF]
     const int tid = threadIdx.x;
@@ -999,10 +1020,48 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
 }
 
 template <typename T, int LZ, int GROUP>
+constexpr int zbuf_size() { return LZ * (ZGroupShape<GROUP>::FB * ZTile<T, LZ>::TB + 1); }
+
+template <typename T, int LZ, int GROUP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GROUP == ZG_ACOU ? 1 : 2))) void k_zpass(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc,
                                               const cpx<T>* __restrict__ twz)
 {
-    zpass_body<T, LZ, GROUP>(c, sp, a, sc, twz, blockIdx.x);
+    __shared__ ZLds<T, LZ, zbuf_size<T, LZ, GROUP>()> lds;
+    zpass_body<T, LZ, GROUP>(c, sp, a, sc, twz, blockIdx.x, lds);
+}
+
+// Both z groups of a step on one tile in one workgroup (the dipole-dipole and
+// the strain or acoustic group read and write disjoint spectra); even
+// workgroups run the dipole-dipole group first and odd ones second, so that
+// half the grid is loading or storing while the other half is in its
+// transforms.
+template <typename T, int LZ, int G2>
+constexpr int zpair_buf_size()
+{
+    return zbuf_size<T, LZ, ZG_DD>() > zbuf_size<T, LZ, G2>() ? zbuf_size<T, LZ, ZG_DD>() : zbuf_size<T, LZ, G2>();
+}
+
+template <typename T, int LZ, int G2>
+__device__ __forceinline__ void zpair_body(const Consts& c, const Spectra<T>& sp, const ZArgs<T>& a, Scalars* sc,
+                                           const cpx<T>* twz, int bx)
+{
+    __shared__ ZLds<T, LZ, zpair_buf_size<T, LZ, G2>()> lds;
+    if (bx & 1) {
+        zpass_body<T, LZ, G2>(c, sp, a, sc, twz, bx, lds);
+        __syncthreads();
+        zpass_body<T, LZ, ZG_DD>(c, sp, a, sc, twz, bx, lds);
+    } else {
+        zpass_body<T, LZ, ZG_DD>(c, sp, a, sc, twz, bx, lds);
+        __syncthreads();
+        zpass_body<T, LZ, G2>(c, sp, a, sc, twz, bx, lds);
+    }
+}
+
+template <typename T, int LZ, int G2>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(G2 == ZG_ACOU ? 1 : 2)))
+void k_zpair(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc, const cpx<T>* __restrict__ twz)
+{
+    zpair_body<T, LZ, G2>(c, sp, a, sc, twz, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1950,7 +2009,18 @@ void kb_zpass(const Rep<T>* __restrict__ reps, int want_acouK)
     const Rep<T>& r = reps[blockIdx.z];
     ZArgs<T> a = r.za;
     if (!want_acouK) a.acouK = nullptr;
-    zpass_body<T, LZ, GROUP>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x);
+    __shared__ ZLds<T, LZ, zbuf_size<T, LZ, GROUP>()> lds;
+    zpass_body<T, LZ, GROUP>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x, lds);
+}
+
+template <typename T, int LZ, int G2>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(G2 == ZG_ACOU ? 1 : 2)))
+void kb_zpair(const Rep<T>* __restrict__ reps, int want_acouK)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    ZArgs<T> a = r.za;
+    if (!want_acouK) a.acouK = nullptr;
+    zpair_body<T, LZ, G2>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x);
 }
 
 template <typename T, int LX, int NF>
