@@ -24,7 +24,8 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_get_long_range",
            "fhip_set_modulation", "fhip_set_cells", "fhip_put", "fhip_get", "fhip_put_soa",
            "fhip_get_soa", "fhip_get_nose", "fhip_set_nose", "fhip_get_strain",
-           "fhip_set_E_field_add", "fhip_set_kelvin", "fhip_set_E_schedule", "fhip_get_step", "fhip_force", "fhip_md_prologue", "fhip_nose_poincare",
+           "fhip_set_E_field_add", "fhip_set_kelvin", "fhip_set_heat_source", "fhip_get_layer_energies",
+           "fhip_set_E_schedule", "fhip_get_step", "fhip_force", "fhip_md_prologue", "fhip_nose_poincare",
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_kernel_stats",
            "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
            "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
@@ -111,6 +112,10 @@ def load():
     L.fhip_set_E_field_add.argtypes = [vp, dp]
     L.fhip_set_kelvin.argtypes = [vp, C.c_double]
     L.fhip_set_kelvin.restype = ip
+    L.fhip_set_heat_source.argtypes = [vp, C.c_double, C.c_double]
+    L.fhip_set_heat_source.restype = ip
+    L.fhip_get_layer_energies.argtypes = [vp, vp, vp, ip]
+    L.fhip_get_layer_energies.restype = ip
     L.fhip_set_E_schedule.argtypes = [vp, ip, dp, ip, ip, ip, C.c_int64]
     L.fhip_set_E_schedule.restype = ip
     L.fhip_get_step.argtypes = [vp, C.POINTER(C.c_int64), dp]

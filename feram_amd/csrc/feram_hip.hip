@@ -251,7 +251,14 @@ int avg_add(fhip_ctx *ctx)
         RC(launch(ctx, K_AVG, [&] { k_avg_cells<0><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); }));
     Scalars *sc = ctx->dsc;
     AvgAcc *acc = ctx->davg;
-    return launch(ctx, K_AVG, [&] { k_avg_fold<<<1, NT, 0, s>>>(sc, acc, part, mb, kb); });
+    RC(launch(ctx, K_AVG, [&] { k_avg_fold<<<1, NT, 0, s>>>(sc, acc, part, mb, kb); }));
+    if (ctx->p.method == FHIP_LF && ctx->delta_e != 0.0) {   // average_module.F:82-85
+        const double *el = ctx->el;
+        double *sum = ctx->el_sum;
+        const int n = 2 * ctx->Lz;
+        RC(launch(ctx, K_AVG, [&] { k_layer_acc<<<1, NT, 0, s>>>(el, sum, n); }));
+    }
+    return FHIP_OK;
 }
 
 // ---- one force evaluation (mirror_force_mirror) inside a step ---------------
@@ -259,10 +266,25 @@ int nforward(const fhip_ctx *ctx) { return ctx->c.acoustic ? 12 : 9; }
 
 enum StepKind { SK_FORCE = 0, SK_NP = 1, SK_LF = 2, SK_VS = 3 };
 
+// Heat-source and sink layers at the start of a leapfrog step (leapfrog.F:38-83).
+int heat_source_layers(fhip_ctx *ctx)
+{
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    const int *nl = ctx->nlayer;
+    double *el = ctx->el;
+    const double de = ctx->delta_e, dea = ctx->delta_e_acou;
+    hipStream_t s = ctx->stream;
+    RC(launch(ctx, K_SCALE, [&] { k_layer_ke<<<ctx->Lz, NT, 0, s>>>(c, f, nl, el); }));
+    const dim3 g((ctx->Lx * ctx->Ly + NT - 1) / NT, 2);
+    return launch(ctx, K_SCALE, [&] { k_layer_scale<<<g, NT, 0, s>>>(c, f, el, de, dea); });
+}
+
 template <typename T>
 int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
 {
     const int nf = nforward(ctx);
+    if (kind == SK_LF && (ctx->delta_e != 0.0 || ctx->delta_e_acou != 0.0)) RC(heat_source_layers(ctx));
     XfwdArgs xa{};
     xa.mode = (kind == SK_NP) ? XF_NP : (kind == SK_LF ? XF_LF : (kind == SK_VS ? XF_VS : XF_FORCE));
     xa.nf = nf;
@@ -793,6 +815,7 @@ int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, 
     if (!ctx) return FHIP_ERR_ARG;
     const fhip_params &p = ctx->p;
     std::vector<unsigned char> h(ctx->N);
+    std::vector<int> layer(ctx->Lz, 0);
     int nsum = 0;
     for (int z = 0; z < ctx->Lz; ++z)
         for (int y = 0; y < ctx->Ly; ++y)
@@ -813,12 +836,18 @@ int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, 
                 h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] =
                     (unsigned char)(s | (frozen ? 4 : 0) | (counted ? 8 : 0));
                 nsum += counted ? 1 : 0;
+                layer[z] += counted ? 1 : 0;
             }
     if (nsum == 0) return fail(ctx, FHIP_ERR_ARG, "no counted cell (sum_p all false)");
     HIPCHK(hipMemcpyAsync(ctx->code, h.data(), ctx->N, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
 
     ctx->sk.n_sum_p = nsum;
+    ctx->nlayer_host = layer;                      // read_defects_u.F:73-75
+    if (ctx->nlayer) {
+        HIPCHK(hipMemcpyAsync(ctx->nlayer, layer.data(), sizeof(int) * ctx->Lz, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return FHIP_OK;
 }
 
@@ -961,6 +990,43 @@ int fhip_get_step(fhip_ctx *ctx, int64_t *i_step, double E_add[3])
     return FHIP_OK;
 }
 
+int fhip_set_heat_source(fhip_ctx *ctx, double delta_e, double delta_e_acou)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (ctx->p.method != FHIP_LF)
+        return fail(ctx, FHIP_ERR_STATE, "heat-source layers act in leapfrog steps (leapfrog.F:38-83)");
+    if (delta_e_acou != 0.0 && !ctx->c.acoustic)
+        return fail(ctx, FHIP_ERR_ARG, "delta_e_acou needs acoustic MD (mass_acou > 0)");
+    if (!ctx->el) {
+        RC(dalloc(ctx, &ctx->el, 2 * (size_t)ctx->Lz));
+        RC(dalloc(ctx, &ctx->el_sum, 2 * (size_t)ctx->Lz));
+        RC(dalloc(ctx, &ctx->nlayer, (size_t)ctx->Lz));
+        if ((int)ctx->nlayer_host.size() != ctx->Lz) ctx->nlayer_host.assign(ctx->Lz, ctx->Lx * ctx->Ly);
+        HIPCHK(hipMemcpyAsync(ctx->nlayer, ctx->nlayer_host.data(), sizeof(int) * ctx->Lz, hipMemcpyHostToDevice,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    for (int z = 0; z < ctx->Lz; ++z)
+        if (ctx->nlayer_host[z] == 0 && delta_e != 0.0)
+            return fail(ctx, FHIP_ERR_ARG, "a z layer without counted cells (sum_p)");
+    ctx->delta_e = delta_e;
+    ctx->delta_e_acou = delta_e_acou;
+    return FHIP_OK;
+}
+
+int fhip_get_layer_energies(fhip_ctx *ctx, double *e_dipo, double *e_acou, int averaged)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    if (!ctx->el) return fail(ctx, FHIP_ERR_STATE, "no heat-source layers (fhip_set_heat_source)");
+    std::vector<double> h(2 * (size_t)ctx->Lz);
+    HIPCHK(hipMemcpyAsync(h.data(), averaged ? ctx->el_sum : ctx->el, sizeof(double) * h.size(),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (e_dipo) std::memcpy(e_dipo, h.data(), sizeof(double) * ctx->Lz);
+    if (e_acou) std::memcpy(e_acou, h.data() + ctx->Lz, sizeof(double) * ctx->Lz);
+    return FHIP_OK;
+}
+
 int fhip_set_kelvin(fhip_ctx *ctx, double kelvin)
 {
     if (!ctx) return FHIP_ERR_ARG;
@@ -1098,6 +1164,7 @@ int fhip_avg_reset(fhip_ctx *ctx, int options)
     }
     ctx->avg_hist = histograms;
     HIPCHK(hipMemsetAsync(ctx->davg, 0, sizeof(AvgAcc), ctx->stream));
+    if (ctx->el_sum) HIPCHK(hipMemsetAsync(ctx->el_sum, 0, sizeof(double) * 2 * ctx->Lz, ctx->stream));
     if (ctx->dist1) {
         HIPCHK(hipMemsetAsync(ctx->dist1, 0, sizeof(unsigned long long) * 3 * DIST_W, ctx->stream));
         HIPCHK(hipMemsetAsync(ctx->dist3, 0, sizeof(int) * (size_t)DIST_W * DIST_W * DIST_W, ctx->stream));

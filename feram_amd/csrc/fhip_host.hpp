@@ -77,6 +77,12 @@ struct fhip_ctx {
     std::vector<void *> allocs;
     int64_t bytes = 0;
     bool acouK_valid = false;
+    // heat-source layers (leapfrog only)
+    double delta_e = 0.0, delta_e_acou = 0.0;
+    double *el = nullptr;             // [2][Lz] e_layer_dipo, e_layer_acou of the last step
+    double *el_sum = nullptr;         // [2][Lz] their sums over the averaging steps
+    int *nlayer = nullptr;            // counted (sum_p) cells per z layer
+    std::vector<int> nlayer_host;
     // measurement
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;

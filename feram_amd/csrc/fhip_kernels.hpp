@@ -1650,6 +1650,68 @@ static __global__ void k_reset_scale(Scalars* sc)
 }
 
 // ---------------------------------------------------------------------------
+// Heat-source and sink layers of leapfrog (leapfrog.F:38-83): the kinetic
+// energy of every z layer (one workgroup per layer), then the momenta of
+// layer Lz/4 scaled up and of layer 3Lz/4 down so that their energies change
+// by +-Lx Ly delta_e (and delta_e_acou for the acoustic momenta).
+// ---------------------------------------------------------------------------
+static __global__ __launch_bounds__(NT) void k_layer_ke(Consts c, Fields f, const int* __restrict__ nlayer, double* el)
+{
+    __shared__ double red[NT];
+    const int iz = blockIdx.x;
+    const int LxLy = c.Lx * c.Ly;
+    double ke = 0.0, kea = 0.0;
+    for (int i = threadIdx.x; i < LxLy; i += NT) {
+        const size_t cc = (size_t)iz * LxLy + i;
+        const unsigned char code = f.code[cc];
+        const int s = code & 3;
+        const double minv = (code & 4) ? 0.0 : c.minv[s];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ke += f.p[q][cc] * f.p[q][cc] * minv;           // leapfrog.F:47-54
+        if (c.acoustic) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) kea += f.pw[q][cc] * f.pw[q][cc] * c.mwinv[s];   // :61-68
+        }
+    }
+    double r[2] = {ke, kea};
+    block_sums<2>(r, red);
+    if (threadIdx.x == 0) {
+        el[iz] = r[0] / 2 * c.Lx * c.Ly / nlayer[iz];                                // :57
+        el[c.Lz + iz] = c.acoustic ? r[1] / 2 : 0.0;                                 // :69, :72
+    }
+}
+
+static __global__ __launch_bounds__(NT) void k_layer_scale(Consts c, Fields f, const double* __restrict__ el,
+                                                    double delta_e, double delta_e_acou)
+{
+    const int LxLy = c.Lx * c.Ly;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= LxLy) return;
+    const int q1 = c.Lz / 4, q3 = q1 * 3;
+    const int iz = blockIdx.y ? q3 : q1;
+    const double sgn = blockIdx.y ? -1.0 : 1.0;
+    const size_t cc = (size_t)iz * LxLy + i;
+    if (delta_e != 0.0) {                                                            // :75-78
+        const double e = el[iz];
+        const double fac = sqrt((e + sgn * ((double)(c.Lx * c.Ly) * delta_e)) / e);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f.p[q][cc] = f.p[q][cc] * fac;
+    }
+    if (delta_e_acou != 0.0 && c.acoustic) {                                         // :80-83
+        const double e = el[c.Lz + iz];
+        const double fac = sqrt((e + sgn * ((double)(c.Lx * c.Ly) * delta_e_acou)) / e);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f.pw[q][cc] = f.pw[q][cc] * fac;
+    }
+}
+
+// Sums of the layer energies over the averaging steps (average_module.F:82-85).
+static __global__ void k_layer_acc(const double* __restrict__ el, double* sum, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sum[i] = sum[i] + el[i];
+}
+
+// ---------------------------------------------------------------------------
 // Averages of the MD steps (average_module.F:57-163): per step, the sums over
 // the counted cells of u, u^2, Z* u and (Z* u)^2, the histograms of u, and
 // the accumulation of energies, strain and the products of the sums.

@@ -14,7 +14,7 @@ module feram_hip_binding
        &    fhip_set_modulation, fhip_set_cells, fhip_put, fhip_get, fhip_get_nose, fhip_set_nose, &
        &    fhip_get_strain, fhip_set_E_field_add, fhip_force, fhip_md_prologue, &
        &    fhip_nose_poincare, fhip_leapfrog, fhip_velocity_scaling, fhip_synchronize, &
-       &    fhip_set_kelvin, fhip_set_E_schedule, fhip_get_step, fhip_avg_reset, fhip_avg_add, fhip_avg_auto, fhip_avg_get, &
+       &    fhip_set_kelvin, fhip_set_heat_source, fhip_get_layer_energies, fhip_set_E_schedule, fhip_get_step, fhip_avg_reset, fhip_avg_add, fhip_avg_auto, fhip_avg_get, &
        &    fhip_batch_create, fhip_batch_destroy, fhip_batch_step
   public :: FHIP_DIPO_R, FHIP_DIPO_P, FHIP_DVDDI, FHIP_ACOU_R, FHIP_ACOU_P, FHIP_DVDAC, &
        &    FHIP_E_LOCAL, FHIP_ACOU_R_OPT, FHIP_DIPO_R_AVG, fhip_error_message
@@ -184,6 +184,17 @@ module feram_hip_binding
        integer(c_int64_t), intent(out) :: i_step
        real(c_double), intent(out) :: E_add(3)
      end function fhip_get_step
+     integer(c_int) function fhip_set_heat_source(ctx, delta_e, delta_e_acou) bind(C, name='fhip_set_heat_source')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: delta_e, delta_e_acou
+     end function fhip_set_heat_source
+     integer(c_int) function fhip_get_layer_energies(ctx, e_dipo, e_acou, averaged) &
+          &                                          bind(C, name='fhip_get_layer_energies')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, e_dipo, e_acou
+       integer(c_int), value :: averaged
+     end function fhip_get_layer_energies
      integer(c_int) function fhip_set_kelvin(ctx, kelvin) bind(C, name='fhip_set_kelvin')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

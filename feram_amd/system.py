@@ -189,6 +189,19 @@ class System:
         self._check(self._L.fhip_get_step(self.ctx, C.byref(i), e), "get_step")
         return int(i.value), np.array(e[:])
 
+    def set_heat_source(self, delta_e: float, delta_e_acou: float = 0.0) -> None:
+        """Heat-source / sink layers of leapfrog (leapfrog.F:38-83)."""
+        self._check(self._L.fhip_set_heat_source(self.ctx, float(delta_e), float(delta_e_acou)), "set_heat_source")
+
+    def layer_energies(self, averaged: bool = False):
+        """(e_layer_dipo, e_layer_acou) of the last step, or their sums over
+        the averaging steps."""
+        Lz = self.p.Lz
+        d, a = np.zeros(Lz), np.zeros(Lz)
+        self._check(self._L.fhip_get_layer_energies(self.ctx, d.ctypes.data, a.ctypes.data, 1 if averaged else 0),
+                    "get_layer_energies")
+        return d, a
+
     def set_kelvin(self, kelvin: float) -> None:
         self._check(self._L.fhip_set_kelvin(self.ctx, float(kelvin)), "set_kelvin")
         self.p = self.p.copy(kelvin=float(kelvin))

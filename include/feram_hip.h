@@ -144,6 +144,17 @@ int fhip_set_E_schedule(fhip_ctx *ctx, int type, const double amplitude[3], int 
                         int n_average, int64_t i_step);
 /* The index of the next step and the E_add it will use. */
 int fhip_get_step(fhip_ctx *ctx, int64_t *i_step, double E_add[3]);
+/* Heat-source and sink layers of leapfrog (leapfrog.F:38-83; param_module.F:88-89
+ * delta_e, delta_e_acou in eV/unitcell): at the start of every leapfrog step
+ * the kinetic energies of the z layers are formed (e_layer_dipo scaled by
+ * Lx Ly / n_sum_layer) and the momenta of layer Lz/4 scaled up and of layer
+ * 3Lz/4 down by sqrt((e +- Lx Ly delta_e)/e).  Leapfrog contexts only. */
+int fhip_set_heat_source(fhip_ctx *ctx, double delta_e, double delta_e_acou);
+/* c%e_layer_dipo, c%e_layer_acou (Lz values each, either may be NULL) of the
+ * last step, or with averaged != 0 their sums over the averaging steps
+ * (avg%e_layer_*, average_module.F:82-85; divide by n_add for the .T_layer
+ * output of average_module.F:225-238). */
+int fhip_get_layer_energies(fhip_ctx *ctx, double *e_dipo, double *e_acou, int averaged);
 /* Thermostat / velocity-scaling temperature p%kelvin, as a new run of a
  * temperature scan reads it (param_module.F:224-226,468); the next
  * fhip_md_prologue starts the run. */

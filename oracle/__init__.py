@@ -135,6 +135,10 @@ def lib():
             getattr(L, n).argtypes = [C.c_double, C.c_double, C.c_double]
         L.fo_wave.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_long, C.c_long, C.c_long, C.c_long,
                               C.POINTER(C.c_double)]
+        L.fo_set_heat_source.argtypes = [vp, C.c_double, C.c_double]
+        L.fo_set_heat_source.restype = None
+        L.fo_layers.argtypes = [vp, C.c_int]
+        L.fo_layers.restype = C.POINTER(C.c_double)
         L.fo_wave.restype = None
         _lib = L
     return _lib
@@ -235,6 +239,15 @@ class Oracle:
 
     def recount_sum_p(self):
         self._L.fo_recount_sum_p(self.ctx)
+
+    def set_heat_source(self, delta_e, delta_e_acou=0.0):
+        """Heat-source / sink layers of leapfrog (leapfrog.F:38-83)."""
+        self._L.fo_set_heat_source(self.ctx, float(delta_e), float(delta_e_acou))
+
+    def layers(self):
+        """(e_layer_dipo, e_layer_acou) of the last leapfrog step, Lz values each."""
+        return tuple(np.ctypeslib.as_array(self._L.fo_layers(self.ctx, w), shape=(self.p.Lz,)).copy()
+                     for w in (0, 1))
 
     @property
     def strain(self):

@@ -48,6 +48,10 @@ struct fo_ctx {
     double homogeneous[36], homogeneous_G[36], homogeneous_G_inv[6];
     double homogeneous_coupling_2[36];  /* col-major (i,j) -> [i + 6*j] */
     double nose[3];                     /* s_Nose, pi_Nose, H0 */
+    /* heat-source layers (leapfrog.F:38-73) */
+    double delta_e, delta_e_acou;
+    double *e_layer_dipo, *e_layer_acou;   /* Lz */
+    int *n_sum_layer;                      /* Lz: counted cells per z layer */
 };
 
 #define H6(a, i, j) ((a)[(i) + 6 * (j)])   /* 0-based (i,j) */
@@ -319,6 +323,8 @@ fo_ctx *fo_create(const fo_params *pp)
     o->long_range = zalloc(9 * Kh); o->short_range = zalloc(9 * Kh);
     o->inhomo_K = zalloc(9 * Kh); o->inhomo_K_G = zalloc(12 * Kh);
     o->inhomo_coup2_K = zalloc(18 * Kh);
+    o->e_layer_dipo = zalloc(p->Lz); o->e_layer_acou = zalloc(p->Lz);
+    o->n_sum_layer = (int *)calloc(p->Lz, sizeof(int));
     /* alloc_Coord (coord_module.F:99-143) and read_defects_u defaults */
     for (size_t c = 0; c < N; ++c) {
         o->mass_dipo_inv[c] = 1.0 / p->mass_dipo[0];
@@ -345,6 +351,12 @@ void fo_recount_sum_p(fo_ctx *o)
     int n = 0;
     for (int c = 0; c < o->N; ++c) n += o->sum_p[c] ? 1 : 0;
     o->n_sum_p = n;
+    const int LxLy = o->p.Lx * o->p.Ly;                     /* read_defects_u.F:73-75 */
+    for (int iz = 0; iz < o->p.Lz; ++iz) {
+        int m = 0;
+        for (int c = 0; c < LxLy; ++c) m += o->sum_p[iz * LxLy + c] ? 1 : 0;
+        o->n_sum_layer[iz] = m;
+    }
 }
 
 void fo_destroy(fo_ctx *o)
@@ -357,6 +369,7 @@ void fo_destroy(fo_ctx *o)
                       o->c_k_modulation_k, o->long_range, o->short_range, o->inhomo_K,
                       o->inhomo_K_G, o->inhomo_coup2_K};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
+    free(o->e_layer_dipo); free(o->e_layer_acou); free(o->n_sum_layer);
     free(o->relaxor); free(o->sum_p); free(o);
 }
 
@@ -1143,11 +1156,64 @@ void fo_nose_poincare(fo_ctx *o, fo_energy *e)
 }
 
 /* leapfrog (leapfrog.F:9-157); heat-source layers (delta_e) not restated. */
+/* Heat-source and sink layers at Lz/4 and 3Lz/4 (leapfrog.F:38-73). */
+static void heat_source_layers(fo_ctx *o)
+{
+    const fo_params *p = &o->p;
+    const int Lx = p->Lx, Ly = p->Ly, Lz = p->Lz;
+    const size_t LxLy = (size_t)Lx * Ly;
+    const int acou = p->mass_acou[0] > 0.0;
+    const double Lx_Ly_delta_e = (double)(Lx * Ly) * o->delta_e;
+    const double Lx_Ly_delta_e_acou = (double)(Lx * Ly) * o->delta_e_acou;
+    const int Lz_1_4 = Lz / 4, Lz_3_4 = Lz_1_4 * 3;
+    if (o->delta_e == 0.0 && o->delta_e_acou == 0.0) return;
+    for (int iz = 0; iz < Lz; ++iz) {                                   /* :44-58 */
+        double el = 0.0;
+        for (size_t c = iz * LxLy; c < (iz + 1) * LxLy; ++c)
+            for (int a = 0; a < 3; ++a) el = el + o->dipoP[3 * c + a] * o->dipoP[3 * c + a] * o->mass_dipo_inv[c];
+        o->e_layer_dipo[iz] = el / 2 * Lx * Ly / o->n_sum_layer[iz];
+    }
+    for (int iz = 0; iz < Lz; ++iz) {                                   /* :59-73 */
+        double ea = 0.0;
+        if (acou)
+            for (size_t c = iz * LxLy; c < (iz + 1) * LxLy; ++c)
+                for (int a = 0; a < 3; ++a) ea = ea + o->acouP[3 * c + a] * o->acouP[3 * c + a] * o->mass_acou_inv[c];
+        o->e_layer_acou[iz] = ea / 2;
+    }
+    if (o->delta_e != 0.0) {                                            /* :75-78 */
+        const double f1 = sqrt((o->e_layer_dipo[Lz_1_4] + Lx_Ly_delta_e) / o->e_layer_dipo[Lz_1_4]);
+        const double f3 = sqrt((o->e_layer_dipo[Lz_3_4] - Lx_Ly_delta_e) / o->e_layer_dipo[Lz_3_4]);
+        for (size_t c = 0; c < LxLy; ++c)
+            for (int a = 0; a < 3; ++a) {
+                o->dipoP[3 * (Lz_1_4 * LxLy + c) + a] *= f1;
+                o->dipoP[3 * (Lz_3_4 * LxLy + c) + a] *= f3;
+            }
+    }
+    if (o->delta_e_acou != 0.0) {                                       /* :80-83 */
+        const double f1 = sqrt((o->e_layer_acou[Lz_1_4] + Lx_Ly_delta_e_acou) / o->e_layer_acou[Lz_1_4]);
+        const double f3 = sqrt((o->e_layer_acou[Lz_3_4] - Lx_Ly_delta_e_acou) / o->e_layer_acou[Lz_3_4]);
+        for (size_t c = 0; c < LxLy; ++c)
+            for (int a = 0; a < 3; ++a) {
+                o->acouP[3 * (Lz_1_4 * LxLy + c) + a] *= f1;
+                o->acouP[3 * (Lz_3_4 * LxLy + c) + a] *= f3;
+            }
+    }
+}
+
+void fo_set_heat_source(fo_ctx *o, double delta_e, double delta_e_acou)
+{
+    o->delta_e = delta_e;
+    o->delta_e_acou = delta_e_acou;
+}
+
+double *fo_layers(fo_ctx *o, int which) { return which == 0 ? o->e_layer_dipo : o->e_layer_acou; }
+
 void fo_leapfrog(fo_ctx *o, fo_energy *e, int i_step)
 {
     const fo_params *p = &o->p;
     const size_t N = o->N;
     const int acou = p->mass_acou[0] > 0.0;
+    heat_source_layers(o);
     if (i_step == 0) {
         fo_mirror_force_mirror(o, e);
         for (size_t i = 0; i < 3 * N; ++i) o->dipoP[i] = o->dipoP[i] - o->dt_2 * o->dVddi[i];

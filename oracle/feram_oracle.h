@@ -92,6 +92,11 @@ double *fo_nose(fo_ctx *o);                  /* {s_Nose, pi_Nose, H0} */
 void fo_mirror_force_mirror(fo_ctx *o, fo_energy *e);      /* mirror_force_mirror.F:6-89 */
 void fo_nose_poincare(fo_ctx *o, fo_energy *e);            /* nose-poincare.F:10-78 */
 void fo_leapfrog(fo_ctx *o, fo_energy *e, int i_step);     /* leapfrog.F:9-157 */
+/* Heat-source / sink layers of leapfrog (leapfrog.F:38-83): delta_e and
+ * delta_e_acou [eV/unitcell]; fo_layers(o, 0|1) are e_layer_dipo /
+ * e_layer_acou (Lz values) of the last step. */
+void fo_set_heat_source(fo_ctx *o, double delta_e, double delta_e_acou);
+double *fo_layers(fo_ctx *o, int which);
 void fo_velocity_scaling(fo_ctx *o, fo_energy *e);         /* velocity_scaling.F:9-103 */
 double fo_kinetic_energy_dipo(fo_ctx *o);                  /* kinetic_energy_dipo.F:1-29 */
 double fo_kinetic_energy_acou(fo_ctx *o);                  /* kinetic_energy_acou.F:1-28 */

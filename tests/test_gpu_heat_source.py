@@ -1,0 +1,62 @@
+"""Leapfrog with heat-source and sink layers (leapfrog.F:38-83) against the oracle.
+
+A 10-step leapfrog trajectory with delta_e (and, with acoustic MD,
+delta_e_acou) on an 8x6x16 supercell with a sum_p mask: positions, momenta
+and the layer energies of every step must follow the oracle (pinned in
+tests/test_oracle_heat_source.py) to the trajectory tolerance, and the
+averaged layer sums must equal the sum of the per-step values.
+"""
+import numpy as np
+import pytest
+
+from feram_amd.params import batio3_zvr
+
+pytestmark = pytest.mark.gpu
+
+TRAJ_TOL = 1e-10
+
+
+@pytest.mark.parametrize("acoustic", [False, True])
+def test_leapfrog_heat_source_matches_oracle(oracle_mod, acoustic):
+    from feram_amd.system import System
+    print(f"[gpu] heat-source layers, acoustic={acoustic}", flush=True)
+    p = batio3_zvr((8, 6, 16), method="lf", kelvin=300.0, acoustic_mass_amu=46.44 if acoustic else None)
+    o = oracle_mod.Oracle(p)
+    g = System(p)
+    rng = np.random.default_rng(9)
+    u = 0.06 + 0.05 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+    pm = 0.1 * rng.standard_normal(u.shape)
+    sum_p = np.ones((p.Lz, p.Ly, p.Lx), np.int32)
+    sum_p[4, 0, :3] = 0
+    o.sum_p[...] = sum_p.astype(np.uint8)
+    o.recount_sum_p()
+    g.set_cells(sum_p=sum_p)
+    for name, v in (("dipoR", u), ("dipoP", pm)):
+        o.field(name)[...] = v
+        g.put(name, v)
+    if acoustic:
+        pw = 0.05 * rng.standard_normal(u.shape)
+        o.field("acouP")[...] = pw
+        g.put("acouP", pw)
+    de, dea = 0.002, (0.001 if acoustic else 0.0)
+    o.set_heat_source(de, dea)
+    g.set_heat_source(de, dea)
+    e = o.mirror_force_mirror()
+    g.mirror_force_mirror()
+    g.avg_reset()
+    g.avg_auto(True)
+    sums = np.zeros((2, p.Lz))
+    for step in range(10):
+        o.leapfrog(e, 1)
+        g.leapfrog(1, 1)
+        ed, ea = g.layer_energies()
+        od, oa = o.layers()
+        sums += (ed, ea)
+        assert np.allclose(ed, od, rtol=1e-11, atol=0), step
+        assert np.allclose(ea, oa, rtol=1e-11, atol=1e-300), step
+    for name in ("dipoR", "dipoP") + (("acouR", "acouP") if acoustic else ()):
+        a, b = g.get(name), o.field(name)
+        assert np.max(np.abs(a - b)) <= TRAJ_TOL * np.max(np.abs(b)), name
+    sd, sa = g.layer_energies(averaged=True)
+    assert np.allclose(sd, sums[0], rtol=1e-12) and np.allclose(sa, sums[1], rtol=1e-12, atol=1e-300)
+    g.close()
