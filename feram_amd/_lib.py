@@ -50,6 +50,7 @@ class FhipParams(C.Structure):
         ("B1xx", C.c_double), ("B1yy", C.c_double), ("B4yz", C.c_double),
         ("modulation_constant", C.c_double), ("Z_star", C.c_double * 3),
         ("epsilon_inf", C.c_double), ("kappa", C.c_double), ("external_E_field", C.c_double * 3),
+        ("gap_id", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

@@ -288,7 +288,13 @@ int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
     XfwdArgs xa{};
     xa.mode = (kind == SK_NP) ? XF_NP : (kind == SK_LF ? XF_LF : (kind == SK_VS ? XF_VS : XF_FORCE));
     xa.nf = nf;
+    const bool film = ctx->c.mir_lo < ctx->Lz;
+    xa.phase = film ? XP_SOURCE : XP_ALL;
     RC(run_xfwd<T>(ctx, xa));
+    if (film) {                                   // mirror_force_mirror.F:20-27
+        xa.phase = XP_MIRROR;
+        RC(run_xfwd<T>(ctx, xa));
+    }
     RC(run_ypass<T>(ctx, -1, 0, nf));
     ZArgs<T> za = zargs<T>(ctx, materialize && !ctx->c.acoustic);
     if (ctx->c.acoustic) RC((run_zpair<T, ZG_ACOU>(ctx, za)));
@@ -309,6 +315,15 @@ int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
     ia.partials = ctx->partials;
     ia.maxblocks = ctx->maxblocks;
     RC(run_xinv<T>(ctx, ia, nf));
+    if (film) {                                   // mirror_force_mirror.F:80-84
+        Consts c = ctx->c;
+        Fields f = ctx->f;
+        const Scalars *sc = ctx->dsc;
+        const int mode = ia.mode, mat = ia.materialize;
+        const int nb = (int)(((size_t)(ctx->Lz - c.mir_lo) * ctx->Lx * ctx->Ly + NT - 1) / NT);
+        hipStream_t s = ctx->stream;
+        RC(launch(ctx, K_XINV, [&] { k_mirror_kick<<<nb, NT, 0, s>>>(c, f, sc, mode, mat); }));
+    }
     {
         ScalarConsts k = ctx->sk;
         Scalars *sc = ctx->dsc;
@@ -395,6 +410,14 @@ void fill_consts(fhip_ctx *ctx)
     c.efield = !p.E_field_always_zero;
     c.has_eloc = 0;
     c.modulation = 0;
+    // film/epit mirror (mirror.F:18-33): without dead layers iz = Lz/2 .. Lz-1
+    // copy Lz-1-iz, with them iz = Lz-(Lz+1)/2+1 .. Lz-1 copy Lz-iz
+    c.mir_lo = p.Lz;
+    c.mir_src = 0;
+    if (p.bulk_or_film == FHIP_FILM || p.bulk_or_film == FHIP_EPIT) {
+        c.mir_lo = p.gap_id == 0 ? p.Lz - p.Lz / 2 : p.Lz - (p.Lz + 1) / 2 + 1;
+        c.mir_src = p.gap_id == 0 ? p.Lz - 1 : p.Lz;
+    }
     c.Ninv = 1.0 / ctx->N;
     c.dt = p.dt;
     c.dt_2 = p.dt / 2;
@@ -584,9 +607,15 @@ int check_params(fhip_ctx *ctx, const fhip_params *p)
         return fail(ctx, FHIP_ERR_UNSUPPORTED,
                     "supercell edges must be one of 4 6 8 12 16 24 32 48 64 96 128 192 256");
     if (p->method < 0 || p->method > 2) return fail(ctx, FHIP_ERR_ARG, "method must be md, vs or lf");
-    if (p->bulk_or_film == FHIP_FILM || p->bulk_or_film == FHIP_EPIT)
-        return fail(ctx, FHIP_ERR_UNSUPPORTED, "film and epit geometries (mirror) are not supported");
     if (p->bulk_or_film < 0 || p->bulk_or_film > 3) return fail(ctx, FHIP_ERR_ARG, "bad geometry");
+    if (p->gap_id < 0 || p->gap_id > 2) return fail(ctx, FHIP_ERR_ARG, "gap_id must be 0, 1 or 2");
+    if (p->bulk_or_film == FHIP_FILM || p->bulk_or_film == FHIP_EPIT) {
+        if (p->Lz % 2 != p->gap_id % 2)       // param_module.F:475-477
+            return fail(ctx, FHIP_ERR_ARG, "film or epit needs mod(Lz,2) == mod(gap_id,2)");
+        if (p->Lz < 4) return fail(ctx, FHIP_ERR_ARG, "film or epit needs Lz >= 4");
+    } else if (p->gap_id != 0) {
+        return fail(ctx, FHIP_ERR_ARG, "gap_id must be 0 unless bulk_or_film is film or epit (param_module.F:470-474)");
+    }
     if (p->precision != FHIP_FP64 && p->precision != FHIP_FP32)
         return fail(ctx, FHIP_ERR_ARG, "precision must be FHIP_FP64 or FHIP_FP32");
     if (p->method == FHIP_MD && p->mass_acou[0] > 0.0)
@@ -595,6 +624,58 @@ int check_params(fhip_ctx *ctx, const fhip_params *p)
         if (!(p->mass_dipo[s] > 0.0)) return fail(ctx, FHIP_ERR_ARG, "mass_dipo must be positive");
     if (!(p->epsilon_inf > 0.0) || !(p->a0 > 0.0) || !(p->dt > 0.0))
         return fail(ctx, FHIP_ERR_ARG, "a0, dt and epsilon_inf must be positive");
+    return FHIP_OK;
+}
+
+// Per-cell code byte from the species, sum_p and 1/m (read_relaxor.F,
+// read_defects_u.F).  film/epit: the upper half of z is never counted and the
+// dead layers (gap_id 1: iz = 0; 2: iz = 0 and Lz/2) are frozen and not
+// counted, as read_defects_u.F:52-68 imposes after reading the .defects file.
+int set_cells_impl(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, const double *minv)
+{
+    const fhip_params &p = ctx->p;
+    const bool film = p.bulk_or_film == FHIP_FILM || p.bulk_or_film == FHIP_EPIT;
+    std::vector<unsigned char> h(ctx->N);
+    std::vector<int> layer(ctx->Lz, 0);
+    int nsum = 0;
+    for (int z = 0; z < ctx->Lz; ++z)
+        for (int y = 0; y < ctx->Ly; ++y)
+            for (int x = 0; x < ctx->Lx; ++x) {
+                const size_t hi = padded_index(ctx, 0, 1, x, y, z);
+                int s = relaxor ? relaxor[hi] - 1 : 0;
+                if (s < 0 || s > 2) return fail(ctx, FHIP_ERR_ARG, "relaxor species must be 1..3");
+                if (p.Z_star[s] != p.Z_star[0])
+                    return fail(ctx, FHIP_ERR_UNSUPPORTED, "species with different Z_star are not supported");
+                bool counted = sum_p ? sum_p[hi] != 0 : true;
+                bool frozen = false;
+                if (minv) {
+                    const double m = minv[hi];
+                    if (m == 0.0) frozen = true;
+                    else if (m != 1.0 / p.mass_dipo[s])
+                        return fail(ctx, FHIP_ERR_UNSUPPORTED, "mass_dipo_inv must be 1/mass_dipo(species) or 0");
+                }
+                if (film) {
+                    if (z >= (p.Lz + 1) / 2) counted = false;
+                    if ((p.gap_id > 0 && z == 0) || (p.gap_id == 2 && z == p.Lz / 2)) {
+                        counted = false;
+                        frozen = true;
+                    }
+                }
+                h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] =
+                    (unsigned char)(s | (frozen ? 4 : 0) | (counted ? 8 : 0));
+                nsum += counted ? 1 : 0;
+                layer[z] += counted ? 1 : 0;
+            }
+    if (nsum == 0) return fail(ctx, FHIP_ERR_ARG, "no counted cell (sum_p all false)");
+    HIPCHK(hipMemcpyAsync(ctx->code, h.data(), ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+
+    ctx->sk.n_sum_p = nsum;
+    ctx->nlayer_host = layer;                      // read_defects_u.F:73-75
+    if (ctx->nlayer) {
+        HIPCHK(hipMemcpyAsync(ctx->nlayer, layer.data(), sizeof(int) * ctx->Lz, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     return FHIP_OK;
 }
 
@@ -674,11 +755,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
         }
     }
     RC(dalloc(ctx, &ctx->code, N));
-    {
-        std::vector<unsigned char> h(N, (unsigned char)8);
-        HIPCHK(hipMemcpyAsync(ctx->code, h.data(), N, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-    }
+    RC(set_cells_impl(ctx, nullptr, nullptr, nullptr));
     ctx->f.code = ctx->code;
     ctx->precision = p->precision;
     if (ctx->precision == FHIP_FP32) {
@@ -706,6 +783,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
         ctx->hsc->scale_p = 1.0;
         ctx->hsc->scale_pw = 1.0;
         for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = p->external_E_field[q];
+        ctx->hsc->e_mir = ctx->c.mir_lo < ctx->Lz ? 1.0 : 0.0;   // initialize_E_field.F:68-71
         RC(push_scalars(ctx));
     }
     RC(upload_table(ctx, &ctx->dset.twx, ctx->Lx, true));
@@ -813,43 +891,9 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
 int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, const double *minv)
 {
     if (!ctx) return FHIP_ERR_ARG;
-    const fhip_params &p = ctx->p;
-    std::vector<unsigned char> h(ctx->N);
-    std::vector<int> layer(ctx->Lz, 0);
-    int nsum = 0;
-    for (int z = 0; z < ctx->Lz; ++z)
-        for (int y = 0; y < ctx->Ly; ++y)
-            for (int x = 0; x < ctx->Lx; ++x) {
-                const size_t hi = padded_index(ctx, 0, 1, x, y, z);
-                int s = relaxor ? relaxor[hi] - 1 : 0;
-                if (s < 0 || s > 2) return fail(ctx, FHIP_ERR_ARG, "relaxor species must be 1..3");
-                if (p.Z_star[s] != p.Z_star[0])
-                    return fail(ctx, FHIP_ERR_UNSUPPORTED, "species with different Z_star are not supported");
-                const bool counted = sum_p ? sum_p[hi] != 0 : true;
-                bool frozen = false;
-                if (minv) {
-                    const double m = minv[hi];
-                    if (m == 0.0) frozen = true;
-                    else if (m != 1.0 / p.mass_dipo[s])
-                        return fail(ctx, FHIP_ERR_UNSUPPORTED, "mass_dipo_inv must be 1/mass_dipo(species) or 0");
-                }
-                h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] =
-                    (unsigned char)(s | (frozen ? 4 : 0) | (counted ? 8 : 0));
-                nsum += counted ? 1 : 0;
-                layer[z] += counted ? 1 : 0;
-            }
-    if (nsum == 0) return fail(ctx, FHIP_ERR_ARG, "no counted cell (sum_p all false)");
-    HIPCHK(hipMemcpyAsync(ctx->code, h.data(), ctx->N, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-
-    ctx->sk.n_sum_p = nsum;
-    ctx->nlayer_host = layer;                      // read_defects_u.F:73-75
-    if (ctx->nlayer) {
-        HIPCHK(hipMemcpyAsync(ctx->nlayer, layer.data(), sizeof(int) * ctx->Lz, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-    }
-    return FHIP_OK;
+    return set_cells_impl(ctx, relaxor, sum_p, minv);
 }
+
 
 static int put_impl(fhip_ctx *ctx, int field, const double *host, bool soa)
 {
@@ -949,6 +993,7 @@ int fhip_set_E_field_add(fhip_ctx *ctx, const double E[3])
     if (!ctx || !E) return FHIP_ERR_ARG;
     RC(pull_scalars(ctx));
     for (int q = 0; q < 3; ++q) ctx->hsc->E_add[q] = E[q];
+    ctx->hsc->e_mir = 0.0;       // per-step rewrite of E_field, not mirrored (molecular-dynamics.F:66-76)
     return push_scalars(ctx);
 }
 
@@ -973,6 +1018,7 @@ int fhip_set_E_schedule(fhip_ctx *ctx, int type, const double amplitude[3], int 
     h.e_ntherm = n_thermalize;
     h.e_navg = n_average;
     h.i_step = (double)i_step;
+    h.e_mir = ctx->c.mir_lo < ctx->Lz ? 1.0 : 0.0;   // the field of initialize_E_field, mirrored
     if (type == FHIP_E_CONSTANT)
         for (int q = 0; q < 3; ++q) h.E_add[q] = amplitude[q];
     else

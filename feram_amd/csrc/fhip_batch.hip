@@ -349,6 +349,8 @@ int refresh(fhip_batch *b)
             if (c->avg_auto && !c->davg) return bfail(b, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
             if (c->delta_e != 0.0 || c->delta_e_acou != 0.0)
                 return bfail(b, FHIP_ERR_UNSUPPORTED, "heat-source layers are stepped by the per-context calls");
+            if (c->c.mir_lo < c->Lz)
+                return bfail(b, FHIP_ERR_UNSUPPORTED, "film and epit contexts are stepped by the per-context calls");
             if (i == 0) {
                 avg = c->avg_auto;
                 hist = c->avg_auto && c->avg_hist;

@@ -40,6 +40,7 @@ struct Consts {
     int efield;          // !E_field_always_zero
     int has_eloc;        // per-cell local field present
     int modulation;      // modulation_constant != 0 with a .modulation field
+    int mir_lo, mir_src; // film/epit: layers iz >= mir_lo mirror layer mir_src - iz (mir_lo = Lz: none)
     double Ninv, dt, dt_2, a0, Z2, Zstar;
     double kappa2[3], fa2g[3], tg[3], sk[3], tk2[3], fk[3], k4x8[3];
     double alpha[3], gamma[3], k1[3], k2[3], k3[3], k4[3];
@@ -67,6 +68,7 @@ struct Scalars {
     double e_type;                   // ESchedule
     double e_period, e_ntherm, e_navg;
     double i_step;                   // index of the next step
+    double e_mir;                    // 1: the field of mirrored layers is mirrored too (film/epit)
 };
 
 // Time dependence of the uniform field.
@@ -116,6 +118,13 @@ __host__ __device__ inline void e_field_next(Scalars* sc)
 {
     e_field_at((int)sc->e_type, sc->e_amp, (long long)sc->e_period, (long long)sc->e_ntherm,
                (long long)sc->e_navg, (long long)sc->i_step, sc->E_add);
+    // initialize_E_field mirrors E_field (initialize_E_field.F:68-71); the
+    // per-step updates of molecular-dynamics.F:66-76 and hysteresis_loop.F:47-58
+    // rewrite it unmirrored
+    const int type = (int)sc->e_type;
+    const bool rewritten = type == ES_HYSTERESIS ? sc->i_step >= sc->e_ntherm
+                                                 : (type != ES_CONSTANT && sc->i_step >= 1.0);
+    if (rewritten) sc->e_mir = 0.0;
 }
 enum EnergyIdx {
     E_ACOU_KIN = 0, E_DIPO_KIN, E_SR, E_LR, E_DE, E_UNH, E_HS, E_HC, E_IS, E_IC, E_IM, E_TOTAL, E_HNP
@@ -204,9 +213,15 @@ __device__ __forceinline__ double block_sum(double v, double* red)
 // ---------------------------------------------------------------------------
 enum XfwdMode { XF_FORCE = 0, XF_NP = 1, XF_LF = 2, XF_VS = 3, XF_GENERIC = 4 };
 
+// Film/epit (mirror.F:9-36): phase 1 drifts and transforms the rows below
+// Consts::mir_lo, phase 2 then fills the mirrored rows from the updated
+// source rows, (u_x, u_y, u_z) -> (-u_x, -u_y, u_z), and transforms them.
+enum XfwdPhase { XP_ALL = 0, XP_SOURCE = 1, XP_MIRROR = 2 };
+
 struct XfwdArgs {
     int mode;
     int nf;                       // number of forward fields
+    int phase;                    // XfwdPhase
     const double* gen[12];        // XF_GENERIC: real input fields
 };
 
@@ -225,6 +240,9 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
     const int tid = threadIdx.x;
     const int nrows = c.Ly * c.Lz;
     const int row0 = bx * RB;
+    const int mrow = c.mir_lo * c.Ly;             // first mirrored row
+    if (a.phase == XP_SOURCE && row0 >= mrow) return;       // uniform over the block
+    if (a.phase == XP_MIRROR && row0 + RB <= mrow) return;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
     double u[CPT][3], w[CPT][3];
@@ -234,7 +252,8 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
     for (int i = 0; i < CPT; ++i) {
         const int e = tid + i * NT;
         const int r = e / LX, x = e - r * LX;
-        ok[i] = (CELLS % NT == 0 || e < CELLS) && (row0 + r < nrows);
+        ok[i] = (CELLS % NT == 0 || e < CELLS) && (row0 + r < nrows)
+                && (a.phase == XP_ALL || ((row0 + r >= mrow) == (a.phase == XP_MIRROR)));
         cell[i] = (size_t)(row0 + r) * LX + x;
         if (a.mode == XF_GENERIC) continue;
         if (!ok[i]) {
@@ -246,7 +265,20 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
         const unsigned char code = f.code[cc];
         const int spc = code & 3;
         const double minv = (code & 4) ? 0.0 : c.minv[spc];
-        if (a.mode == XF_NP) {
+        if (a.phase == XP_MIRROR) {
+            const int rr = row0 + r, iz = rr / c.Ly, iy = rr - iz * c.Ly;
+            const size_t cs = ((size_t)(c.mir_src - iz) * c.Ly + iy) * LX + x;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double uq = q < 2 ? -f.u[q][cs] : f.u[q][cs];         // mirror.F:18-24
+                u[i][q] = uq;
+                f.u[q][cc] = uq;
+            }
+            if (a.mode == XF_VS) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) f.p[q][cc] = f.p[q][cc] * sc->scale_p;
+            }
+        } else if (a.mode == XF_NP) {
             const double cd = sc->c_drift;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
@@ -329,6 +361,7 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
         for (int e = tid; e < RB * MX; e += NT) {
             const int r = e / MX, kx = e - r * MX;
             if (row0 + r >= nrows) continue;
+            if (a.phase != XP_ALL && ((row0 + r >= mrow) != (a.phase == XP_MIRROR))) continue;
             const cpx<T> zk = buf[kx * ST + r];
             const cpx<T> zn = buf[(kx == 0 ? MX : LX - kx) * ST + r];
             const size_t o = (size_t)(row0 + r) * MX + kx;
@@ -1164,6 +1197,10 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
             const unsigned char code = f.code[cc];
             const int s = code & 3;
             const bool counted = (code & 8) != 0;
+            // film/epit: the dipole force and kick of a mirrored cell come from
+            // its source cell (mirror_force_mirror.F:80-84, k_mirror_kick)
+            const bool film = c.mir_lo < c.Lz;
+            const bool mirrored = film && row0 + r >= c.mir_lo * c.Ly;
             const double minv = (code & 4) ? 0.0 : c.minv[s];
             double u[3], F[3];
 #pragma unroll
@@ -1190,8 +1227,18 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
             // external and local field (mirror_force_mirror.F:39-65)
             if (c.efield) {
                 double E[3];
+                if (mirrored && sc->e_mir != 0.0) {                     // initialize_E_field.F:68-71
+                    const int rr = row0 + r, iz = rr / c.Ly, iy = rr - iz * c.Ly;
+                    const size_t cs = ((size_t)(c.mir_src - iz) * c.Ly + iy) * LX + x;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
+                    for (int q = 0; q < 3; ++q) {
+                        const double Es = c.has_eloc ? f.eloc[q][cs] + sc->E_add[q] : sc->E_add[q];
+                        E[q] = q < 2 ? -Es : Es;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
+                }
 #pragma unroll
                 for (int q = 0; q < 3; ++q) F[q] = F[q] - c.Zstar * E[q];
                 ede += c.Zstar * (u[0] * E[0] + u[1] * E[1] + u[2] * E[2]);
@@ -1218,7 +1265,8 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
                 F[2] = F[2] + 2 * ei[2] * u[2] + ei[3] * u[1] + ei[4] * u[0];
             }
             // kicks
-            if (a.mode == XI_NP) {
+            if (mirrored) {
+            } else if (a.mode == XI_NP) {
                 const double dt2s = c.dt_2 * sc->s;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
@@ -1243,7 +1291,7 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
                     f.p[q][cc] = pq;
                 }
             }
-            if (a.materialize || a.mode == XI_FORCE) {
+            if ((a.materialize || a.mode == XI_FORCE || film) && !mirrored) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) f.F[q][cc] = F[q];
             }
@@ -1281,6 +1329,53 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
                                              const cpx<T>* __restrict__ twx)
 {
     xinv_body<T, LX, NF>(c, f, sp, a, sc, twx, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Film/epit: dV/du of the mirrored layers is the mirror of their source
+// layers' (mirror_force_mirror.F:80-84, mirror.F:9-36), and their momenta are
+// kicked with it as xinv kicks the other cells.  Mirrored layers are never
+// counted (read_defects_u.F:52-54), so they add nothing to the kinetic sums.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void mirror_kick_body(const Consts& c, const Fields& f, const Scalars* sc, int mode,
+                                                    int materialize, int bx)
+{
+    const size_t plane = (size_t)c.Lx * c.Ly;
+    const size_t i = (size_t)bx * NT + threadIdx.x;
+    if (i >= (size_t)(c.Lz - c.mir_lo) * plane) return;
+    const int iz = c.mir_lo + (int)(i / plane);
+    const size_t xy = i - (size_t)(iz - c.mir_lo) * plane;
+    const size_t cc = (size_t)iz * plane + xy;
+    const size_t cs = (size_t)(c.mir_src - iz) * plane + xy;
+    double F[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        F[q] = q < 2 ? -f.F[q][cs] : f.F[q][cs];
+        f.F[q][cc] = F[q];
+    }
+    const bool counted = (f.code[cc] & 8) != 0;
+    if (mode == XI_NP) {
+        const double dt2s = c.dt_2 * sc->s;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double p1 = f.pk[q][cc] - dt2s * F[q];                    // nose-poincare.F:66-71
+            const double p2 = p1 - dt2s * F[q];
+            f.pk[q][cc] = p2;
+            if (materialize) f.p[q][cc] = p1;
+        }
+    } else if (mode == XI_LF) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f.p[q][cc] = f.p[q][cc] - c.dt * F[q];     // leapfrog.F:134-139
+    } else if (mode == XI_VS) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f.p[q][cc] = counted ? f.p[q][cc] - c.dt * F[q] : 0.0;   // velocity_scaling.F:62-76
+    }
+}
+
+static __global__ __launch_bounds__(NT) void k_mirror_kick(Consts c, Fields f, const Scalars* __restrict__ sc,
+                                                           int mode, int materialize)
+{
+    mirror_kick_body(c, f, sc, mode, materialize, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,6 +37,7 @@ module feram_hip_binding
      real(c_double) :: Z_star(3)
      real(c_double) :: epsilon_inf, kappa
      real(c_double) :: external_E_field(3)
+     integer(c_int32_t) :: gap_id, reserved
   end type fhip_params
 
   ! Energy_type twin (energy_module.F:7-23)

@@ -69,7 +69,7 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
             for j, g in enumerate(systems):
                 a = g.avg_get()
                 g.avg_auto(False)
-                u[it, j] = np.array(a["u"]) / (a["n_add"] * g.N)
+                u[it, j] = np.array(a["u"]) / (a["n_add"] * g.avg_u_norm())
                 strain[it, j] = np.array(a["strain"]) / a["n_add"]
                 if variance:
                     var[it, j] = g.get("dipoR").reshape(-1, 3).var(axis=0)
@@ -96,6 +96,22 @@ def hl_field(amplitude, n_thermalize: int, n_average: int, i_step: int) -> np.nd
     return A - 2 * A * float(i_step - n_thermalize) / float(n_average)
 
 
+class _Each:
+    """Steps each system through its own calls (the Batch interface)."""
+
+    def __init__(self, systems):
+        self.systems = systems
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+    def step(self, nsteps: int = 1) -> list:
+        return [g.step(nsteps) for g in self.systems]
+
+
 def hysteresis_loop(systems, n_thermalize: int, n_average: int, n_hl_freq: int, amplitude,
                     i_start: int = 1, progress=None) -> list:
     """The 'hl' driver (hysteresis_loop.F:41-96) for one or more supercells:
@@ -114,7 +130,8 @@ def hysteresis_loop(systems, n_thermalize: int, n_average: int, n_hl_freq: int, 
     rows = [[] for _ in systems]
     n_end = n_thermalize + n_average
     outputs = [i for i in range(n_thermalize, n_end + 1, n_hl_freq) if i >= i_start]
-    with Batch(systems) as bt:
+    # film/epit supercells step through their own contexts
+    with (_Each(systems) if any(g.film for g in systems) else Batch(systems)) as bt:
         i = i_start
         for out_step in outputs:
             bt.step(out_step - i + 1)

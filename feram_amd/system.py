@@ -29,6 +29,7 @@ def params_struct(p: Param, precision: int = 0) -> _lib.FhipParams:
     s.bulk_or_film = GEOMETRIES[p.bulk_or_film]
     s.precision = precision
     s.E_field_always_zero = 1 if p.E_field_always_zero else 0
+    s.gap_id = int(p.gap_id)
     for name in ("dt", "kelvin", "Q_Nose", "a0", "epi_strain", "B11", "B12", "B44", "B1xx",
                  "B1yy", "B4yz", "modulation_constant", "epsilon_inf", "kappa"):
         setattr(s, name, float(getattr(p, name)))
@@ -145,11 +146,33 @@ class System:
         self._check(rc, "set_cells")
         self._sum_p = None if sum_p is None else np.asarray(sum_p).astype(bool)
 
+    @property
+    def film(self) -> bool:
+        return self.p.bulk_or_film in ("film", "epit")
+
+    def counted(self) -> np.ndarray:
+        """The sum_p mask the device uses, (Lz, Ly, Lx): the one given to
+        set_cells (all cells by default) with, for film/epit, the upper half
+        of z and the dead layers removed (read_defects_u.F:52-69)."""
+        p = self.p
+        m = getattr(self, "_sum_p", None)
+        m = np.ones((p.Lz, p.Ly, p.Lx), bool) if m is None else m.copy()
+        if self.film:
+            m[(p.Lz + 1) // 2:] = False
+            if p.gap_id >= 1:
+                m[0] = False
+            if p.gap_id == 2:
+                m[p.Lz // 2] = False
+        return m
+
     def mean_u(self) -> np.ndarray:
         """<u> over the sum_p cells (hysteresis_loop.F:85-89)."""
-        u = self.get("dipoR")
-        m = getattr(self, "_sum_p", None)
-        return u.reshape(-1, 3).mean(axis=0) if m is None else u[m].mean(axis=0)
+        return self.get("dipoR")[self.counted()].mean(axis=0)
+
+    def avg_u_norm(self) -> int:
+        """Divisor of the u, u^2 sums per averaging step: n_sum_p for film/epit,
+        N otherwise (average_module.F:206-215)."""
+        return int(self.counted().sum()) if self.film else self.N
 
     # ---- scalars -----------------------------------------------------------
     @property

@@ -67,6 +67,8 @@ typedef struct fhip_params {
     double epsilon_inf;
     double kappa;                 /* <= 0: sqrt(pi)/a0/N^(1/6) (param_module.F:462-464) */
     double external_E_field[3];
+    int32_t gap_id;               /* film/epit dead layers 0, 1, 2 (param_module.F:268-270, 470-478) */
+    int32_t reserved;             /* 0 */
 } fhip_params;
 
 /* Energy_type (energy_module.F:7-23), same member order, supercell totals in eV. */
@@ -112,7 +114,11 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *modulation_R);
 /* Per-cell data set by read_relaxor / read_defects_u (read_relaxor.F:24-33,
  * read_defects_u.F:34-52): relaxor species 1..3 (NULL: all 1), sum_p as 0/1
  * (NULL: all counted), mass_dipo_inv (NULL: 1/mass_dipo(species)); a cell's
- * mass_dipo_inv must be 1/mass_dipo(species) or 0 (frozen defect). */
+ * mass_dipo_inv must be 1/mass_dipo(species) or 0 (frozen defect).  For film
+ * and epit the rules of read_defects_u.F:52-69 are applied on top (also at
+ * fhip_create): layers (Lz+1)/2 .. Lz-1 are not counted, and the dead layers
+ * (gap_id 1: iz = 0; 2: iz = 0 and Lz/2) are frozen and not counted; their u
+ * (gap_dipole_u) is the caller's, as fhip_put leaves it. */
 int fhip_set_cells(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p,
                    const double *mass_dipo_inv);
 
@@ -127,7 +133,10 @@ int fhip_get_nose(fhip_ctx *ctx, double *s, double *pi, double *H0);
 int fhip_set_nose(fhip_ctx *ctx, double s, double pi, double H0);
 /* c%strain(6) of the last force evaluation. */
 int fhip_get_strain(fhip_ctx *ctx, double strain[6]);
-/* Uniform part of E(R) = E_local(R) + E_add (molecular-dynamics.F:66-76). */
+/* Uniform part of E(R) = E_local(R) + E_add (molecular-dynamics.F:66-76).
+ * film/epit: fhip_create starts from initialize_E_field's mirrored field
+ * (initialize_E_field.F:68-71); this call, like the per-step updates of the
+ * reference, leaves the field unmirrored. */
 int fhip_set_E_field_add(fhip_ctx *ctx, const double E_add[3]);
 /* Time-dependent uniform field, evaluated on the device at the start of every
  * step of fhip_nose_poincare / fhip_leapfrog / fhip_velocity_scaling (and of
@@ -137,7 +146,9 @@ int fhip_set_E_field_add(fhip_ctx *ctx, const double E_add[3]);
  * (hysteresis_loop.F:47-49; amplitude before n_thermalize).  i_step is the
  * index of the next step (molecular_dynamics counts from 1, hysteresis_loop
  * from 0); every step advances it.  FHIP_E_CONSTANT holds E_add = amplitude
- * (as fhip_set_E_field_add). */
+ * (as fhip_set_E_field_add).  film/epit: the field is mirrored until the
+ * schedule first rewrites it (step >= 1 for the waves, step >= n_thermalize
+ * for the sweep), unmirrored afterwards, as the reference's loops leave it. */
 enum { FHIP_E_CONSTANT = 0, FHIP_E_TRIANGULAR_COS = 1, FHIP_E_TRIANGULAR_SIN = 2, FHIP_E_RAMPING_ON = 3,
        FHIP_E_RAMPING_OFF = 4, FHIP_E_HYSTERESIS = 5 };
 int fhip_set_E_schedule(fhip_ctx *ctx, int type, const double amplitude[3], int period, int n_thermalize,

@@ -287,6 +287,8 @@ static double *zalloc(size_t n)
     return a;
 }
 
+static void mirror(fo_ctx *o, double *f);
+
 fo_ctx *fo_create(const fo_params *pp)
 {
     fo_ctx *o = (fo_ctx *)calloc(1, sizeof(fo_ctx));
@@ -333,15 +335,28 @@ fo_ctx *fo_create(const fo_params *pp)
         o->Z_starR[c] = p->Z_star[0];
         o->sum_p[c] = 1;
     }
-    /* film/epit: upper half of z does not count (read_defects_u.F:55-57) */
+    /* film/epit: upper half of z does not count (read_defects_u.F:52-54) */
     if (p->bulk_or_film == 1 || p->bulk_or_film == 2) {
         for (int z = (p->Lz + 1) / 2; z < p->Lz; ++z)
             for (int c = 0; c < p->Lx * p->Ly; ++c) o->sum_p[c + (size_t)p->Lx * p->Ly * z] = 0;
     }
+    /* dead layers iz = 0 (gap_id 1, 2) and Lz/2 (gap_id 2): fixed u, 1/m = 0,
+     * not counted (read_defects_u.F:56-69) */
+    for (int g = 0; g < p->gap_id; ++g) {
+        const int z = g == 0 ? 0 : p->Lz / 2;
+        for (int c = 0; c < p->Lx * p->Ly; ++c) {
+            const size_t cc = c + (size_t)p->Lx * p->Ly * z;
+            for (int a = 0; a < 3; ++a) o->dipoR[3 * cc + a] = p->gap_dipole_u[a];
+            o->mass_dipo_inv[cc] = 0.0;
+            o->sum_p[cc] = 0;
+        }
+    }
     fo_recount_sum_p(o);
-    /* E_field from a static external field (initialize_E_field.F:58-66) */
+    /* E_field from a static external field (initialize_E_field.F:58-66),
+     * mirrored for film/epit (:68-71) */
     for (size_t c = 0; c < N; ++c)
         for (int a = 0; a < 3; ++a) o->E_field[3 * c + a] = p->external_E_field[a];
+    if (p->bulk_or_film == 1 || p->bulk_or_film == 2) mirror(o, o->E_field);
     o->nose[0] = 1.0; o->nose[1] = 0.0; o->nose[2] = 0.0;
     return o;
 }

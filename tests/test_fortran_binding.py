@@ -21,6 +21,7 @@ program t
   integer(c_int) :: rc
   p%Lx = 10; p%Ly = 8; p%Lz = 8; p%padding_y = 1
   p%method = 0; p%bulk_or_film = 0; p%precision = 0; p%E_field_always_zero = 1
+  p%gap_id = 0; p%reserved = 0
   p%dt = 0.002d0; p%a0 = 3.94d0; p%epsilon_inf = 5.24d0; p%mass_dipo = 0.004d0; p%mass_acou = -1d0
   rc = fhip_create(p, ctx)
   print '(i0,1x,i0,1x,i0,1x,a)', rc, c_sizeof(p), c_sizeof(e), fhip_error_message(ctx)

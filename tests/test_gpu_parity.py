@@ -233,7 +233,12 @@ def test_rejects_unsupported():
         _system(p)
     p = batio3_zvr(8)
     p.bulk_or_film = "film"
-    with pytest.raises(FhipError, match="film"):
+    p.gap_id = 1                         # needs odd Lz (param_module.F:475-477)
+    with pytest.raises(FhipError, match="gap_id"):
+        _system(p)
+    p = batio3_zvr(8)
+    p.gap_id = 2                         # dead layers only in film/epit
+    with pytest.raises(FhipError, match="gap_id"):
         _system(p)
 
 
