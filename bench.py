@@ -331,6 +331,61 @@ def bench_heating16(args, sync_extra):
     print(json.dumps(out))
 
 
+def bench_epit_hl(args, sync_extra):
+    """The reference's epitaxial-film hysteresis example
+    (11example-BaTiO3-new-hysteresis-loop/hysteresis16x16x128d0.feram):
+    BaTiO3 epit 16x16x128, gap_id 0, epi_strain -1 %, 150 K, velocity
+    scaling, E_z swept from -0.03 to +0.03 V/A over n_average steps after
+    n_thermalize, a .hl row every n_hl_freq steps.  Its 00TIMING.txt lists
+    the wall time of the whole run (20000 + 3000000 steps) on the reference's
+    machines; the fastest is feram-0.18.05 on an SR11000 node (16 cores),
+    7638 s."""
+    from feram_amd.params import batio3_zvr
+    from feram_amd.sweeps import hysteresis_loop
+    from feram_amd.system import System
+    p = batio3_zvr((16, 16, 128), method="vs", kelvin=150.0)
+    p.bulk_or_film, p.gap_id, p.epi_strain = "epit", 0, -0.01
+    p.dt, p.Q_Nose = 0.001, 0.05
+    p.external_E_field = [0.0, 0.0, -0.03]
+    p.E_field_always_zero = False
+    prec = 1 if args.precision == "fp32" else 0
+    g = System(p, precision=prec)
+    rng = np.random.default_rng(20111113)
+    g.put("dipoR", np.array([0.0, 0.0, -0.10]) + 0.02 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
+    g.md_prologue()
+    n_therm, n_avg, freq = args.hl_therm, args.hl_avg, args.hl_freq
+    t0 = time.perf_counter()
+
+    def progress(step, rows):
+        if (step - n_therm) % (50 * freq) == 0:
+            print(f"[bench] hl step {step}  t={time.perf_counter() - t0:.0f}s", file=sys.stderr, flush=True)
+
+    rows = np.array(hysteresis_loop([g], n_therm, n_avg, freq, p.external_E_field, i_start=1,
+                                    progress=progress)[0])
+    sync_extra()
+    dt = time.perf_counter() - t0
+    nsteps = n_therm + n_avg
+    # P_z in uC/cm^2 from <u_z> (hysteresis16x16x128d0.gp: 1.6e3 Z* / a0^3)
+    c = 1.6e3 * p.Z_star[0] / p.a0 ** 3
+    E, Pz = rows[:, 4], rows[:, 13] * c
+    flips = [float(E[i]) for i in range(1, len(Pz)) if np.sign(Pz[i]) != np.sign(Pz[i - 1])]
+    out = {
+        "metric": "MD steps/s, the reference's 16x16x128 epit hysteresis-loop example",
+        "value": nsteps / dt, "unit": "steps/s", "n_gpus": 1, "steps": nsteps, "warmup": 0,
+        "ms_per_step": 1e3 * dt / nsteps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if prec else "f64", "data": "synthetic",
+        "config": {"workload": f"BaTiO3 epit 16x16x128 gap_id 0, epi_strain -0.01, 150 K, vs, "
+                               f"E_z -0.03 -> +0.03 V/A, {n_therm}+{n_avg} steps, .hl every {freq}"},
+        "wall_s": dt,
+        "reference_wall_s_full_run": {"feram-0.18.05 SR11000 16 cores": 7638,
+                                      "feram-0.18.00 Xeon X5690 12 cores": 9582},
+        "full_run_s_at_this_rate": 3020000 / (nsteps / dt),
+        "loop": {"Pz_uC_cm2_start": float(Pz[0]), "Pz_uC_cm2_end": float(Pz[-1]),
+                 "E_at_sign_changes_V_A": flips, "rows": len(rows)},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,10 +396,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
-    ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16"), default="md128",
+    ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16", "epit_hl"),
+                    default="md128",
                     help="md128: the single-supercell NP-MD step (default); replicas: the "
                          "batched replica sweep; warmstart: the learned warm start on a heating scan; "
-                         "heating16: the reference's 16^3 heating-scan example against its .avg output")
+                         "heating16: the reference's 16^3 heating-scan example against its .avg output; "
+                         "epit_hl: the reference's 16x16x128 epit hysteresis-loop example")
+    ap.add_argument("--hl-therm", type=int, default=20000)
+    ap.add_argument("--hl-avg", type=int, default=3000000)
+    ap.add_argument("--hl-freq", type=int, default=10000)
     ap.add_argument("--seeds", type=int, default=4)
     ap.add_argument("--avg-steps", type=int, default=10000)
     ap.add_argument("--replicas", type=int, default=48)
@@ -388,6 +448,9 @@ def main():
 
     if args.workload == "replicas":
         bench_replicas(args, sync_extra)
+        return
+    if args.workload == "epit_hl":
+        bench_epit_hl(args, sync_extra)
         return
     if args.workload == "heating16":
         if args.scan_steps == 4000:

@@ -39,6 +39,8 @@ struct BClass {
     int Lx = 0, Ly = 0, Lz = 0, acoustic = 0, nf = 9;
     int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
     size_t zshm_g = 0;                  // dynamic LDS of the paired z pass
+    bool film = false;                  // a member mirrors its upper layers (film/epit)
+    int mblocks = 0;                    // blocks of the mirrored-layer kick
     std::vector<fhip_ctx *> members;
     void *dreps = nullptr;              // Rep<T>[members.size()] on the device
     std::vector<unsigned char> hreps;   // host image of the same
@@ -134,13 +136,13 @@ int upload_reps(fhip_batch *b, BClass &k)
 
 // ---- per-stage launches of one class -----------------------------------------
 template <typename T>
-int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s)
+int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s, int phase)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.xblocks, 1, (unsigned)k.members.size());
     switch (k.Lx) {
 #define FHIP_CASE(n) \
-    case n: kb_xfwd<T, n><<<g, NT, 0, s>>>(r); BLAUNCH("xfwd");
+    case n: kb_xfwd<T, n><<<g, NT, 0, s>>>(r, phase); BLAUNCH("xfwd");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -216,7 +218,10 @@ int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int ne
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const unsigned n = (unsigned)k.members.size();
-    RC(l_xfwd<T>(b, k, s));
+    // film/epit members: source rows first, then the mirrored rows from them
+    // (bulk members do everything in the first launch and skip the second)
+    RC(l_xfwd<T>(b, k, s, k.film ? XP_SOURCE : XP_ALL));
+    if (k.film) RC(l_xfwd<T>(b, k, s, XP_MIRROR));
     RC(l_ypass<T>(b, k, s, -1));
     const int want = (materialize && !k.acoustic) ? 1 : 0;
     if (k.acoustic) RC((l_zpair<T, ZG_ACOU>(b, k, s, want)));
@@ -228,6 +233,7 @@ int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int ne
     }
     RC(l_ypass<T>(b, k, s, +1));
     RC(l_xinv<T>(b, k, s, materialize ? 1 : 0));
+    if (k.film) kb_mirror_kick<T><<<dim3(k.mblocks, 1, n), NT, 0, s>>>(r, xi_mode(b->method), materialize ? 1 : 0);
     kb_finish<T><<<dim3(1, 1, n), NT, 0, s>>>(r, fin_mode(b->method), next_step);
     if (k.avg) {
         if (k.hist) kb_avg_cells<T, 1><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
@@ -349,8 +355,6 @@ int refresh(fhip_batch *b)
             if (c->avg_auto && !c->davg) return bfail(b, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
             if (c->delta_e != 0.0 || c->delta_e_acou != 0.0)
                 return bfail(b, FHIP_ERR_UNSUPPORTED, "heat-source layers are stepped by the per-context calls");
-            if (c->c.mir_lo < c->Lz)
-                return bfail(b, FHIP_ERR_UNSUPPORTED, "film and epit contexts are stepped by the per-context calls");
             if (i == 0) {
                 avg = c->avg_auto;
                 hist = c->avg_auto && c->avg_hist;
@@ -443,6 +447,11 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
             k->yblocks = c->yblocks[t];
             k->zblocks = c->zblocks[t];
             k->kblocks = c->kblocks;
+        }
+        if (c->c.mir_lo < c->Lz) {
+            k->film = true;
+            const size_t cells = (size_t)(c->Lz - c->c.mir_lo) * c->Lx * c->Ly;
+            k->mblocks = std::max(k->mblocks, (int)((cells + NT - 1) / NT));
         }
         k->members.push_back(c);
     }

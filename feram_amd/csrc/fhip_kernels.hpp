@@ -2146,10 +2146,12 @@ struct Rep {
 };
 
 template <typename T, int LX>
-__global__ __launch_bounds__(NT) void kb_xfwd(const Rep<T>* __restrict__ reps)
+__global__ __launch_bounds__(NT) void kb_xfwd(const Rep<T>* __restrict__ reps, int phase)
 {
     const Rep<T>& r = reps[blockIdx.z];
-    xfwd_body<T, LX>(r.c, r.f, r.sp, r.xa, r.sc, r.twx, blockIdx.x);
+    XfwdArgs a = r.xa;
+    a.phase = phase;
+    xfwd_body<T, LX>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
 }
 
 template <typename T, int LY, int SIGN>
@@ -2187,6 +2189,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF == 12 ? 2
     XinvArgs a = r.ia;
     a.materialize = materialize;
     xinv_body<T, LX, NF>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_mirror_kick(const Rep<T>* __restrict__ reps, int mode, int materialize)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    mirror_kick_body(r.c, r.f, r.sc, mode, materialize, blockIdx.x);
 }
 
 template <typename T>

@@ -141,10 +141,37 @@ def test_film_field_unmirrored_after_per_step_update(oracle_mod):
     g.close()
 
 
-def test_film_rejected_in_batches():
-    from feram_amd.system import Batch, FhipError, System
-    p = _params((8, 8, 8), "epit", 0, "lf")
-    with System(p) as a, System(p) as b:
-        with pytest.raises(FhipError):
-            with Batch([a, b]) as bt:
-                bt.step(1)
+@pytest.mark.parametrize("method", ["md", "vs"])
+def test_film_batch_matches_individual_steps(method):
+    """film, epit (gap_id 0 and 2) and bulk replicas of one size class in one
+    batch evolve as their own step calls make them evolve."""
+    from feram_amd.system import Batch, System
+    print(f"[gpu] film batch {method}", flush=True)
+    kinds = [("epit", 0), ("film", 2), ("bulk", 0), ("epit", 2)]
+
+    def build():
+        out = []
+        for i, (geom, gap) in enumerate(kinds):
+            p = _params((8, 8, 16), geom, gap, method)
+            p.Q_Nose = 0.1 * p.N / 16 ** 3
+            g = System(p)
+            rng = np.random.default_rng(40 + i)
+            u = 0.05 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+            for z in _dead(p):
+                u[z] = GAP_U
+            g.put("dipoR", u)
+            g.md_prologue()
+            out.append(g)
+        return out
+
+    single, batched = build(), build()
+    es = [g.step(6) for g in single]
+    with Batch(batched) as bt:
+        assert bt.classes == 1
+        eb = bt.step(6)
+    for i, (a, b) in enumerate(zip(single, batched)):
+        for f in ("dipoR", "dipoP", "dVddi"):
+            assert _nw(b.get(f), a.get(f)) <= 1e-12, (i, f)
+        scale = sum(abs(v) for v in es[i].values())
+        for k, v in es[i].items():
+            assert abs(eb[i][k] - v) <= 1e-12 * scale, (i, k)
