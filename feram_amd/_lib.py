@@ -26,7 +26,7 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_get_soa", "fhip_get_nose", "fhip_set_nose", "fhip_get_strain",
            "fhip_set_E_field_add", "fhip_set_kelvin", "fhip_set_heat_source", "fhip_get_layer_energies",
            "fhip_set_E_schedule", "fhip_get_step", "fhip_force", "fhip_md_prologue", "fhip_nose_poincare",
-           "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_kernel_stats",
+           "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_use_graph", "fhip_kernel_stats",
            "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
            "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
            "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size", "fhip_batch_classes",
@@ -127,6 +127,7 @@ def load():
     L.fhip_leapfrog.argtypes = [vp, ip, ip, C.POINTER(FhipEnergy)]
     L.fhip_velocity_scaling.argtypes = [vp, ip, C.POINTER(FhipEnergy)]
     L.fhip_profile.argtypes = [vp, ip]
+    L.fhip_use_graph.argtypes = [vp, ip]
     L.fhip_kernel_stats.argtypes = [vp, ip, C.POINTER(C.c_char_p), dp, C.POINTER(C.c_int64)]
     L.fhip_synchronize.argtypes = [vp]
     L.fhip_device_bytes.argtypes = [vp]
@@ -165,7 +166,7 @@ def load():
     for n in ("fhip_set_long_range", "fhip_get_long_range", "fhip_set_modulation", "fhip_set_cells", "fhip_put",
               "fhip_get", "fhip_put_soa", "fhip_get_soa", "fhip_get_nose", "fhip_set_nose",
               "fhip_get_strain", "fhip_set_E_field_add", "fhip_force", "fhip_md_prologue",
-              "fhip_nose_poincare", "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile",
+              "fhip_nose_poincare", "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_use_graph",
               "fhip_kernel_stats", "fhip_synchronize", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
               "fhip_avg_get"):
         getattr(L, n).restype = ip

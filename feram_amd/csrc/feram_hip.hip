@@ -343,6 +343,74 @@ int force_pipeline(fhip_ctx *ctx, int kind, bool materialize, int next_step = 0)
     return force_pipeline_t<double>(ctx, kind, materialize, next_step);
 }
 
+void drop_step_graph(fhip_ctx *ctx)
+{
+    if (ctx->sgexec) hipGraphExecDestroy(ctx->sgexec);
+    if (ctx->sgraph) hipGraphDestroy(ctx->sgraph);
+    ctx->sgexec = nullptr;
+    ctx->sgraph = nullptr;
+    ctx->skey.clear();
+}
+
+// Everything a middle step passes to its kernels by value or decides on the
+// host: a graph captured with one key replays exactly that step.
+std::vector<unsigned char> step_key(const fhip_ctx *ctx, int kind, int next_step)
+{
+    std::vector<unsigned char> k;
+    auto put = [&](const void *v, size_t n) {
+        const unsigned char *b = (const unsigned char *)v;
+        k.insert(k.end(), b, b + n);
+    };
+    const int flags[4] = {kind, next_step, ctx->avg_auto ? 1 + (ctx->avg_hist ? 2 : 0) + (ctx->avg_ravg ? 4 : 0) : 0,
+                          ctx->precision};
+    put(flags, sizeof(flags));
+    put(&ctx->c, sizeof(Consts));
+    put(&ctx->sk, sizeof(ScalarConsts));
+    put(&ctx->f, sizeof(Fields));
+    const double de[2] = {ctx->delta_e, ctx->delta_e_acou};
+    put(de, sizeof(de));
+    const void *ptrs[7] = {ctx->davg, ctx->dist1, ctx->dist3, ctx->el, ctx->el_sum, ctx->nlayer, ctx->modk};
+    put(ptrs, sizeof(ptrs));
+    return k;
+}
+
+int one_step(fhip_ctx *ctx, int kind, bool materialize, int next_step)
+{
+    RC(force_pipeline(ctx, kind, materialize, next_step));
+    if (ctx->avg_auto) RC(avg_add(ctx));
+    return FHIP_OK;
+}
+
+// count steps that are not the last of a call (nothing materialised): with two
+// or more, one such step is captured as a hipGraph (kernel nodes only) and
+// replayed; the capture is kept while the launch state stays the same.
+int middle_steps(fhip_ctx *ctx, int kind, int next_step, int count)
+{
+    if (count <= 0) return FHIP_OK;
+    if (!ctx->use_graph || ctx->prof || count < 2) {
+        for (int i = 0; i < count; ++i) RC(one_step(ctx, kind, false, next_step));
+        return FHIP_OK;
+    }
+    std::vector<unsigned char> key = step_key(ctx, kind, next_step);
+    if (!ctx->sgexec || key != ctx->skey) {
+        drop_step_graph(ctx);
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = one_step(ctx, kind, false, next_step);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc != FHIP_OK) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ctx, FHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        ctx->sgraph = g;
+        HIPCHK(hipGraphInstantiate(&ctx->sgexec, g, nullptr, nullptr, 0));
+        ctx->skey = key;
+    }
+    for (int i = 0; i < count; ++i) HIPCHK(hipGraphLaunch(ctx->sgexec, ctx->stream));
+    return FHIP_OK;
+}
+
 int pull_scalars(fhip_ctx *ctx)
 {
     HIPCHK(hipMemcpyAsync(ctx->hsc, ctx->dsc, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->stream));
@@ -701,6 +769,7 @@ void fhip_destroy(fhip_ctx *ctx)
         hipEventDestroy(r.b);
     }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
+    drop_step_graph(ctx);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1144,11 +1213,8 @@ int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e)
         hipStream_t s = ctx->stream;
         RC(launch(ctx, K_NPBEGIN, [&] { k_np_begin<<<1, NT, 0, s>>>(k, sc, part, mb, kb); }));
     }
-    for (int i = 0; i < nsteps; ++i) {
-        const bool last = (i == nsteps - 1);
-        RC(force_pipeline(ctx, SK_NP, last, last ? 0 : 1));
-        if (ctx->avg_auto) RC(avg_add(ctx));
-    }
+    RC(middle_steps(ctx, SK_NP, 1, nsteps - 1));
+    RC(one_step(ctx, SK_NP, true, 0));
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
     return FHIP_OK;
@@ -1162,10 +1228,8 @@ int fhip_leapfrog(fhip_ctx *ctx, int i_step, int nsteps, fhip_energy *e)
     if (i_step == 0)
         return fail(ctx, FHIP_ERR_UNSUPPORTED,
                     "the i_step == 0 half kick is not reachable from molecular_dynamics (molecular-dynamics.F:64)");
-    for (int i = 0; i < nsteps; ++i) {
-        RC(force_pipeline(ctx, SK_LF, i == nsteps - 1));
-        if (ctx->avg_auto) RC(avg_add(ctx));
-    }
+    RC(middle_steps(ctx, SK_LF, 0, nsteps - 1));
+    RC(one_step(ctx, SK_LF, true, 0));
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
     return FHIP_OK;
@@ -1176,10 +1240,8 @@ int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e)
     if (!ctx) return FHIP_ERR_ARG;
     if (nsteps < 1) return fail(ctx, FHIP_ERR_ARG, "nsteps must be >= 1");
     if (ctx->p.method != FHIP_VS) return fail(ctx, FHIP_ERR_STATE, "context was created for another method");
-    for (int i = 0; i < nsteps; ++i) {
-        RC(force_pipeline(ctx, SK_VS, i == nsteps - 1));
-        if (ctx->avg_auto) RC(avg_add(ctx));
-    }
+    RC(middle_steps(ctx, SK_VS, 0, nsteps - 1));
+    RC(one_step(ctx, SK_VS, true, 0));
     {
         Consts c = ctx->c;
         Fields f = ctx->f;
@@ -1259,6 +1321,14 @@ int fhip_avg_get(fhip_ctx *ctx, fhip_average *out, int64_t *distribution, int32_
         for (int k = 0; k < DIST_W; ++k)
             for (int q = 0; q < 3; ++q) distribution[q + 3 * k] = (int64_t)d1[q * DIST_W + k];
     return collect_events(ctx);
+}
+
+int fhip_use_graph(fhip_ctx *ctx, int enable)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    ctx->use_graph = enable != 0;
+    if (!ctx->use_graph) drop_step_graph(ctx);
+    return FHIP_OK;
 }
 
 int fhip_profile(fhip_ctx *ctx, int enable)

@@ -85,6 +85,12 @@ struct fhip_ctx {
     std::vector<int> nlayer_host;
     // measurement
     bool prof = false;
+    // graph of one middle step of a multi-step call, and the by-value launch
+    // state it was captured with
+    bool use_graph = true;
+    hipGraph_t sgraph = nullptr;
+    hipGraphExec_t sgexec = nullptr;
+    std::vector<unsigned char> skey;
     std::vector<hipEvent_t> ev_pool;
     struct Rec { int id; hipEvent_t a, b; };
     std::vector<Rec> recs;

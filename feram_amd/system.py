@@ -296,6 +296,10 @@ class System:
     def profile(self, enable: bool = True) -> None:
         self._check(self._L.fhip_profile(self.ctx, 1 if enable else 0), "profile")
 
+    def use_graph(self, enable: bool = True) -> None:
+        """Replay the middle steps of a multi-step call as a captured hipGraph."""
+        self._check(self._L.fhip_use_graph(self.ctx, 1 if enable else 0), "use_graph")
+
     def kernel_stats(self) -> dict:
         n = 32
         names = (C.c_char_p * n)()

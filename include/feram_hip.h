@@ -272,6 +272,11 @@ int fhip_mlp_train(fhip_mlp *mlp, int n, const float *x, const float *y, const f
  * context's stream and accumulate per-kernel time.  fhip_kernel_stats fills
  * up to n entries (name, total ms, launches) and returns the entry count. */
 int fhip_profile(fhip_ctx *ctx, int enable);
+/* No counterpart in the reference (its steps are loops of subroutine calls):
+ * multi-step calls replay one captured step as a hipGraph for all but the last
+ * step (default on; off while profiling).  enable = 0 launches every kernel
+ * directly; the arithmetic is the same either way. */
+int fhip_use_graph(fhip_ctx *ctx, int enable);
 int fhip_kernel_stats(fhip_ctx *ctx, int n, const char **names, double *total_ms, int64_t *launches);
 int fhip_synchronize(fhip_ctx *ctx);
 /* Bytes of device memory the context holds. */
