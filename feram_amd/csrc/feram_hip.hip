@@ -117,7 +117,7 @@ int run_xfwd(fhip_ctx *ctx, const XfwdArgs &a)
     Spectra<T> sp = tset<T>(ctx).sp;
     const Scalars *sc = ctx->dsc;
     const cpx<T> *tw = tset<T>(ctx).twx;
-    const int nb = ctx->xblocks;
+    const int nb = ctx->xfblocks;
     hipStream_t s = ctx->stream;
     switch (ctx->Lx) {
 #define FHIP_CASE(n)                                                                         \
@@ -798,6 +798,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     fill_consts(ctx);
     ctx->nslots = ctx->c.acoustic ? 12 : 9;
     ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
+    ctx->xfblocks = (ctx->Ly * ctx->Lz + xfwd_tile_rows(ctx->Lx) - 1) / xfwd_tile_rows(ctx->Lx);
     for (int t = 0; t < 2; ++t) {
         const int ty = t ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
         const int tz = t ? ztile_rows<float>(ctx->Lz) : ztile_rows<double>(ctx->Lz);

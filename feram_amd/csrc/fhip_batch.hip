@@ -37,7 +37,7 @@ bool batch_size_ok(int n)
 struct BClass {
     int prec = FHIP_FP64;
     int Lx = 0, Ly = 0, Lz = 0, acoustic = 0, nf = 9;
-    int xblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
+    int xblocks = 0, xfblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
     size_t zshm_g = 0;                  // dynamic LDS of the paired z pass
     bool film = false;                  // a member mirrors its upper layers (film/epit)
     int mblocks = 0;                    // blocks of the mirrored-layer kick
@@ -139,7 +139,7 @@ template <typename T>
 int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s, int phase)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
-    const dim3 g(k.xblocks, 1, (unsigned)k.members.size());
+    const dim3 g(k.xfblocks, 1, (unsigned)k.members.size());
     switch (k.Lx) {
 #define FHIP_CASE(n) \
     case n: kb_xfwd<T, n><<<g, NT, 0, s>>>(r, phase); BLAUNCH("xfwd");
@@ -444,6 +444,7 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
             k->nf = c->c.acoustic ? 12 : 9;
             const int t = c->precision == FHIP_FP32 ? 1 : 0;
             k->xblocks = c->xblocks;
+            k->xfblocks = c->xfblocks;
             k->yblocks = c->yblocks[t];
             k->zblocks = c->zblocks[t];
             k->kblocks = c->kblocks;

@@ -25,6 +25,9 @@ inline int tile_rows(int n, int cap = 1024)   // must agree with tile_of<n, cap>
     return p;
 }
 
+// Rows per xfwd workgroup (xfwd_rows<n> on the device).
+inline int xfwd_tile_rows(int n) { const int t = tile_rows(n); return t > 1 ? t / 2 : 1; }
+
 // Lines per y tile of spectral type T (line_tile_cap<T>() on the device) and
 // per z tile (ZTile<T, n>::TB).
 template <typename T>
@@ -50,7 +53,7 @@ struct fhip_ctx {
     fhip_params p{};
     int Lx = 0, Ly = 0, Lz = 0, Kx = 0, Mx = 0, N = 0, Kh = 0, nslots = 0;
     size_t Mh = 0;
-    int xblocks = 0, kblocks = 0, maxblocks = 0;
+    int xblocks = 0, xfblocks = 0, kblocks = 0, maxblocks = 0;   // xinv, xfwd, cell-wise grids
     int yblocks[2] = {0, 0}, zblocks[2] = {0, 0};   // per spectral type: [0] FP64, [1] FP32
     hipStream_t stream = nullptr;
     std::string err;
