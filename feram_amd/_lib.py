@@ -29,7 +29,8 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_use_graph", "fhip_kernel_stats",
            "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
            "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
-           "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size", "fhip_batch_classes",
+           "fhip_batch_step", "fhip_batch_graph", "fhip_batch_tune", "fhip_batch_plan", "fhip_batch_size",
+           "fhip_batch_classes",
            "fhip_mlp_nparams", "fhip_mlp_create", "fhip_mlp_destroy", "fhip_mlp_last_error",
            "fhip_mlp_set_params", "fhip_mlp_get_params", "fhip_mlp_forward", "fhip_mlp_gradient",
            "fhip_mlp_train")
@@ -143,10 +144,12 @@ def load():
     L.fhip_batch_last_error.restype = C.c_char_p
     L.fhip_batch_step.argtypes = [vp, ip, C.POINTER(FhipEnergy)]
     L.fhip_batch_graph.argtypes = [vp, ip]
+    L.fhip_batch_tune.argtypes = [vp, ip]
+    L.fhip_batch_plan.argtypes = [vp, ip, C.POINTER(C.c_int32), dp]
     L.fhip_batch_size.argtypes = [vp]
     L.fhip_batch_classes.argtypes = [vp]
-    for n in ("fhip_batch_create", "fhip_batch_step", "fhip_batch_graph", "fhip_batch_size",
-              "fhip_batch_classes"):
+    for n in ("fhip_batch_create", "fhip_batch_step", "fhip_batch_graph", "fhip_batch_tune", "fhip_batch_plan",
+              "fhip_batch_size", "fhip_batch_classes"):
         getattr(L, n).restype = ip
     fp = C.POINTER(C.c_float)
     L.fhip_mlp_nparams.argtypes = []

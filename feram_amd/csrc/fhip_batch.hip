@@ -11,7 +11,10 @@
 // sequences; they are captured once into a HIP graph and replayed.
 #include "fhip_host.hpp"
 
+#include <array>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 using namespace fhip;
 using namespace fhip_host;
@@ -41,6 +44,11 @@ struct BClass {
     size_t zshm_g = 0;                  // dynamic LDS of the paired z pass
     bool film = false;                  // a member mirrors its upper layers (film/epit)
     int mblocks = 0;                    // blocks of the mirrored-layer kick
+    // tile-shape plan: rows per xfwd workgroup and lines per y tile, each the
+    // full or the half tile, picked by timing (tune_class)
+    int xrows = 0, ytile = 0;
+    bool tuned = false;
+    double tx[2] = {0, 0}, ty[2] = {0, 0};   // measured us per launch: full, half
     std::vector<fhip_ctx *> members;
     void *dreps = nullptr;              // Rep<T>[members.size()] on the device
     std::vector<unsigned char> hreps;   // host image of the same
@@ -52,6 +60,7 @@ struct BClass {
 }  // namespace
 
 struct fhip_batch {
+    bool tune = true;
     std::vector<fhip_ctx *> ctxs;
     std::vector<BClass> classes;
     int method = FHIP_MD;
@@ -134,15 +143,43 @@ int upload_reps(fhip_batch *b, BClass &k)
     return FHIP_OK;
 }
 
+// ---- tile-shape plan -----------------------------------------------------------
+template <typename T, int n>
+constexpr int ytile_full() { return tile_of<n, line_tile_cap<T>()>::value; }
+template <typename T, int n>
+constexpr int ytile_half() { return ytile_full<T, n>() > 1 ? ytile_full<T, n>() / 2 : 1; }
+
+void set_plan(BClass &k, int xrows, int ytile)
+{
+    k.xrows = xrows;
+    k.ytile = ytile;
+    k.xfblocks = (k.Ly * k.Lz + xrows - 1) / xrows;
+    k.yblocks = ((k.Lx / 2) * k.Lz + ytile - 1) / ytile;
+}
+
+// Plans found by timing, shared by every batch of the process: (Lx, Ly, Lz,
+// precision, acoustic, replicas) -> (xrows, ytile).
+std::mutex g_plan_mu;
+std::map<std::array<int, 6>, std::pair<int, int>> g_plans;
+
+std::array<int, 6> plan_key(const BClass &k)
+{
+    return {k.Lx, k.Ly, k.Lz, k.prec, k.acoustic, (int)k.members.size()};
+}
+
 // ---- per-stage launches of one class -----------------------------------------
 template <typename T>
-int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s, int phase)
+int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s, int phase, int mode = -1)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.xfblocks, 1, (unsigned)k.members.size());
+    const bool half = k.xrows != tile_rows(k.Lx);
     switch (k.Lx) {
-#define FHIP_CASE(n) \
-    case n: kb_xfwd<T, n><<<g, NT, 0, s>>>(r, phase); BLAUNCH("xfwd");
+#define FHIP_CASE(n)                                                                               \
+    case n:                                                                                        \
+        if (half) kb_xfwd<T, n, xfwd_rows<n>::value><<<g, NT, 0, s>>>(r, phase, mode);             \
+        else kb_xfwd<T, n, tile_of<n>::value><<<g, NT, 0, s>>>(r, phase, mode);                    \
+        BLAUNCH("xfwd");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -154,11 +191,17 @@ int l_ypass(fhip_batch *b, BClass &k, hipStream_t s, int sign)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.yblocks, k.nf, (unsigned)k.members.size());
+    const bool half = k.ytile != line_tile_rows<T>(k.Ly);
     switch (k.Ly) {
-#define FHIP_CASE(n)                                                  \
-    case n:                                                           \
-        if (sign < 0) kb_ypass<T, n, -1><<<g, NT, 0, s>>>(r);         \
-        else kb_ypass<T, n, +1><<<g, NT, 0, s>>>(r);                  \
+#define FHIP_CASE(n)                                                                             \
+    case n:                                                                                      \
+        if (half) {                                                                              \
+            if (sign < 0) kb_ypass<T, n, -1, ytile_half<T, n>()><<<g, NT, 0, s>>>(r);            \
+            else kb_ypass<T, n, +1, ytile_half<T, n>()><<<g, NT, 0, s>>>(r);                     \
+        } else {                                                                                 \
+            if (sign < 0) kb_ypass<T, n, -1, ytile_full<T, n>()><<<g, NT, 0, s>>>(r);            \
+            else kb_ypass<T, n, +1, ytile_full<T, n>()><<<g, NT, 0, s>>>(r);                     \
+        }                                                                                        \
         BLAUNCH("ypass");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
@@ -311,6 +354,8 @@ std::vector<int> graph_key(const fhip_batch *b)
     for (const auto &k : b->classes) {
         key.push_back(k.avg ? 1 : 0);
         key.push_back(k.hist ? 1 : 0);
+        key.push_back(k.xrows);
+        key.push_back(k.ytile);
     }
     return key;
 }
@@ -343,6 +388,71 @@ int ensure_graph(fhip_batch *b)
     b->graph = g;
     BHIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->gkey = key;
+    return FHIP_OK;
+}
+
+// The autotuner: for each of the two tile knobs, time the full and the half
+// tile on the class's own replicas (8 launches after 2 of warm-up) and keep
+// the faster.  The timing launches have no lasting effect: xfwd runs without
+// drift and writes only the spectra, the y pass transforms the spectra in
+// place, and both are rewritten by the next step before anything reads them.
+template <typename T>
+int tune_class(fhip_batch *b, BClass &k)
+{
+    const int xf = tile_rows(k.Lx), xh = xfwd_tile_rows(k.Lx);
+    const int yf = line_tile_rows<T>(k.Ly), yh = yf > 1 ? yf / 2 : 1;
+    {
+        std::lock_guard<std::mutex> lock(g_plan_mu);
+        auto it = g_plans.find(plan_key(k));
+        if (it != g_plans.end()) {
+            set_plan(k, it->second.first, it->second.second);
+            k.tuned = true;
+            return FHIP_OK;
+        }
+    }
+    hipStream_t s = b->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BHIP(hipEventCreate(&e0));
+    BHIP(hipEventCreate(&e1));
+    int rc = FHIP_OK;
+    auto timed = [&](auto &&launch, double &us) {
+        for (int i = 0; i < 10 && rc == FHIP_OK; ++i) {
+            if (i == 2 && hipEventRecord(e0, s) != hipSuccess) rc = bfail(b, FHIP_ERR_HIP, "tuner event");
+            rc = launch();
+        }
+        if (rc != FHIP_OK) return;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = bfail(b, FHIP_ERR_HIP, "tuner event");
+            return;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        us = 1e3 * ms / 8;
+    };
+    int bx = xf, by = yf;
+    set_plan(k, xf, yf);
+    timed([&] { return l_xfwd<T>(b, k, s, XP_ALL, XF_FORCE); }, k.tx[0]);
+    k.tx[1] = k.tx[0];
+    if (rc == FHIP_OK && xh != xf) {
+        set_plan(k, xh, yf);
+        timed([&] { return l_xfwd<T>(b, k, s, XP_ALL, XF_FORCE); }, k.tx[1]);
+        if (k.tx[1] < k.tx[0]) bx = xh;
+    }
+    set_plan(k, bx, yf);
+    if (rc == FHIP_OK) timed([&] { return l_ypass<T>(b, k, s, -1); }, k.ty[0]);
+    k.ty[1] = k.ty[0];
+    if (rc == FHIP_OK && yh != yf) {
+        set_plan(k, bx, yh);
+        timed([&] { return l_ypass<T>(b, k, s, -1); }, k.ty[1]);
+        if (k.ty[1] < k.ty[0]) by = yh;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    RC(rc);
+    set_plan(k, bx, by);
+    k.tuned = true;
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    g_plans[plan_key(k)] = {bx, by};
     return FHIP_OK;
 }
 
@@ -446,6 +556,8 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
             k->xblocks = c->xblocks;
             k->xfblocks = c->xfblocks;
             k->yblocks = c->yblocks[t];
+            k->xrows = xfwd_tile_rows(c->Lx);            // the single-context plan until tuned
+            k->ytile = t ? line_tile_rows<float>(c->Ly) : line_tile_rows<double>(c->Ly);
             k->zblocks = c->zblocks[t];
             k->kblocks = c->kblocks;
         }
@@ -481,6 +593,12 @@ int fhip_batch_step(fhip_batch *b, int nsteps, fhip_energy *e)
     // make sure of it before the batch streams touch their memory
     for (fhip_ctx *c : b->ctxs) BHIP(hipStreamSynchronize(c->stream));
     RC(refresh(b));
+    if (b->tune)
+        for (auto &k : b->classes)
+            if (!k.tuned) {
+                RC(k.prec == FHIP_FP32 ? tune_class<float>(b, k) : tune_class<double>(b, k));
+                BHIP(hipStreamSynchronize(b->stream));
+            }
     if (b->method == FHIP_MD) RC(batch_phase(b, PH_PRELUDE));
     const int middle = nsteps - 1;
     if (middle > 0) {
@@ -502,6 +620,25 @@ int fhip_batch_step(fhip_batch *b, int nsteps, fhip_energy *e)
         energies_out(c, e ? e + i : nullptr);
     }
     return FHIP_OK;
+}
+
+int fhip_batch_tune(fhip_batch *b, int enable)
+{
+    if (!b) return FHIP_ERR_ARG;
+    b->tune = enable != 0;
+    return FHIP_OK;
+}
+
+int fhip_batch_plan(const fhip_batch *b, int cls, int32_t shape[5], double us[4])
+{
+    if (!b || !shape) return FHIP_ERR_ARG;
+    if (cls < 0 || cls >= (int)b->classes.size()) return FHIP_ERR_ARG;
+    const BClass &k = b->classes[cls];
+    shape[0] = k.Lx; shape[1] = k.Ly; shape[2] = k.Lz; shape[3] = k.xrows; shape[4] = k.ytile;
+    if (us) {
+        us[0] = k.tx[0]; us[1] = k.tx[1]; us[2] = k.ty[0]; us[3] = k.ty[1];
+    }
+    return k.tuned ? 1 : 0;
 }
 
 int fhip_batch_size(const fhip_batch *b) { return b ? (int)b->ctxs.size() : 0; }

@@ -233,12 +233,11 @@ struct xfwd_rows {
     static constexpr int value = tile_of<LX>::value > 1 ? tile_of<LX>::value / 2 : 1;
 };
 
-template <typename T, int LX>
+template <typename T, int LX, int RB = xfwd_rows<LX>::value>
 __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XfwdArgs& a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* twx, int bx)
 {
-    constexpr int RB = xfwd_rows<LX>::value;
     constexpr int ST = RB + 1;
     constexpr int CELLS = RB * LX;
     constexpr int CPT = (CELLS + NT - 1) / NT;
@@ -421,11 +420,10 @@ __device__ __forceinline__ cpx<T> join_pq(cpx<T> v, cpx<T> m, bool lower)
     return lower ? cpx<T>{v.x - m.y, v.y + m.x} : cpx<T>{m.x + v.y, v.x - m.y};
 }
 
-template <typename T, int LY, int SIGN>
+template <typename T, int LY, int SIGN, int TB = tile_of<LY, line_tile_cap<T>()>::value>
 __device__ __forceinline__ void ypass_body(const Consts& c, const Spectra<T>& sp, int slot0,
                                               const cpx<T>* twy, int bx, int by)
 {
-    constexpr int TB = tile_of<LY, line_tile_cap<T>()>::value;
     __shared__ cpx<T> buf[LY * TB];
     __shared__ cpx<T> tws[LY];
     const int tid = threadIdx.x;
@@ -2153,20 +2151,21 @@ struct Rep {
     int maxblocks, xblocks, zblocks, kblocks, ravg;
 };
 
-template <typename T, int LX>
-__global__ __launch_bounds__(NT) void kb_xfwd(const Rep<T>* __restrict__ reps, int phase)
+template <typename T, int LX, int RB>
+__global__ __launch_bounds__(NT) void kb_xfwd(const Rep<T>* __restrict__ reps, int phase, int mode)
 {
     const Rep<T>& r = reps[blockIdx.z];
     XfwdArgs a = r.xa;
     a.phase = phase;
-    xfwd_body<T, LX>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
+    if (mode >= 0) a.mode = mode;          // XF_FORCE: no drift (the tuner's timing runs)
+    xfwd_body<T, LX, RB>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
 }
 
-template <typename T, int LY, int SIGN>
+template <typename T, int LY, int SIGN, int TB>
 __global__ __launch_bounds__(NT) void kb_ypass(const Rep<T>* __restrict__ reps)
 {
     const Rep<T>& r = reps[blockIdx.z];
-    ypass_body<T, LY, SIGN>(r.c, r.sp, 0, r.twy, blockIdx.x, blockIdx.y);
+    ypass_body<T, LY, SIGN, TB>(r.c, r.sp, 0, r.twy, blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int LZ, int GROUP>

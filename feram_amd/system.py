@@ -366,6 +366,23 @@ class Batch:
     def use_graph(self, enable: bool = True) -> None:
         self._check(self._L.fhip_batch_graph(self.b, 1 if enable else 0), "batch_graph")
 
+    def tune(self, enable: bool = True) -> None:
+        """Time the tile shapes of each size class at its first step (default on)."""
+        self._check(self._L.fhip_batch_tune(self.b, 1 if enable else 0), "batch_tune")
+
+    def plan(self) -> list:
+        """Per size class: edges, chosen x rows and y lines, measured us per launch."""
+        out = []
+        for k in range(self.classes):
+            shape = (C.c_int32 * 5)()
+            us = (C.c_double * 4)()
+            tuned = self._L.fhip_batch_plan(self.b, k, shape, us)
+            if tuned < 0:
+                raise FhipError(f"batch_plan: status {tuned}")
+            out.append({"L": tuple(shape[:3]), "x_rows": shape[3], "y_lines": shape[4], "tuned": bool(tuned),
+                        "x_us": (us[0], us[1]), "y_us": (us[2], us[3])})
+        return out
+
     @property
     def classes(self) -> int:
         return int(self._L.fhip_batch_classes(self.b))

@@ -236,6 +236,16 @@ const char *fhip_batch_last_error(const fhip_batch *batch);
 int fhip_batch_step(fhip_batch *batch, int nsteps, fhip_energy *e);
 /* enable = 0: launch every step directly instead of replaying the graph. */
 int fhip_batch_graph(fhip_batch *batch, int enable);
+/* No counterpart in the reference (its tile shapes are FFTW's business).
+ * The first fhip_batch_step of a size class times two tile shapes for each of
+ * two knobs on the device -- rows per x-transform workgroup and lines per y
+ * tile, full or half -- and keeps the faster (default on; enable = 0 keeps
+ * the per-context shapes).  Plans are cached per (edges, precision, acoustic,
+ * replicas) for the process.  fhip_batch_plan reports class cls: shape =
+ * (Lx, Ly, Lz, x rows, y lines), us = measured us per launch (x full, x half,
+ * y full, y half); returns 1 once the class is tuned, 0 before. */
+int fhip_batch_tune(fhip_batch *batch, int enable);
+int fhip_batch_plan(const fhip_batch *batch, int cls, int32_t shape[5], double us[4]);
 int fhip_batch_size(const fhip_batch *batch);
 int fhip_batch_classes(const fhip_batch *batch);
 

@@ -165,3 +165,27 @@ def test_batch_rejects_what_it_cannot_run():
     with Batch([a, c]) as bt:
         with pytest.raises(FhipError, match="averaging settings"):
             bt.step(1)
+
+
+def test_autotuned_plan_agrees_with_untuned():
+    """The tuner picks, per size class, the full or half tile for the x rows
+    and the y lines; either plan computes the same step to rounding."""
+    from feram_amd.system import Batch
+    print("[gpu] batch tile-shape tuner", flush=True)
+    a, b = pair("md")
+    for s in a + b:
+        s.md_prologue()
+    with Batch(a) as ba, Batch(b) as bb:
+        bb.tune(False)
+        assert not any(p["tuned"] for p in ba.plan())
+        ea = ba.step(6)
+        eb = bb.step(6)
+        plan, fixed = ba.plan(), bb.plan()
+    for p, q in zip(plan, fixed):
+        assert p["tuned"] and not q["tuned"]
+        assert p["L"] == q["L"]
+        assert p["x_rows"] in (q["x_rows"], 2 * q["x_rows"])
+        assert p["y_lines"] in (q["y_lines"], q["y_lines"] // 2, 1)
+        print(f"  class {p['L']}: x rows {p['x_rows']} (us {p['x_us'][0]:.1f}/{p['x_us'][1]:.1f}), "
+              f"y lines {p['y_lines']} (us {p['y_us'][0]:.1f}/{p['y_us'][1]:.1f})", flush=True)
+    compare(a, b, ea, eb)

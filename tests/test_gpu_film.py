@@ -175,3 +175,32 @@ def test_film_batch_matches_individual_steps(method):
         scale = sum(abs(v) for v in es[i].values())
         for k, v in es[i].items():
             assert abs(eb[i][k] - v) <= 1e-12 * scale, (i, k)
+
+
+FP32_TOL = 2e-5
+
+
+@pytest.mark.parametrize("geometry,gap,method", [("epit", 0, "md"), ("film", 2, "lf")])
+def test_film_fp32_trajectory(oracle_mod, geometry, gap, method):
+    """FP32 spectra with FP64 state: the film path against the FP64 oracle
+    to the FP32 trajectory tolerance of the bulk tests."""
+    from feram_amd.system import System
+    p = _params((16, 16, 16), geometry, gap, method)
+    print(f"[gpu] fp32 {geometry} gap_id={gap} {method}", flush=True)
+    o = oracle_mod.Oracle(p)
+    g = System(p, precision=1)
+    rng = np.random.default_rng(23)
+    u = rng.uniform(-0.05, 0.05, size=(p.Lz, p.Ly, p.Lx, 3))
+    for z in _dead(p):
+        u[z] = GAP_U
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    g.md_prologue()
+    for _ in range(20):
+        o.step(e_o)
+    eg = g.step(20)
+    for name in ("dipoR", "dipoP"):
+        assert _nw(g.get(name), o.field(name)) <= FP32_TOL, name
+    _check_energies(eg, e_o.as_dict(), FP32_TOL)
+    g.close()

@@ -60,3 +60,32 @@ def test_leapfrog_heat_source_matches_oracle(oracle_mod, acoustic):
     sd, sa = g.layer_energies(averaged=True)
     assert np.allclose(sd, sums[0], rtol=1e-12) and np.allclose(sa, sums[1], rtol=1e-12, atol=1e-300)
     g.close()
+
+
+def test_leapfrog_heat_source_fp32(oracle_mod):
+    """FP32 spectra with FP64 state and FP64 layer sums, 10 steps."""
+    from feram_amd.system import System
+    print("[gpu] heat-source layers, fp32", flush=True)
+    p = batio3_zvr((16, 16, 16), method="lf", kelvin=300.0)
+    o = oracle_mod.Oracle(p)
+    g = System(p, precision=1)
+    rng = np.random.default_rng(19)
+    u = 0.06 + 0.05 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+    pm = 0.1 * rng.standard_normal(u.shape)
+    for name, v in (("dipoR", u), ("dipoP", pm)):
+        o.field(name)[...] = v
+        g.put(name, v)
+    o.set_heat_source(0.002)
+    g.set_heat_source(0.002)
+    e = o.mirror_force_mirror()
+    g.mirror_force_mirror()
+    for _ in range(10):
+        o.leapfrog(e, 1)
+    g.leapfrog(1, 10)
+    ed, _ = g.layer_energies()
+    od, _ = o.layers()
+    assert np.allclose(ed, od, rtol=2e-5, atol=0)
+    for name in ("dipoR", "dipoP"):
+        a, b = g.get(name), o.field(name)
+        assert np.max(np.abs(a - b)) <= 2e-5 * np.max(np.abs(b)), name
+    g.close()
