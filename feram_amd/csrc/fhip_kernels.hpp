@@ -255,6 +255,12 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
     double u[CPT][3], w[CPT][3];
     bool ok[CPT];
     size_t cell[CPT];
+    // every cell's code, displacement and momentum are loaded before any
+    // drift is stored: a cell's loads cannot move above the previous cell's
+    // stores to u, so loading in the drift loop waits once per cell
+    const bool pre = a.phase != XP_MIRROR && (a.mode == XF_NP || a.mode == XF_LF || a.mode == XF_VS);
+    unsigned char cpre[CPT];
+    double upre[CPT][3], ppre[CPT][3];
 #pragma unroll
     for (int i = 0; i < CPT; ++i) {
         const int e = tid + i * NT;
@@ -262,6 +268,23 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
         ok[i] = (CELLS % NT == 0 || e < CELLS) && (row0 + r < nrows)
                 && (a.phase == XP_ALL || ((row0 + r >= mrow) == (a.phase == XP_MIRROR)));
         cell[i] = (size_t)(row0 + r) * LX + x;
+        cpre[i] = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) upre[i][q] = ppre[i][q] = 0.0;
+        if (pre && ok[i]) {
+            const size_t cc = cell[i];
+            cpre[i] = f.code[cc];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                upre[i][q] = f.u[q][cc];
+                ppre[i][q] = (a.mode == XF_NP ? f.pk[q] : f.p[q])[cc];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int e = tid + i * NT;
+        const int r = e / LX, x = e - r * LX;
         if (a.mode == XF_GENERIC) continue;
         if (!ok[i]) {
             u[i][0] = u[i][1] = u[i][2] = 0.0;
@@ -269,7 +292,7 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
             continue;
         }
         const size_t cc = cell[i];
-        const unsigned char code = f.code[cc];
+        const unsigned char code = pre ? cpre[i] : f.code[cc];
         const int spc = code & 3;
         const double minv = (code & 4) ? 0.0 : c.minv[spc];
         if (a.phase == XP_MIRROR) {
@@ -289,19 +312,19 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
             const double cd = sc->c_drift;
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                double uq = f.u[q][cc] + cd * f.pk[q][cc] * minv;     // nose-poincare.F:41-53
+                double uq = upre[i][q] + cd * ppre[i][q] * minv;     // nose-poincare.F:41-53
                 u[i][q] = uq;
                 f.u[q][cc] = uq;
             }
         } else if (a.mode == XF_LF || a.mode == XF_VS) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                double pq = f.p[q][cc];
+                double pq = ppre[i][q];
                 if (a.mode == XF_VS) {                                 // velocity_scaling.F:77-83
                     pq = pq * sc->scale_p;
                     f.p[q][cc] = pq;
                 }
-                double uq = f.u[q][cc] + c.dt * pq * minv;            // leapfrog.F:104-114
+                double uq = upre[i][q] + c.dt * pq * minv;            // leapfrog.F:104-114
                 u[i][q] = uq;
                 f.u[q][cc] = uq;
             }
@@ -431,26 +454,72 @@ __device__ __forceinline__ void ypass_body(const Consts& c, const Spectra<T>& sp
     const int l0 = bx * TB;
     const int nlines = c.Mx * c.Lz;
     cpx<T>* s = sp.s[slot0 + by];
-    for (int e = tid; e < LY * TB; e += NT) {
-        const int t = e / TB, b = e - t * TB, l = l0 + b;
-        if (l < nlines) {
-            const int z = l / c.Mx, kx = l - z * c.Mx;
-            const size_t col = (size_t)z * LY * c.Mx + kx;
-            cpx<T> v = s[col + (size_t)t * c.Mx];
-            if (SIGN > 0 && kx == 0 && t != 0 && 2 * t != LY)
-                v = join_pq(v, s[col + (size_t)(LY - t) * c.Mx], 2 * t < LY);
-            buf[e] = v;
+    if constexpr (NT % TB == 0) {
+        // a thread keeps one column b = tid % TB and every (NT / TB)-th row:
+        // the column's (z, kx) is found once, and all its loads are issued
+        // before the first LDS store
+        constexpr int RPI = NT / TB;
+        constexpr int IT = (LY + RPI - 1) / RPI;
+        const int b = tid % TB, t0 = tid / TB, l = l0 + b;
+        const bool okl = l < nlines;
+        const int z = okl ? l / c.Mx : 0, kx = okl ? l - z * c.Mx : 1;
+        const size_t col = (size_t)z * LY * c.Mx + kx;
+        cpx<T> v[IT];
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int t = t0 + i * RPI;
+            v[i] = cpx<T>{T(0), T(0)};
+            if (okl && (LY % RPI == 0 || t < LY)) v[i] = s[col + (size_t)t * c.Mx];
         }
-    }
-    __syncthreads();
-    fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
-    for (int e = tid; e < LY * TB; e += NT) {
-        const int t = e / TB, b = e - t * TB, l = l0 + b;
-        if (l < nlines) {
-            const int z = l / c.Mx, kx = l - z * c.Mx;
-            cpx<T> v = buf[e];
-            if (SIGN < 0 && kx == 0 && t != 0 && 2 * t != LY) v = split_pq(v, buf[(LY - t) * TB + b], 2 * t < LY);
-            s[((size_t)z * LY + t) * c.Mx + kx] = v;
+        if (SIGN > 0 && okl && kx == 0) {
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const int t = t0 + i * RPI;
+                if ((LY % RPI == 0 || t < LY) && t != 0 && 2 * t != LY)
+                    v[i] = join_pq(v[i], s[col + (size_t)(LY - t) * c.Mx], 2 * t < LY);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            const int t = t0 + i * RPI;
+            if (LY % RPI == 0 || t < LY) buf[t * TB + b] = v[i];
+        }
+        __syncthreads();
+        fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
+        if (okl) {
+#pragma unroll
+            for (int i = 0; i < IT; ++i) {
+                const int t = t0 + i * RPI;
+                if (LY % RPI == 0 || t < LY) {
+                    cpx<T> w = buf[t * TB + b];
+                    if (SIGN < 0 && kx == 0 && t != 0 && 2 * t != LY)
+                        w = split_pq(w, buf[(LY - t) * TB + b], 2 * t < LY);
+                    s[col + (size_t)t * c.Mx] = w;
+                }
+            }
+        }
+    } else {
+        for (int e = tid; e < LY * TB; e += NT) {
+            const int t = e / TB, b = e - t * TB, l = l0 + b;
+            if (l < nlines) {
+                const int z = l / c.Mx, kx = l - z * c.Mx;
+                const size_t col = (size_t)z * LY * c.Mx + kx;
+                cpx<T> v = s[col + (size_t)t * c.Mx];
+                if (SIGN > 0 && kx == 0 && t != 0 && 2 * t != LY)
+                    v = join_pq(v, s[col + (size_t)(LY - t) * c.Mx], 2 * t < LY);
+                buf[e] = v;
+            }
+        }
+        __syncthreads();
+        fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
+        for (int e = tid; e < LY * TB; e += NT) {
+            const int t = e / TB, b = e - t * TB, l = l0 + b;
+            if (l < nlines) {
+                const int z = l / c.Mx, kx = l - z * c.Mx;
+                cpx<T> v = buf[e];
+                if (SIGN < 0 && kx == 0 && t != 0 && 2 * t != LY) v = split_pq(v, buf[(LY - t) * TB + b], 2 * t < LY);
+                s[((size_t)z * LY + t) * c.Mx + kx] = v;
+            }
         }
     }
 }
@@ -1153,9 +1222,39 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
     }
 
     T acc[CPT][NF];
+    // FP64 spectra: the cells' displacement, the momenta the kick reads and
+    // the cell codes are requested before the last transform, which covers
+    // their latency (FP64 128^3: 94 -> 91 us).  FP32 reads them in the cell
+    // loop: the 48 extra registers would cost a wave per SIMD there
+    // (FP32 128^3: 58 -> 71 us with the prefetch).
+    constexpr bool PREF = NF >= 9 && sizeof(T) == 8;
+    constexpr int CPL = PREF ? CPT : 1;
+    double up[CPL][3], pp[CPL][3];
+    unsigned char cp[CPL];
+    const bool kick = a.mode == XI_NP || a.mode == XI_LF || a.mode == XI_VS;
 #pragma unroll
     for (int f0 = 0; f0 < NF; f0 += 2) {
         const bool two = (f0 + 1 < NF);
+        if (PREF && f0 + 2 >= NF && a.mode != XI_GENERIC) {
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) {
+                const int e = tid + i * NT;
+                const int r = e / LX, x = e - r * LX;
+                cp[i] = 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) up[i][q] = pp[i][q] = 0.0;
+                if ((CELLS % NT == 0 || e < CELLS) && row0 + r < nrows) {
+                    const size_t cc = (size_t)(row0 + r) * LX + x;
+                    cp[i] = f.code[cc];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) up[i][q] = f.u[q][cc];
+                    if (kick) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) pp[i][q] = (a.mode == XI_NP ? f.pk[q] : f.p[q])[cc];
+                    }
+                }
+            }
+        }
         // Z = A + i B with both half spectra completed Hermitian; kx = 0
         // holds the real A[0] + i A[Lx/2].
 #pragma unroll
@@ -1200,7 +1299,13 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
             continue;
         }
         if constexpr (NF >= 9) {
-            const unsigned char code = f.code[cc];
+            // the cell's state: prefetched (FP64) or loaded here (FP32)
+            auto u_of = [&](int q) -> double { if constexpr (PREF) return up[i][q]; else return f.u[q][cc]; };
+            auto p_of = [&](int q) -> double {
+                if constexpr (PREF) return pp[i][q]; else return (a.mode == XI_NP ? f.pk[q] : f.p[q])[cc];
+            };
+            unsigned char code;
+            if constexpr (PREF) code = cp[i]; else code = f.code[cc];
             const int s = code & 3;
             const bool counted = (code & 8) != 0;
             // film/epit: the dipole force and kick of a mirrored cell come from
@@ -1211,7 +1316,7 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
             double u[3], F[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                u[q] = f.u[q][cc];
+                u[q] = u_of(q);
                 F[q] = (double)acc[i][q] * c.Ninv;                    // dVddi-dipole-dipole.F:34-38
             }
             // on-site anharmonic terms (dVddi-unharmonic.F:34-65)
@@ -1276,7 +1381,7 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
                 const double dt2s = c.dt_2 * sc->s;
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double p1 = f.pk[q][cc] - dt2s * F[q];            // nose-poincare.F:66-71
+                    const double p1 = p_of(q) - dt2s * F[q];                 // nose-poincare.F:66-71
                     const double p2 = p1 - dt2s * F[q];                      // nose-poincare.F:24-29 (next step)
                     if (counted) { ke1 += p1 * p1 * minv; ke2 += p2 * p2 * minv; }
                     f.pk[q][cc] = p2;
@@ -1285,14 +1390,14 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
             } else if (a.mode == XI_LF) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double pq = f.p[q][cc] - c.dt * F[q];             // leapfrog.F:134-139
+                    const double pq = p_of(q) - c.dt * F[q];                 // leapfrog.F:134-139
                     if (counted) ke1 += pq * pq * minv;
                     f.p[q][cc] = pq;
                 }
             } else if (a.mode == XI_VS) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
-                    const double pq = counted ? f.p[q][cc] - c.dt * F[q] : 0.0;   // velocity_scaling.F:62-76
+                    const double pq = counted ? p_of(q) - c.dt * F[q] : 0.0;      // velocity_scaling.F:62-76
                     if (counted) ke1 += pq * pq * minv;
                     f.p[q][cc] = pq;
                 }
