@@ -233,12 +233,18 @@ struct xfwd_rows {
     static constexpr int value = tile_of<LX>::value > 1 ? tile_of<LX>::value / 2 : 1;
 };
 
+// Row stride of the x-pass LDS tiles (element x of row r at buf[x*ST + r]):
+// RB + XPAD.
+#ifndef FHIP_XPAD
+#define FHIP_XPAD 1
+#endif
+
 template <typename T, int LX, int RB = xfwd_rows<LX>::value>
 __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XfwdArgs& a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* twx, int bx)
 {
-    constexpr int ST = RB + 1;
+    constexpr int ST = RB + FHIP_XPAD;
     constexpr int CELLS = RB * LX;
     constexpr int CPT = (CELLS + NT - 1) / NT;
     constexpr int MX = LX / 2;
@@ -687,8 +693,15 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
                     KT s2 = KT(0);
 #pragma unroll
                     for (int l2 = 0; l2 < bb; ++l2) s2 += G[bb][l2] * G[bb][l2];
-                    G[bb][bb] = sqrt(K[bb][bb] - s2);
-                    Gi[bb] = KT(1) / G[bb][bb];
+                    if constexpr (sizeof(KT) == 4) {
+                        // FP32 mode: the reciprocal square root in one
+                        // instruction (1 ulp) instead of a correctly rounded
+                        // sqrt and divide; G's diagonal is not used again
+                        Gi[bb] = __builtin_amdgcn_rsqf(K[bb][bb] - s2);
+                    } else {
+                        G[bb][bb] = sqrt(K[bb][bb] - s2);
+                        Gi[bb] = KT(1) / G[bb][bb];
+                    }
 #pragma unroll
                     for (int aa = bb + 1; aa < 3; ++aa) {
                         KT s3 = KT(0);
@@ -1192,7 +1205,7 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
                                              const cpx<T>* twx, int bx)
 {
     constexpr int RB = tile_of<LX>::value;
-    constexpr int ST = RB + 1;
+    constexpr int ST = RB + FHIP_XPAD;
     constexpr int CELLS = RB * LX;
     constexpr int CPT = (CELLS + NT - 1) / NT;
     constexpr int MX = LX / 2;
