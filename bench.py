@@ -432,8 +432,9 @@ def main():
         print(json.dumps({"copy_bytes_per_launch_read": n * 8, "copy_bytes_per_launch_write": n * 8}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not run "
-              "(replica ensembles shard with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the step is measured on one GPU; --gpus {args.gpus} is not run "
+              "(replica ensembles would shard across GPUs with no exchange, one supercell as z-slabs "
+              "with one all-to-all per transform direction)", file=sys.stderr)
         sys.exit(2)
 
     try:

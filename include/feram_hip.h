@@ -38,7 +38,7 @@ extern "C" {
 enum {
     FHIP_OK = 0,
     FHIP_ERR_ARG = -1,          /* bad argument or size */
-    FHIP_ERR_UNSUPPORTED = -2,  /* configuration outside what this build runs */
+    FHIP_ERR_UNSUPPORTED = -2,  /* configuration outside what the library runs */
     FHIP_ERR_HIP = -3,          /* HIP runtime error (message has the detail) */
     FHIP_ERR_ALLOC = -4,        /* device memory allocation failed */
     FHIP_ERR_STATE = -5         /* call out of order */
