@@ -477,20 +477,29 @@ __device__ __forceinline__ void ypass_body(const Consts& c, const Spectra<T>& sp
             v[i] = cpx<T>{T(0), T(0)};
             if (okl && (LY % RPI == 0 || t < LY)) v[i] = s[col + (size_t)t * c.Mx];
         }
-        if (SIGN > 0 && okl && kx == 0) {
-#pragma unroll
-            for (int i = 0; i < IT; ++i) {
-                const int t = t0 + i * RPI;
-                if ((LY % RPI == 0 || t < LY) && t != 0 && 2 * t != LY)
-                    v[i] = join_pq(v[i], s[col + (size_t)(LY - t) * c.Mx], 2 * t < LY);
-            }
-        }
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
             const int t = t0 + i * RPI;
             if (LY % RPI == 0 || t < LY) buf[t * TB + b] = v[i];
         }
         __syncthreads();
+        if constexpr (SIGN > 0) {
+            // a kx = 0 column carries P (t < Ly/2) and Q (Ly - t): rejoin
+            // C = P + iQ pairwise in LDS (the pair's owner reads both rows
+            // before writing either), only in tiles that hold such a column
+            const int lend = min(l0 + TB, nlines) - 1;
+            if (l0 % c.Mx == 0 || l0 / c.Mx != lend / c.Mx) {
+                if (okl && kx == 0) {
+                    for (int t = t0; 2 * t < LY; t += RPI) {
+                        if (t == 0) continue;
+                        const cpx<T> a = buf[t * TB + b], m = buf[(LY - t) * TB + b];
+                        buf[t * TB + b] = join_pq(a, m, true);
+                        buf[(LY - t) * TB + b] = join_pq(m, a, false);
+                    }
+                }
+                __syncthreads();
+            }
+        }
         fft_lds<T, LY, TB, NT, SIGN>(buf, tws, tid);
         if (okl) {
 #pragma unroll

@@ -95,3 +95,87 @@ def read_restart(system, path: str) -> dict:
         system.put("acouR", fields["acouR"])
         system.put("acouP", fields["acouP"])
     return fields
+
+
+# ---------------------------------------------------------------------------
+# .avg: one record per run (average_module.F:180-260)
+# ---------------------------------------------------------------------------
+
+# The record's edit descriptors in order: kelvin f8.3, E(3) f9.5, strain(6),
+# <u>(3), <uu>(6) as (e15.5, e13.5...), 15 energies as (e17.7, e15.7...),
+# <p>(3), <pp>(6) as (e15.5, e13.5...) (average_module.F:240).
+_AVG_GROUPS = [(1, "f", 8, 3), (3, "f", 9, 5), (6, "e", 13, 5), (3, "e", 13, 5), (6, "e", 13, 5),
+               (15, "e", 15, 7), (3, "e", 13, 5), (6, "e", 13, 5)]
+_AVG_ENERGIES = ("dipo_kinetic", "long_range", "dipole_E_field", "unharmonic", "homo_strain", "homo_coupling",
+                 "inho_strain", "inho_coupling", "total", "H_Nose_Poincare")
+
+
+def format_avg_values(values) -> str:
+    """The .avg record of its 43 values, in the reference's format
+    (f8.3,3f9.5,1(e15.5,5e13.5),1(e15.5,2e13.5),1(e15.5,5e13.5),1(1e17.7,14e15.7),
+    1(e15.5,2e13.5),1(e15.5,5e13.5)); the first E field of each e group is
+    two characters wider."""
+    v = [float(x) for x in values]
+    if len(v) != 43:
+        raise ValueError(f"an .avg record holds 43 values, got {len(v)}")
+    out, k = [], 0
+    for n, kind, w, d in _AVG_GROUPS:
+        for j in range(n):
+            x = v[k]
+            k += 1
+            if kind == "f":
+                s = f"{x:{w}.{d}f}"
+                out.append(s if len(s) <= w else "*" * w)
+            else:
+                out.append(fortran_e(x, w + 2 if j == 0 else w, d))
+    return "".join(out)
+
+
+def avg_values(avg: dict, p, n_sum_p: int | None = None, n_average: int | None = None) -> list:
+    """The 43 values of divide_and_write_Average (average_module.F:198-259)
+    from the Average_type sums of System.avg_get().
+
+    n_average defaults to the number of averaging steps summed; n_sum_p is
+    the count of sum_p cells (film and epit divide <u>, <uu>, <p>, <pp> by
+    it instead of N).  For 'lf' the written temperature is the one the run
+    reached (average_module.F:218-224); the other methods write p.kelvin."""
+    from .params import KELVIN_EV
+    n = int(avg["n_add"] if n_average is None else n_average)
+    if n <= 0:
+        raise ValueError("no averaging step has been summed (n_average = 0 writes no record)")
+    N = p.N
+    Ninv = 1.0 / N
+    e = {k: v / n for k, v in avg["energy"].items()}                 # div_Energy(avg%energy, n_average)
+    e = {k: v / N for k, v in e.items()}                              # div_Energy(avg%energy, N)
+    e2 = avg["e2"] / n * Ninv
+    dkt = avg["dipo_kinetic_true"] / n * Ninv
+    strain = [s / n for s in avg["strain"]]
+    if p.bulk_or_film in ("film", "epit"):
+        if not n_sum_p:
+            raise ValueError("film and epit records need n_sum_p")
+        norm = lambda xs: [x / n / n_sum_p for x in xs]                 # noqa: E731
+    else:
+        norm = lambda xs: [x / n * Ninv for x in xs]                    # noqa: E731
+    u, uu, pm, pp = norm(avg["u"]), norm(avg["uu"]), norm(avg["p"]), norm(avg["pp"])
+    kelvin = p.kelvin
+    if p.method == "lf":
+        if p.mass_acou[0] > 0.0:
+            kelvin = (dkt + e["acou_kinetic"]) / 3 * KELVIN_EV
+        else:
+            kelvin = dkt * 2 / 3 * KELVIN_EV
+    vals = [kelvin] + list(p.external_E_field[:3]) + strain + u + uu
+    vals += [e[k] for k in _AVG_ENERGIES] + [e2, dkt, e["acou_kinetic"], e["short_range"], e["inho_modulation"]]
+    vals += pm + pp
+    return vals
+
+
+def write_avg(system, path: str, n_average: int | None = None) -> str:
+    """Write the .avg record of a System's device-side averages to path
+    (replacing the file, as the reference opens it with status='REPLACE')
+    and return the line."""
+    avg = system.avg_get()
+    n_sum_p = int(system.counted().sum()) if system.film else None
+    line = format_avg_values(avg_values(avg, system.p, n_sum_p, n_average))
+    with open(path, "w") as f:
+        f.write(line + "\n")
+    return line

@@ -70,3 +70,42 @@ def test_average_over_np_trajectory(oracle_mod):
     o.add_to_average(e_o, acc, d1, d3)
     assert g.avg_get()["n_add"] == nsteps + 1
     assert np.array(g.avg_get()["u"]) == pytest.approx(np.array(acc.as_dict()["u"]), rel=TOL)
+
+
+def test_avg_record_from_device_sums(oracle_mod, tmp_path):
+    """write_avg on a device run gives the record the oracle's add_to_Average
+    sums give (to the record's printed digits)."""
+    from feram_amd import formats
+    from feram_amd.system import System
+    p = batio3_zvr(12, method="md", kelvin=300.0)
+    print("[gpu] .avg record 12^3 md", flush=True)
+    o = oracle_mod.Oracle(p)
+    g = System(p)
+    rng = np.random.default_rng(7)
+    u = 0.1 + 0.05 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    g.md_prologue()
+    acc = oracle_mod.FoAverage()
+    d1 = np.zeros((199, 3), np.int64)
+    d3 = np.zeros((199, 199, 199), np.int32)
+    g.avg_reset()
+    g.avg_auto(True)
+    nsteps = 4
+    g.nose_poincare(nsteps)
+    for _ in range(nsteps):
+        o.nose_poincare(e_o)
+        e_o.total = (e_o.dipo_kinetic / o.nose[0] ** 2 + e_o.acou_kinetic
+                     + oracle_mod.lib().fo_sum_potential_energies(e_o))
+        o.add_to_average(e_o, acc, d1, d3)
+    path = tmp_path / "run.avg"
+    line = formats.write_avg(g, str(path))
+    assert path.read_text() == line + "\n"
+    got = np.array(line.split(), dtype=float)
+    want = np.array(formats.avg_values(acc.as_dict() | {"n_add": nsteps}, p), dtype=float)
+    assert got.size == 43 and got[0] == 300.0
+    # each E field prints a mantissa 0.ddddd (5 or 7 digits): rounding is at
+    # most 5e-5 of the value
+    assert np.allclose(got, want, rtol=6e-5, atol=1e-12)
+    g.close()
