@@ -51,11 +51,19 @@ def link_command(objdir: str, out: str = LIB) -> list:
         [os.path.join(objdir, os.path.basename(src) + ".o") for src in SOURCES]
 
 
+def _source_stamps() -> dict:
+    return {p: os.stat(p).st_mtime_ns for p in SOURCES + HEADERS}
+
+
 def build(verbose: bool = True) -> str:
     tmp = LIB + ".tmp"
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     t0 = time.time()
+    # hipcc reads a source twice, for the device pass and minutes later for the
+    # host pass: a source edited in between gives a library whose host launch
+    # stubs do not match its device code, so such a build is refused
+    stamps = _source_stamps()
     procs = []
     for cmd in compile_commands(objdir):
         if verbose:
@@ -75,6 +83,10 @@ def build(verbose: bool = True) -> str:
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError(f"hipcc link failed with status {res.returncode}")
+    changed = [os.path.relpath(p, ROOT) for p, t in _source_stamps().items() if stamps[p] != t]
+    if changed:
+        os.remove(tmp)
+        raise RuntimeError(f"sources changed while compiling ({', '.join(changed)}): build again")
     os.replace(tmp, LIB)
     if verbose:
         warn = [l for e in errs for l in e.splitlines() if "warning" in l and "packed-fp32" not in l]
