@@ -763,15 +763,22 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         }
         // energies with weight 1/2 on the kx = 0 and kx = Lx/2 planes
         // (energy_module.F:162-239)
+        // FP32 mode with optimised strain: w solves K w = -(c_mod + C uu), so
+        // Re(w^H K w) = -(E_ic + E_im) per point and the quadratic form is
+        // not evaluated; FP64 keeps the reference's sum (energy_module.F:162-177)
+        constexpr bool EIS_BY_SOLVE = sizeof(KT) == 4 && GROUP == ZG_STRAIN;
         KT eis = KT(0), eic = KT(0), eim = KT(0);
 #pragma unroll
         for (int aa = 0; aa < 3; ++aa) {
+            if constexpr (!EIS_BY_SOLVE) {
 #pragma unroll
-            for (int bb = 0; bb < 3; ++bb)
-                eis += K[aa][bb] * (w[aa].x * w[bb].x + w[aa].y * w[bb].y);
+                for (int bb = 0; bb < 3; ++bb)
+                    eis += K[aa][bb] * (w[aa].x * w[bb].x + w[aa].y * w[bb].y);
+            }
             eic += w[aa].x * Cu[aa].x + w[aa].y * Cu[aa].y;
             eim += w[aa].x * cm[aa].x + w[aa].y * cm[aa].y;
         }
+        if constexpr (EIS_BY_SOLVE) eis = -(eic + eim);
         acc0 += wm * (double)eis;
         acc1 += wm * (double)eic;
         acc2 += wm * (double)eim;
