@@ -1,4 +1,4 @@
-"""NP-MD throughput of the per-step path on one MI355X, FP32 mode vs FP64 plain mode.
+"""NP-MD throughput of the per-step path on one MI355X, FP64 plain mode and FP32 mode.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--precision both|fp32|fp64]
 
@@ -10,17 +10,21 @@ acoustic displacement optimised each step.  Initial u is uniform in
 tables are resident in HBM before the timed region; K steps run back to back
 on the library's stream, bracketed by device synchronisation.
 
-Two modes are timed on the same workload: FP64 plain (everything FP64) and
-FP32 (spectra, FFTs and the k-space stage in FP32; state, local forces,
-thermostat, energy sums and homogeneous strain in FP64).  ``value`` is the
-FP32-mode rate; the FP64 rate, their ratio and the FP32 force/energy error
-against FP64 on the initial configuration are in ``modes``.
+Two modes are timed on the same workload: FP64 plain (everything FP64, the
+reference's arithmetic) and FP32 (spectra, FFTs and the k-space stage in
+FP32; state, local forces, thermostat, energy sums and homogeneous strain in
+FP64).  ``value`` is the FP64 plain-mode rate; the FP32 rate and the ratio
+are in ``modes``.  Each mode's force and energy error against the FP64 oracle
+(the C restatement of the reference) after the prologue and after the first
+two steps is reported per run in ``modes.<mode>.vs_oracle``: the oracle runs
+the same prologue and two steps from the same doubles in the CPU-baseline leg.
 
 ``roofline`` is the dominant kernel's algorithmic HBM bytes per launch over
 its average launch time, taken with HIP events on the library's stream in a
-second pass of the same K steps right after the timed one.  ``cpu_baseline``
-times the CPU FP64 oracle (C restatement of the reference, OpenMP over the
-host threads) on a bounded sample of the same workload.
+second pass of the same K steps right after the timed one; ``roofline.
+copy_GBs`` is a 1 GiB device copy timed in the same run (the attainable HBM
+rate on this box).  ``cpu_baseline`` times the CPU FP64 oracle (OpenMP over
+the host threads) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -69,20 +73,74 @@ def step_bytes(N: int, Kh: int, ws: int) -> float:
     return 18 * N * 8 + 78 * Kh * ws
 
 
-def cpu_baseline(p, u0, steps: int) -> dict:
+def cpu_baseline(p, u0, steps: int):
+    """The oracle (FP64, OpenMP over the host threads) on the same workload:
+    prologue untimed, `steps` NP steps timed.  Returns the baseline record and
+    the oracle's state after the prologue and after two steps."""
     import oracle
     oracle.build()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = oracle.Oracle(p)
     o.field("dipoR")[...] = u0
     e = o.md_prologue()
+    ref = {"e0": e.as_dict(), "F0": o.field("dVddi").copy()}
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         o.nose_poincare(e)
+        if i == 1:
+            ref.update(e2=e.as_dict(), u2=o.field("dipoR").copy(), p2=o.field("dipoP").copy(),
+                       F2=o.field("dVddi").copy())
     dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} NP steps of the same {p.Lx}^3 workload with the C FP64 oracle "
-                      f"(OpenMP, {threads} threads), after its untimed table set-up"}
+    o.close()
+    rec = {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} NP steps of the same {p.Lx}^3 workload with the C FP64 oracle "
+                     f"(OpenMP, {threads} threads), after its untimed table set-up and prologue"}
+    return rec, (ref if steps >= 2 else None)
+
+
+def errors_vs(first, ref) -> dict:
+    """Force and energy errors of a device run against a reference run
+    (normwise max|dF| / max|F|; energies against the sum of |terms|)."""
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+    def erel(a, b):
+        scale = sum(abs(b[t]) for t in ENERGY_TERMS)
+        return float(max(abs(a[t] - b[t]) for t in ENERGY_TERMS) / scale)
+
+    return {"force_rel_err_prologue": rel(first["F0"], ref["F0"]),
+            "energy_rel_err_prologue": erel(first["e0"], ref["e0"]),
+            "force_rel_err_step2": rel(first["F2"], ref["F2"]),
+            "energy_rel_err_step2": erel(first["e2"], ref["e2"]),
+            "u_rel_err_step2": rel(first["u2"], ref["u2"]),
+            "p_rel_err_step2": rel(first["p2"], ref["p2"])}
+
+
+def copy_rate(sync_extra) -> float | None:
+    """GB/s of a 1 GiB device-to-device copy (read + write bytes over the
+    copy time), timed with events on torch's stream, the stream it runs on."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            return None
+    except Exception:
+        return None
+    n = (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 10
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
 
 
 def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
@@ -91,6 +149,9 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
     g.put("dipoR", u0)
     e0 = g.md_prologue()
     F0 = g.get("dVddi")
+    # the first two steps, compared with the oracle's (the CPU-baseline leg)
+    e2 = g.nose_poincare(2)
+    first = {"e0": e0, "F0": F0, "e2": e2, "u2": g.get("dipoR"), "p2": g.get("dipoP"), "F2": g.get("dVddi")}
     if warmup > 0:
         g.nose_poincare(warmup)
 
@@ -128,30 +189,26 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
                              "frac": sb * steps / dt / 1e9 / HBM_PEAK_GBS},
            "device_bytes": g.device_bytes, "total_energy_eV_per_uc": e["total"] / N}
     g.close()
-    return res, e0, F0
+    return res, first
 
 
 def sweep_replicas(n: int, precision: int, seed: int = 20241203):
-    """The batched-ensemble workload: n BaTiO3 bulk replicas with edges drawn
-    from {12, 16, 24, 32} (seeded, uneven mix), T ~ U[150, 450] K, static
-    E_z ~ U[0, 100] kV/cm (0 to 1e-3 V/A), NP-MD, u0 ~ U[-0.05, 0.05] A."""
-    from feram_amd.params import batio3_zvr
+    """The batched-ensemble workload (feram_amd.sweeps.replica_ensemble): n
+    BaTiO3 bulk replicas with edges drawn from {12, 16, 24, 32} (seeded,
+    uneven mix), T ~ U[150, 450] K, static E_z ~ U[0, 100] kV/cm, NP-MD,
+    u0 ~ U[-0.05, 0.05] A."""
+    from feram_amd.sweeps import replica_ensemble
     from feram_amd.system import System
-    rng = np.random.default_rng(seed)
-    edges = rng.choice([12, 16, 24, 32], size=n)
-    temps = rng.uniform(150.0, 450.0, size=n)
-    fields = rng.uniform(0.0, 1e-3, size=n)
+    params, u0 = replica_ensemble(n, seed)
     systems = []
-    for L, T, Ez in zip(edges, temps, fields):
-        L = int(L)
-        p = batio3_zvr(L, method="md", kelvin=float(T))
-        p.Q_Nose = 0.1 * p.N / 16 ** 3          # heating example's 0.1 at 16^3, scaled with N
-        p.external_E_field = [0.0, 0.0, float(Ez)]
-        p.E_field_always_zero = False
+    for p, u in zip(params, u0):
         g = System(p, precision=precision)
-        g.put("dipoR", rng.uniform(-0.05, 0.05, size=(L, L, L, 3)))
+        g.put("dipoR", u)
         g.md_prologue()
         systems.append(g)
+    edges = np.array([p.Lx for p in params])
+    temps = np.array([p.kelvin for p in params])
+    fields = np.array([p.external_E_field[2] for p in params])
     return systems, edges, temps, fields
 
 
@@ -478,25 +535,21 @@ def main():
     u0 = rng.uniform(-0.05, 0.05, size=(L, L, L, 3))
 
     modes = {}
-    refs = {}
+    firsts = {}
     for name, prec in (("fp64", 0), ("fp32", 1)):
         if args.precision not in ("both", name):
             continue
         print(f"[bench] {L}^3 {name}: warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
-        res, e0, F0 = measure(p, u0, prec, args.steps, args.warmup, sync_extra)
+        res, first = measure(p, u0, prec, args.steps, args.warmup, sync_extra)
         print(f"[bench] {name}: {res['steps_per_s']:.1f} steps/s", file=sys.stderr, flush=True)
         modes[name] = res
-        refs[name] = (e0, F0)
-    if "fp64" in refs and "fp32" in refs:
-        (e64, F64), (e32, F32) = refs["fp64"], refs["fp32"]
-        scale = sum(abs(e64[t]) for t in ENERGY_TERMS)
-        modes["fp32"]["force_rel_err_vs_fp64"] = float(np.max(np.abs(F32 - F64)) / np.max(np.abs(F64)))
-        modes["fp32"]["energy_rel_err_vs_fp64"] = float(
-            max(abs(e32[t] - e64[t]) for t in ENERGY_TERMS) / scale)
+        firsts[name] = first
+    if "fp64" in modes and "fp32" in modes:
+        modes["fp32"]["vs_fp64_mode"] = errors_vs(firsts["fp32"], firsts["fp64"])
         modes["fp32"]["speedup_vs_fp64"] = modes["fp32"]["steps_per_s"] / modes["fp64"]["steps_per_s"]
 
-    head = modes["fp32"] if "fp32" in modes else modes["fp64"]
-    head_name = "fp32" if "fp32" in modes else "fp64"
+    head_name = "fp64" if "fp64" in modes else "fp32"
+    head = modes[head_name]
     dom = head["dominant"]
     kd = head["kernels"][dom]
     achieved = kd["GBs"]
@@ -505,15 +558,21 @@ def main():
     if os.path.exists(tr_path):
         with open(tr_path) as f:
             traffic = json.load(f).get(f"{L}:{head_name}:{dom}")
+    copy_gbs = copy_rate(sync_extra)
 
     cpu = None
     if not args.no_cpu:
-        print(f"[bench] cpu baseline ({args.cpu_steps} steps)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(p, u0, args.cpu_steps)
+        print(f"[bench] cpu baseline ({args.cpu_steps} steps) and oracle comparison", file=sys.stderr, flush=True)
+        cpu, ref = cpu_baseline(p, u0, args.cpu_steps)
+        if ref is not None:
+            for name in modes:
+                modes[name]["vs_oracle"] = errors_vs(firsts[name], ref)
+                print(f"[bench] {name} vs oracle: force {modes[name]['vs_oracle']['force_rel_err_step2']:.2e}, "
+                      f"energy {modes[name]['vs_oracle']['energy_rel_err_step2']:.2e}", file=sys.stderr, flush=True)
 
     N = p.N
     out = {
-        "metric": "NP-MD steps/s at 128^3 (FP32 mode; FP64 plain mode alongside)",
+        "metric": f"NP-MD steps/s at {L}^3 ({head_name.upper()} plain mode; the other mode alongside)",
         "value": head["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
@@ -531,7 +590,8 @@ def main():
                    "cell_updates_per_s": head["steps_per_s"] * N},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": traffic, "bytes_per_launch": kd["bytes"], "avg_launch_us": kd["avg_us"]},
+                     "traffic": traffic, "bytes_per_launch": kd["bytes"], "avg_launch_us": kd["avg_us"],
+                     "copy_GBs": copy_gbs},
         "step_roofline": head["step_roofline"],
         "modes": modes,
         "cpu_baseline": cpu,

@@ -19,7 +19,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libferam_hip.so")
 SOURCES = [os.path.join(CSRC, "feram_hip.hip"), os.path.join(CSRC, "fhip_batch.hip"),
-           os.path.join(CSRC, "fhip_mlp.hip")]
+           os.path.join(CSRC, "fhip_mlp.hip"), os.path.join(CSRC, "fhip_stages.hip")]
+# fhip_stages.hip holds the per-edge launchers and is compiled once per group
+# of edges (FHIP_STAGE_GROUP), so the edges build in parallel
+STAGE_GROUPS = 9
 HEADERS = [os.path.join(CSRC, "fhip_fft.hpp"), os.path.join(CSRC, "fhip_kernels.hpp"),
            os.path.join(CSRC, "fhip_host.hpp"), os.path.join(ROOT, "include", "feram_hip.h")]
 ARCH = os.environ.get("FHIP_ARCH", "gfx950")
@@ -40,15 +43,27 @@ def flags() -> list:
             "-I", os.path.join(ROOT, "include")]
 
 
+def _units() -> list:
+    """(source, extra flags, object name) of every translation unit."""
+    units = []
+    for src in SOURCES:
+        base = os.path.basename(src)
+        if base == "fhip_stages.hip":
+            units += [(src, [f"-DFHIP_STAGE_GROUP={g}"], f"{base}.{g}.o") for g in range(STAGE_GROUPS)]
+        else:
+            units.append((src, [], base + ".o"))
+    return units
+
+
 def compile_commands(objdir: str) -> list:
     """One hipcc -c per translation unit (run in parallel)."""
-    return [[hipcc()] + flags() + ["-c", src, "-o", os.path.join(objdir, os.path.basename(src) + ".o")]
-            for src in SOURCES]
+    return [[hipcc()] + flags() + extra + ["-c", src, "-o", os.path.join(objdir, obj)]
+            for src, extra, obj in _units()]
 
 
 def link_command(objdir: str, out: str = LIB) -> list:
     return [hipcc(), f"--offload-arch={ARCH}", "-fPIC", "-shared", "-o", out] + \
-        [os.path.join(objdir, os.path.basename(src) + ".o") for src in SOURCES]
+        [os.path.join(objdir, obj) for _, _, obj in _units()]
 
 
 def _source_stamps() -> dict:
@@ -64,18 +79,34 @@ def build(verbose: bool = True) -> str:
     # host pass: a source edited in between gives a library whose host launch
     # stubs do not match its device code, so such a build is refused
     stamps = _source_stamps()
-    procs = []
-    for cmd in compile_commands(objdir):
-        if verbose:
-            print("[feram_amd.build]", " ".join(cmd), flush=True)
-        procs.append(subprocess.Popen(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
-    errs = []
-    for pr in procs:
+    # the longest units first, at most `jobs` compilers at a time
+    cmds = sorted(compile_commands(objdir), key=lambda c: "fhip_stages" not in " ".join(c))
+    jobs = max(1, int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)))
+    errs, running = [], []
+
+    def reap(pr):
         out, err = pr.communicate()
         errs.append(err)
         if pr.returncode != 0:
+            for other in running:
+                other.kill()
             sys.stderr.write(out + err)
             raise RuntimeError(f"hipcc failed with status {pr.returncode}")
+
+    for cmd in cmds:
+        while len(running) >= jobs:
+            done = [pr for pr in running if pr.poll() is not None]
+            if not done:
+                time.sleep(0.2)
+                continue
+            for pr in done:
+                running.remove(pr)
+                reap(pr)
+        if verbose:
+            print("[feram_amd.build]", " ".join(cmd), flush=True)
+        running.append(subprocess.Popen(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+    while running:
+        reap(running.pop(0))
     cmd = link_command(objdir, tmp)
     if verbose:
         print("[feram_amd.build]", " ".join(cmd), flush=True)

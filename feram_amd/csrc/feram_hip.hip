@@ -34,11 +34,6 @@ bool size_supported(int n)
     }
 }
 
-const char *kKernelNames[K_NKERNEL] = {
-    "xfwd_drift_r2c", "y_forward", "z_dipole_dipole", "z_strain", "homo_strain", "y_inverse",
-    "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup",
-    "average", "z_pair"};
-
 template <typename P>
 int dalloc(fhip_ctx *ctx, P **ptr, size_t count)
 {
@@ -55,74 +50,15 @@ int dalloc(fhip_ctx *ctx, P **ptr, size_t count)
     return FHIP_OK;
 }
 
-hipEvent_t take_event(fhip_ctx *ctx)
-{
-    if (!ctx->ev_pool.empty()) {
-        hipEvent_t e = ctx->ev_pool.back();
-        ctx->ev_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-}
-
-// Launch helper: brackets the launch with events when measuring.
-template <typename F>
-int launch(fhip_ctx *ctx, int id, F &&fn, hipStream_t st = nullptr)
-{
-    if (!st) st = ctx->stream;
-    hipEvent_t a = nullptr, b = nullptr;
-    if (ctx->prof) {
-        a = take_event(ctx);
-        b = take_event(ctx);
-        if (a) hipEventRecord(a, st);
-    }
-    fn();
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return fail(ctx, FHIP_ERR_HIP, std::string("launch ") + kKernelNames[id] + ": " + hipGetErrorString(e));
-    if (ctx->prof && a && b) {
-        hipEventRecord(b, st);
-        ctx->recs.push_back({id, a, b});
-    }
-    return FHIP_OK;
-}
-
-int collect_events(fhip_ctx *ctx)
-{
-    if (ctx->recs.empty()) return FHIP_OK;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (auto &r : ctx->recs) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
-            ctx->kms[r.id] += ms;
-            ctx->kcount[r.id] += 1;
-        }
-        ctx->ev_pool.push_back(r.a);
-        ctx->ev_pool.push_back(r.b);
-    }
-    ctx->recs.clear();
-    return FHIP_OK;
-}
-
-
 
 // ---- size dispatch ---------------------------------------------------------
+// Each case calls the launcher instantiated for that edge (fhip_stages.hip).
 template <typename T>
 int run_xfwd(fhip_ctx *ctx, const XfwdArgs &a)
 {
-    Consts c = ctx->c;
-    Fields f = ctx->f;
-    Spectra<T> sp = tset<T>(ctx).sp;
-    const Scalars *sc = ctx->dsc;
-    const cpx<T> *tw = tset<T>(ctx).twx;
-    const int nb = ctx->xfblocks;
-    hipStream_t s = ctx->stream;
     switch (ctx->Lx) {
-#define FHIP_CASE(n)                                                                         \
-    case n:                                                                                  \
-        return launch(ctx, K_XFWD, [&] { k_xfwd<T, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+#define FHIP_CASE(n) \
+    case n: return fhip_stage::xfwd<T, n>(ctx, a);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -132,21 +68,9 @@ int run_xfwd(fhip_ctx *ctx, const XfwdArgs &a)
 template <typename T>
 int run_xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
 {
-    Consts c = ctx->c;
-    Fields f = ctx->f;
-    Spectra<T> sp = tset<T>(ctx).sp;
-    const Scalars *sc = ctx->dsc;
-    const cpx<T> *tw = tset<T>(ctx).twx;
-    const int nb = ctx->xblocks;
-    hipStream_t s = ctx->stream;
     switch (ctx->Lx) {
-#define FHIP_CASE(n)                                                                         \
-    case n:                                                                                  \
-        if (nf == 12)                                                                        \
-            return launch(ctx, K_XINV, [&] { k_xinv<T, n, 12><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
-        if (nf == 9)                                                                         \
-            return launch(ctx, K_XINV, [&] { k_xinv<T, n, 9><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); }); \
-        return launch(ctx, K_XINV, [&] { k_xinv<T, n, 3><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+#define FHIP_CASE(n) \
+    case n: return fhip_stage::xinv<T, n>(ctx, a, nf);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -156,48 +80,24 @@ int run_xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
 template <typename T>
 int run_ypass(fhip_ctx *ctx, int sign, int slot0, int nf, hipStream_t st = nullptr)
 {
-    Consts c = ctx->c;
-    Spectra<T> sp = tset<T>(ctx).sp;
-    const cpx<T> *tw = tset<T>(ctx).twy;
-    dim3 grid(ctx->yblocks[tix<T>()], nf);
     hipStream_t s = st ? st : ctx->stream;
-    const int id = sign < 0 ? K_YFWD : K_YINV;
     switch (ctx->Ly) {
-#define FHIP_CASE(n)                                                                                   \
-    case n:                                                                                            \
-        if (sign < 0)                                                                                  \
-            return launch(ctx, id, [&] { k_ypass<T, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }, s); \
-        return launch(ctx, id, [&] { k_ypass<T, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }, s);
+#define FHIP_CASE(n) \
+    case n: return fhip_stage::ypass<T, n>(ctx, sign, slot0, nf, s);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return fail(ctx, FHIP_ERR_UNSUPPORTED, "Ly not supported");
 }
 
-template <typename T, int G>
-int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t st = nullptr)
+// Plain z transforms of nplain slots (set-up work).
+template <typename T>
+int run_zplain(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t st = nullptr)
 {
-    Consts c = ctx->c;
-    Spectra<T> sp = tset<T>(ctx).sp;
-    Scalars *sc = ctx->dsc;
-    const cpx<T> *tw = tset<T>(ctx).twz;
-    const int nb = ctx->zblocks[tix<T>()];
     hipStream_t s = st ? st : ctx->stream;
-    const int id = (G == ZG_DD) ? K_ZDD : (G == ZG_PLAIN ? K_SETUP : K_ZSTRAIN);
-    // side buffer of the kx = 0 pencils: two NF-field values per point of up
-    // to (TB - 1) / Mx + 1 pencils per tile
-    const int nf = (G == ZG_DD) ? 3 : (G == ZG_STRAIN ? 6 : 9);
-    const int tz = ztile_rows<T>(ctx->Lz);
-    const ZArgs<T> za = a;
-    const size_t shm = (G == ZG_PLAIN) ? 0 : 2 * (size_t)((tz - 1) / ctx->Mx + 1) * ctx->Lz * nf * sizeof(cpx<T>);
-    if (shm > 96 * 1024)
-        return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
-    auto lds_ok = [&](const void *fn) { return ensure_lds(ctx, fn, shm); };
     switch (ctx->Lz) {
-#define FHIP_CASE(n)                                                                           \
-    case n:                                                                                    \
-        RC(lds_ok((const void *)k_zpass<T, n, G>));                                            \
-        return launch(ctx, id, [&] { k_zpass<T, n, G><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); }, s);
+#define FHIP_CASE(n) \
+    case n: return fhip_stage::zplain<T, n>(ctx, a, s, 0);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -208,24 +108,15 @@ int run_zpass_g(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t st = nullptr)
 template <typename T, int G2>
 int run_zpair(fhip_ctx *ctx, const ZArgs<T> &a)
 {
-    Consts c = ctx->c;
-    Spectra<T> sp = tset<T>(ctx).sp;
-    Scalars *sc = ctx->dsc;
-    const cpx<T> *tw = tset<T>(ctx).twz;
-    const int nb = ctx->zblocks[tix<T>()];
-    hipStream_t s = ctx->stream;
-    const int nf = (G2 == ZG_STRAIN) ? 6 : 9;      // the larger side buffer of the two groups
-    const int tz = ztile_rows<T>(ctx->Lz);
-    const ZArgs<T> za = a;
-    const size_t shm = 2 * (size_t)((tz - 1) / ctx->Mx + 1) * ctx->Lz * nf * sizeof(cpx<T>);
+    // side buffer of the kx = 0 pencils: two NF-field values per point of up
+    // to (TB - 1) / Mx + 1 pencils per tile, NF the larger of the two groups
+    const int nf = (G2 == ZG_STRAIN) ? 6 : 9;
+    const size_t shm = zside_bytes<T>(ctx->Lz, ctx->Mx, nf);
     if (shm > 96 * 1024)
         return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
-    auto lds_ok = [&](const void *fn) { return ensure_lds(ctx, fn, shm); };
     switch (ctx->Lz) {
-#define FHIP_CASE(n)                                                                           \
-    case n:                                                                                    \
-        RC(lds_ok((const void *)k_zpair<T, n, G2>));                                           \
-        return launch(ctx, K_ZPAIR, [&] { k_zpair<T, n, G2><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
+#define FHIP_CASE(n) \
+    case n: return fhip_stage::zpair<T, n, G2>(ctx, a, shm);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -599,7 +490,7 @@ int forward_generic(fhip_ctx *ctx, const double *const *fields, int nf)
     ZArgs<double> za = zargs<double>(ctx, false);
     za.nplain = nf;
     za.plain_sign = -1;
-    return run_zpass_g<double, ZG_PLAIN>(ctx, za);
+    return run_zplain<double>(ctx, za);
 }
 
 // FP64 spectral slots for set-up work in FP32 mode, released afterwards.
@@ -1387,7 +1278,7 @@ static int reconstruct_w_t(fhip_ctx *ctx, std::vector<double> &h)
         ZArgs<T> za = zargs<T>(ctx, false);
         za.nplain = 3;
         za.plain_sign = +1;
-        rc = run_zpass_g<T, ZG_PLAIN>(ctx, za);
+        rc = run_zplain<T>(ctx, za);
     }
     if (rc == FHIP_OK) rc = run_ypass<T>(ctx, +1, 0, 3);
     if (rc == FHIP_OK) {

@@ -21,10 +21,6 @@ using namespace fhip_host;
 
 namespace {
 
-// Edges of the batched instantiations.  Supercells above 64 per edge fill the
-// device alone and are stepped with the per-context calls.
-#define FHIP_BATCH_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
-
 bool batch_size_ok(int n)
 {
     switch (n) {
@@ -144,11 +140,6 @@ int upload_reps(fhip_batch *b, BClass &k)
 }
 
 // ---- tile-shape plan -----------------------------------------------------------
-template <typename T, int n>
-constexpr int ytile_full() { return tile_of<n, line_tile_cap<T>()>::value; }
-template <typename T, int n>
-constexpr int ytile_half() { return ytile_full<T, n>() > 1 ? ytile_full<T, n>() / 2 : 1; }
-
 void set_plan(BClass &k, int xrows, int ytile)
 {
     k.xrows = xrows;
@@ -168,6 +159,7 @@ std::array<int, 6> plan_key(const BClass &k)
 }
 
 // ---- per-stage launches of one class -----------------------------------------
+// Each case calls the launcher instantiated for that edge (fhip_stages.hip).
 template <typename T>
 int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s, int phase, int mode = -1)
 {
@@ -175,11 +167,8 @@ int l_xfwd(fhip_batch *b, BClass &k, hipStream_t s, int phase, int mode = -1)
     const dim3 g(k.xfblocks, 1, (unsigned)k.members.size());
     const bool half = k.xrows != tile_rows(k.Lx);
     switch (k.Lx) {
-#define FHIP_CASE(n)                                                                               \
-    case n:                                                                                        \
-        if (half) kb_xfwd<T, n, xfwd_rows<n>::value><<<g, NT, 0, s>>>(r, phase, mode);             \
-        else kb_xfwd<T, n, tile_of<n>::value><<<g, NT, 0, s>>>(r, phase, mode);                    \
-        BLAUNCH("xfwd");
+#define FHIP_CASE(n) \
+    case n: fhip_stage::b_xfwd<T, n>(r, g, s, half, phase, mode); BLAUNCH("xfwd");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -193,16 +182,8 @@ int l_ypass(fhip_batch *b, BClass &k, hipStream_t s, int sign)
     const dim3 g(k.yblocks, k.nf, (unsigned)k.members.size());
     const bool half = k.ytile != line_tile_rows<T>(k.Ly);
     switch (k.Ly) {
-#define FHIP_CASE(n)                                                                             \
-    case n:                                                                                      \
-        if (half) {                                                                              \
-            if (sign < 0) kb_ypass<T, n, -1, ytile_half<T, n>()><<<g, NT, 0, s>>>(r);            \
-            else kb_ypass<T, n, +1, ytile_half<T, n>()><<<g, NT, 0, s>>>(r);                     \
-        } else {                                                                                 \
-            if (sign < 0) kb_ypass<T, n, -1, ytile_full<T, n>()><<<g, NT, 0, s>>>(r);            \
-            else kb_ypass<T, n, +1, ytile_full<T, n>()><<<g, NT, 0, s>>>(r);                     \
-        }                                                                                        \
-        BLAUNCH("ypass");
+#define FHIP_CASE(n) \
+    case n: fhip_stage::b_ypass<T, n>(r, g, s, half, sign); BLAUNCH("ypass");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -214,7 +195,7 @@ const void *zpair_fn(int Lz)
 {
     switch (Lz) {
 #define FHIP_CASE(n) \
-    case n: return (const void *)kb_zpair<T, n, G2>;
+    case n: return fhip_stage::b_zpair_fn<T, n, G2>();
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -226,10 +207,9 @@ int l_zpair(fhip_batch *b, BClass &k, hipStream_t s, int want_acouK)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.zblocks, 1, (unsigned)k.members.size());
-    const size_t shm = k.zshm_g;
     switch (k.Lz) {
 #define FHIP_CASE(n) \
-    case n: kb_zpair<T, n, G2><<<g, NT, shm, s>>>(r, want_acouK); BLAUNCH("zpair");
+    case n: fhip_stage::b_zpair<T, n, G2>(r, g, s, k.zshm_g, want_acouK); BLAUNCH("zpair");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -242,11 +222,8 @@ int l_xinv(fhip_batch *b, BClass &k, hipStream_t s, int materialize)
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.xblocks, 1, (unsigned)k.members.size());
     switch (k.Lx) {
-#define FHIP_CASE(n)                                                          \
-    case n:                                                                   \
-        if (k.nf == 12) kb_xinv<T, n, 12><<<g, NT, 0, s>>>(r, materialize);   \
-        else kb_xinv<T, n, 9><<<g, NT, 0, s>>>(r, materialize);               \
-        BLAUNCH("xinv");
+#define FHIP_CASE(n) \
+    case n: fhip_stage::b_xinv<T, n>(r, g, s, k.nf, materialize); BLAUNCH("xinv");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -486,9 +463,7 @@ int prepare_lds(fhip_batch *b, BClass &k)
     // side buffer of the kx = 0 pencils: two NF-field values per point of up
     // to (TB - 1) / Mx + 1 pencils per tile, NF = 3 (dipole-dipole), 6
     // (strain) or 9 (acoustic)
-    const int tz = ztile_rows<T>(k.Lz);
-    const size_t per_field = 2 * (size_t)((tz - 1) / (k.Lx / 2) + 1) * k.Lz * sizeof(cpx<T>);
-    k.zshm_g = (k.acoustic ? 9 : 6) * per_field;
+    k.zshm_g = zside_bytes<T>(k.Lz, k.Lx / 2, k.acoustic ? 9 : 6);
     if (k.zshm_g > 96 * 1024)
         return bfail(b, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
     fhip_ctx *ctx = nullptr;

@@ -35,6 +35,14 @@ inline int line_tile_rows(int n) { return tile_rows(n, 16384 / (int)sizeof(cpx<T
 template <typename T>
 inline int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX); }
 
+// Dynamic LDS of a z pass (bytes): the side buffer of the kx = 0 pencils, two
+// nf-field values per point of up to (TB - 1) / Mx + 1 such pencils per tile.
+template <typename T>
+inline size_t zside_bytes(int Lz, int Mx, int nf)
+{
+    return 2 * (size_t)((ztile_rows<T>(Lz) - 1) / Mx + 1) * Lz * nf * sizeof(cpx<T>);
+}
+
 enum KernelId {
     K_XFWD = 0, K_YFWD, K_ZDD, K_ZSTRAIN, K_HOMO, K_YINV, K_XINV, K_FINISH, K_NPBEGIN, K_KICK,
     K_KE, K_SCALE, K_SETUP, K_AVG, K_ZPAIR, K_NKERNEL
@@ -138,11 +146,71 @@ inline int ensure_lds(fhip_ctx *ctx, const void *fn, size_t dyn)
     if (it != done.end() && it->second >= dyn) return FHIP_OK;
     hipFuncAttributes fa{};
     HIPCHK(hipFuncGetAttributes(&fa, fn));
+    // every launch passes NT threads per workgroup, the block size the
+    // kernel's index maps and LDS arrays are sized for (__launch_bounds__(NT))
+    if (fa.maxThreadsPerBlock > 0 && fa.maxThreadsPerBlock < NT)
+        return fail(ctx, FHIP_ERR_STATE, "kernel cannot run the launch block size");
     if (fa.sharedSizeBytes + dyn > 160 * 1024)
         return fail(ctx, FHIP_ERR_UNSUPPORTED, "kernel needs more LDS than a CU holds");
     if (fa.sharedSizeBytes + dyn > 64 * 1024)
         HIPCHK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
     done[fn] = dyn;
+    return FHIP_OK;
+}
+
+inline const char *const kKernelNames[K_NKERNEL] = {
+    "xfwd_drift_r2c", "y_forward", "z_dipole_dipole", "z_strain", "homo_strain", "y_inverse",
+    "xinv_c2r_local_kick", "finish_scalars", "np_begin", "np_kick", "kinetic", "apply_scale", "setup",
+    "average", "z_pair"};
+
+inline hipEvent_t take_event(fhip_ctx *ctx)
+{
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch helper: brackets the launch with events when measuring.
+template <typename F>
+int launch(fhip_ctx *ctx, int id, F &&fn, hipStream_t st = nullptr)
+{
+    if (!st) st = ctx->stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->prof) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (a) (void)hipEventRecord(a, st);
+    }
+    fn();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(ctx, FHIP_ERR_HIP, std::string("launch ") + kKernelNames[id] + ": " + hipGetErrorString(e));
+    if (ctx->prof && a && b) {
+        (void)hipEventRecord(b, st);
+        ctx->recs.push_back({id, a, b});
+    }
+    return FHIP_OK;
+}
+
+inline int collect_events(fhip_ctx *ctx)
+{
+    if (ctx->recs.empty()) return FHIP_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto &r : ctx->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ctx->kms[r.id] += ms;
+            ctx->kcount[r.id] += 1;
+        }
+        ctx->ev_pool.push_back(r.a);
+        ctx->ev_pool.push_back(r.b);
+    }
+    ctx->recs.clear();
     return FHIP_OK;
 }
 
@@ -186,3 +254,30 @@ inline ZArgs<T> zargs(fhip_ctx *ctx, bool want_acouK)
 }
 
 }  // namespace fhip_host
+
+// ---- per-edge stage launchers ----------------------------------------------
+// The kernels whose FFT length is a template parameter are launched through
+// these functions, one explicit instantiation per edge; fhip_stages.hip
+// defines them and is compiled once per group of edges, so the edges build in
+// parallel translation units.
+namespace fhip_stage {
+
+// single context (arguments from the context, launched on its stream)
+template <typename T, int n> int xfwd(fhip_ctx *ctx, const XfwdArgs &a);
+template <typename T, int n> int xinv(fhip_ctx *ctx, const XinvArgs &a, int nf);
+template <typename T, int n> int ypass(fhip_ctx *ctx, int sign, int slot0, int nf, hipStream_t s);
+template <typename T, int n> int zplain(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t s, size_t shm);
+template <typename T, int n, int G2> int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm);
+
+// replica batches: one launch over every replica of a size class
+template <typename T, int n> void b_xfwd(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int phase, int mode);
+template <typename T, int n> void b_ypass(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int sign);
+template <typename T, int n, int G2> void b_zpair(const Rep<T> *r, dim3 g, hipStream_t s, size_t shm, int want_acouK);
+template <typename T, int n, int G2> const void *b_zpair_fn();
+template <typename T, int n> void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize);
+
+}  // namespace fhip_stage
+
+// Edges of the batched instantiations.  Supercells above 64 per edge fill the
+// device alone and are stepped with the per-context calls.
+#define FHIP_BATCH_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64)

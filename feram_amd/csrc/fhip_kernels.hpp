@@ -859,6 +859,48 @@ struct ZTile {
     static constexpr int PPT = (EL + NT - 1) / NT;
 };
 
+// Thread-to-element maps of a z tile (TB pencils of LZ points, NTH threads),
+// host and device: the kernels index through them and the host-side bounds
+// check enumerates every (thread, iteration) pair through the same code.
+template <int NTH, int TB, int LZ>
+struct ZMap {
+    static constexpr int EL = LZ * TB;
+    static constexpr int PPT = (EL + NTH - 1) / NTH;
+    // iteration i of thread tid holds element tid + i*NTH of the tile
+    __host__ __device__ static constexpr bool live(int tid, int i) { return EL % NTH == 0 || tid + i * NTH < EL; }
+    // element e in HBM order (pencil b fastest, coalesced): kz index t
+    __host__ __device__ static void hbm_e(int e, int& t, int& b)
+    {
+        t = e / TB;
+        b = e - t * TB;
+    }
+    // element tid + i*NTH in HBM order: a per-thread base plus a uniform step in i
+    __host__ __device__ static void hbm(int tid, int i, int& t, int& b)
+    {
+        if constexpr (NTH % TB == 0) {
+            b = tid % TB;
+            t = tid / TB + i * (NTH / TB);
+        } else {
+            hbm_e(tid + i * NTH, t, b);
+        }
+    }
+    // element tid + i*NTH in line order (t fastest), for the k-space stage
+    __host__ __device__ static void line(int tid, int i, int& t, int& b)
+    {
+        if constexpr (NTH % LZ == 0) {
+            t = tid % LZ;
+            b = tid / LZ + i * (NTH / LZ);
+        } else if constexpr (LZ % NTH == 0) {
+            b = i / (LZ / NTH);
+            t = (i % (LZ / NTH)) * NTH + tid;
+        } else {
+            const int e = tid + i * NTH;
+            b = e / LZ;
+            t = e - b * LZ;
+        }
+    }
+};
+
 // Fields of a z group, fields transformed together (FB tiles side by side in
 // LDS, so that each Stockham pass has FB times the butterflies between two
 // barriers) and the LDS row stride that follows.
@@ -905,24 +947,19 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
     const size_t zstride = (size_t)nlines;
     for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
 
-    // element e of the tile in HBM order (b fastest, coalesced)
+    using Map = ZMap<NT, TB, LZ>;
+    // element e of the tile in HBM order: LDS slot and HBM offset
     auto hbm_at = [&](int e, int& lds, size_t& off) -> bool {
-        const int t = e / TB, b = e - t * TB, l = w * TB + b;
+        int t, b;
+        Map::hbm_e(e, t, b);
+        const int l = w * TB + b;
         lds = t * STB + b;
         off = (size_t)l + (size_t)t * zstride;
         return l < nlines;
     };
-    // element tid + i*NT in HBM order: a per-thread base plus a uniform step in i
     auto hbm_i = [&](int i, int& lds, size_t& off) -> bool {
         int t, b;
-        if constexpr (NT % TB == 0) {
-            b = tid % TB;
-            t = tid / TB + i * (NT / TB);
-        } else {
-            const int e = tid + i * NT;
-            t = e / TB;
-            b = e - t * TB;
-        }
+        Map::hbm(tid, i, t, b);
         const int l = w * TB + b;
         lds = t * STB + b;
         off = (size_t)l + (size_t)t * zstride;
@@ -980,19 +1017,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
         }
         // element tid + i*NT of the tile in line order (t fastest) for the
         // k-space stage
-        auto line_i = [&](int i, int& t, int& b) {
-            if constexpr (NT % LZ == 0) {
-                t = tid % LZ;
-                b = tid / LZ + i * (NT / LZ);
-            } else if constexpr (LZ % NT == 0) {
-                b = i / (LZ / NT);
-                t = (i % (LZ / NT)) * NT + tid;
-            } else {
-                const int e = tid + i * NT;
-                b = e / LZ;
-                t = e - b * LZ;
-            }
-        };
+        auto line_i = [&](int i, int& t, int& b) { Map::line(tid, i, t, b); };
 #pragma unroll
         for (int fb = 0; fb < NF; fb += FB) {
 #pragma unroll

@@ -1,0 +1,181 @@
+// fhip_stages.hip -- launchers of the kernels whose FFT length is a template
+// parameter, for one group of supercell edges.
+//
+// The file is compiled once per group (FHIP_STAGE_GROUP = 0 .. 8), each
+// object holding the device code of its edges only, so the thirteen edges
+// compile in parallel translation units; feram_hip.hip and fhip_batch.hip
+// dispatch on the edge and call the instantiation of that edge.
+#include "fhip_host.hpp"
+
+using namespace fhip;
+using namespace fhip_host;
+
+#ifndef FHIP_STAGE_GROUP
+#error "compile with -DFHIP_STAGE_GROUP=<group>"
+#endif
+
+#if FHIP_STAGE_GROUP == 0
+#define FHIP_GROUP_SIZES(X) X(4) X(6)
+#elif FHIP_STAGE_GROUP == 1
+#define FHIP_GROUP_SIZES(X) X(8) X(12)
+#elif FHIP_STAGE_GROUP == 2
+#define FHIP_GROUP_SIZES(X) X(16) X(24)
+#elif FHIP_STAGE_GROUP == 3
+#define FHIP_GROUP_SIZES(X) X(32) X(48)
+#elif FHIP_STAGE_GROUP == 4
+#define FHIP_GROUP_SIZES(X) X(64)
+#elif FHIP_STAGE_GROUP == 5
+#define FHIP_GROUP_SIZES(X) X(96)
+#elif FHIP_STAGE_GROUP == 6
+#define FHIP_GROUP_SIZES(X) X(128)
+#elif FHIP_STAGE_GROUP == 7
+#define FHIP_GROUP_SIZES(X) X(192)
+#elif FHIP_STAGE_GROUP == 8
+#define FHIP_GROUP_SIZES(X) X(256)
+#else
+#error "unknown FHIP_STAGE_GROUP"
+#endif
+
+namespace fhip_stage {
+
+template <typename T, int n>
+int xfwd(fhip_ctx *ctx, const XfwdArgs &a)
+{
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    Spectra<T> sp = tset<T>(ctx).sp;
+    const Scalars *sc = ctx->dsc;
+    const cpx<T> *tw = tset<T>(ctx).twx;
+    const int nb = ctx->xfblocks;
+    hipStream_t s = ctx->stream;
+    return launch(ctx, K_XFWD, [&] { k_xfwd<T, n><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+}
+
+template <typename T, int n>
+int xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
+{
+    Consts c = ctx->c;
+    Fields f = ctx->f;
+    Spectra<T> sp = tset<T>(ctx).sp;
+    const Scalars *sc = ctx->dsc;
+    const cpx<T> *tw = tset<T>(ctx).twx;
+    const int nb = ctx->xblocks;
+    hipStream_t s = ctx->stream;
+    if (nf == 12) return launch(ctx, K_XINV, [&] { k_xinv<T, n, 12><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+    if (nf == 9) return launch(ctx, K_XINV, [&] { k_xinv<T, n, 9><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+    return launch(ctx, K_XINV, [&] { k_xinv<T, n, 3><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
+}
+
+template <typename T, int n>
+int ypass(fhip_ctx *ctx, int sign, int slot0, int nf, hipStream_t s)
+{
+    Consts c = ctx->c;
+    Spectra<T> sp = tset<T>(ctx).sp;
+    const cpx<T> *tw = tset<T>(ctx).twy;
+    dim3 grid(ctx->yblocks[tix<T>()], nf);
+    const int id = sign < 0 ? K_YFWD : K_YINV;
+    if (sign < 0) return launch(ctx, id, [&] { k_ypass<T, n, -1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }, s);
+    return launch(ctx, id, [&] { k_ypass<T, n, +1><<<grid, NT, 0, s>>>(c, sp, slot0, tw); }, s);
+}
+
+template <typename T, int n>
+int zplain(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t s, size_t shm)
+{
+    Consts c = ctx->c;
+    Spectra<T> sp = tset<T>(ctx).sp;
+    Scalars *sc = ctx->dsc;
+    const cpx<T> *tw = tset<T>(ctx).twz;
+    const int nb = ctx->zblocks[tix<T>()];
+    const ZArgs<T> za = a;
+    RC(ensure_lds(ctx, (const void *)k_zpass<T, n, ZG_PLAIN>, shm));
+    return launch(ctx, K_SETUP, [&] { k_zpass<T, n, ZG_PLAIN><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); }, s);
+}
+
+template <typename T, int n, int G2>
+int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm)
+{
+    Consts c = ctx->c;
+    Spectra<T> sp = tset<T>(ctx).sp;
+    Scalars *sc = ctx->dsc;
+    const cpx<T> *tw = tset<T>(ctx).twz;
+    const int nb = ctx->zblocks[tix<T>()];
+    hipStream_t s = ctx->stream;
+    const ZArgs<T> za = a;
+    RC(ensure_lds(ctx, (const void *)k_zpair<T, n, G2>, shm));
+    return launch(ctx, K_ZPAIR, [&] { k_zpair<T, n, G2><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
+}
+
+// ---- replica batches -------------------------------------------------------
+template <typename T, int n>
+constexpr int ytile_full() { return tile_of<n, line_tile_cap<T>()>::value; }
+template <typename T, int n>
+constexpr int ytile_half() { return ytile_full<T, n>() > 1 ? ytile_full<T, n>() / 2 : 1; }
+
+template <typename T, int n>
+void b_xfwd(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int phase, int mode)
+{
+    if (half) kb_xfwd<T, n, xfwd_rows<n>::value><<<g, NT, 0, s>>>(r, phase, mode);
+    else kb_xfwd<T, n, tile_of<n>::value><<<g, NT, 0, s>>>(r, phase, mode);
+}
+
+template <typename T, int n>
+void b_ypass(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int sign)
+{
+    if (half) {
+        if (sign < 0) kb_ypass<T, n, -1, ytile_half<T, n>()><<<g, NT, 0, s>>>(r);
+        else kb_ypass<T, n, +1, ytile_half<T, n>()><<<g, NT, 0, s>>>(r);
+    } else {
+        if (sign < 0) kb_ypass<T, n, -1, ytile_full<T, n>()><<<g, NT, 0, s>>>(r);
+        else kb_ypass<T, n, +1, ytile_full<T, n>()><<<g, NT, 0, s>>>(r);
+    }
+}
+
+template <typename T, int n, int G2>
+void b_zpair(const Rep<T> *r, dim3 g, hipStream_t s, size_t shm, int want_acouK)
+{
+    kb_zpair<T, n, G2><<<g, NT, shm, s>>>(r, want_acouK);
+}
+
+template <typename T, int n, int G2>
+const void *b_zpair_fn()
+{
+    return (const void *)kb_zpair<T, n, G2>;
+}
+
+template <typename T, int n>
+void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize)
+{
+    if (nf == 12) kb_xinv<T, n, 12><<<g, NT, 0, s>>>(r, materialize);
+    else kb_xinv<T, n, 9><<<g, NT, 0, s>>>(r, materialize);
+}
+
+// ---- explicit instantiations of this group's edges ------------------------------
+#define FHIP_INST_T(T, n)                                                                   \
+    template int xfwd<T, n>(fhip_ctx *, const XfwdArgs &);                                  \
+    template int xinv<T, n>(fhip_ctx *, const XinvArgs &, int);                             \
+    template int ypass<T, n>(fhip_ctx *, int, int, int, hipStream_t);                       \
+    template int zplain<T, n>(fhip_ctx *, const ZArgs<T> &, hipStream_t, size_t);           \
+    template int zpair<T, n, ZG_STRAIN>(fhip_ctx *, const ZArgs<T> &, size_t);              \
+    template int zpair<T, n, ZG_ACOU>(fhip_ctx *, const ZArgs<T> &, size_t);
+#define FHIP_INST_B(T, n)                                                                   \
+    template void b_xfwd<T, n>(const Rep<T> *, dim3, hipStream_t, bool, int, int);          \
+    template void b_ypass<T, n>(const Rep<T> *, dim3, hipStream_t, bool, int);              \
+    template void b_zpair<T, n, ZG_STRAIN>(const Rep<T> *, dim3, hipStream_t, size_t, int); \
+    template void b_zpair<T, n, ZG_ACOU>(const Rep<T> *, dim3, hipStream_t, size_t, int);   \
+    template const void *b_zpair_fn<T, n, ZG_STRAIN>();                                     \
+    template const void *b_zpair_fn<T, n, ZG_ACOU>();                                       \
+    template void b_xinv<T, n>(const Rep<T> *, dim3, hipStream_t, int, int);
+
+#define FHIP_INST(n)         \
+    FHIP_INST_T(double, n)   \
+    FHIP_INST_T(float, n)
+FHIP_GROUP_SIZES(FHIP_INST)
+
+#if FHIP_STAGE_GROUP <= 4          // edges up to 64 are batched
+#define FHIP_INSTB(n)        \
+    FHIP_INST_B(double, n)   \
+    FHIP_INST_B(float, n)
+FHIP_GROUP_SIZES(FHIP_INSTB)
+#endif
+
+}  // namespace fhip_stage

@@ -145,3 +145,27 @@ def hysteresis_loop(systems, n_thermalize: int, n_average: int, n_hl_freq: int, 
         if i <= n_end:
             bt.step(n_end - i + 1)
     return rows
+
+
+def replica_ensemble(n: int = 48, seed: int = 20241203, edges=(12, 16, 24, 32)):
+    """The batched-ensemble workload: n BaTiO3 bulk NP-MD replicas with edges
+    drawn (seeded, uneven mix) from `edges`, T ~ U[150, 450] K and a static
+    E_z ~ U[0, 100] kV/cm (0 to 1e-3 V/A, param_module.F:15 units;
+    external_E_field = 0 0 E_z as initialize_E_field.F:58-66 applies it).
+    Returns the Params (Q_Nose 0.1 at 16^3, the heating example's, scaled
+    with N) and the initial u of each, uniform in [-0.05, 0.05] A."""
+    from .params import batio3_zvr
+    rng = np.random.default_rng(seed)
+    Ls = rng.choice(list(edges), size=n)
+    temps = rng.uniform(150.0, 450.0, size=n)
+    fields = rng.uniform(0.0, 1e-3, size=n)
+    params, u0 = [], []
+    for L, T, Ez in zip(Ls, temps, fields):
+        L = int(L)
+        p = batio3_zvr(L, method="md", kelvin=float(T))
+        p.Q_Nose = 0.1 * p.N / 16 ** 3
+        p.external_E_field = [0.0, 0.0, float(Ez)]
+        p.E_field_always_zero = False
+        params.append(p)
+        u0.append(rng.uniform(-0.05, 0.05, size=(L, L, L, 3)))
+    return params, u0

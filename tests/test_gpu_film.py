@@ -204,3 +204,36 @@ def test_film_fp32_trajectory(oracle_mod, geometry, gap, method):
         assert _nw(g.get(name), o.field(name)) <= FP32_TOL, name
     _check_energies(eg, e_o.as_dict(), FP32_TOL)
     g.close()
+
+
+@pytest.mark.parametrize("method,f", [("md", 0.004), ("vs", -0.006), ("md", 0.0)])
+def test_e110_trajectory(oracle_mod, method, f):
+    """'e110' homogeneous strain (optimize-homo-strain.F:42-56, the reference's
+    formula kept, test_oracle_e110 pins it): the device follows the oracle's
+    trajectory, strain and energies."""
+    from feram_amd.system import System
+    p = batio3_zvr((12, 8, 16), method=method, kelvin=250.0)
+    p.bulk_or_film = "e110"
+    p.epi_strain = f
+    p.GPa = [-4.0, -6.0, -2.0]
+    print(f"[gpu] e110 {method} epi_strain={f}", flush=True)
+    o = oracle_mod.Oracle(p)
+    g = System(p)
+    rng = np.random.default_rng(31)
+    u = 0.06 + 0.05 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3))
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    e_o = o.md_prologue()
+    eg = g.md_prologue()
+    _check_energies(eg, e_o.as_dict(), FORCE_TOL)
+    assert np.allclose(g.strain, o.strain, rtol=1e-12, atol=1e-16)
+    for chunk in (1, 9):
+        for _ in range(chunk):
+            o.step(e_o)
+        eg = g.step(chunk)
+        for name in ("dipoR", "dipoP", "dVddi"):
+            assert _nw(g.get(name), o.field(name)) <= TRAJ_TOL, name
+        _check_energies(eg, e_o.as_dict(), TRAJ_TOL)
+        assert np.allclose(g.strain, o.strain, rtol=TRAJ_TOL, atol=1e-14)
+        assert g.strain[2] == f and g.strain[3] == 0.0 and g.strain[4] == 0.0
+    g.close()
