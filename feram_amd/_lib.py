@@ -25,7 +25,8 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_set_modulation", "fhip_set_cells", "fhip_put", "fhip_get", "fhip_put_soa",
            "fhip_get_soa", "fhip_get_nose", "fhip_set_nose", "fhip_get_strain",
            "fhip_set_E_field_add", "fhip_set_kelvin", "fhip_set_heat_source", "fhip_get_layer_energies",
-           "fhip_set_E_schedule", "fhip_get_step", "fhip_force", "fhip_md_prologue", "fhip_nose_poincare",
+           "fhip_set_E_schedule", "fhip_get_step", "fhip_force", "fhip_force_static", "fhip_md_prologue",
+           "fhip_nose_poincare",
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_use_graph", "fhip_kernel_stats",
            "fhip_synchronize", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
            "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
@@ -123,6 +124,8 @@ def load():
     L.fhip_get_step.argtypes = [vp, C.POINTER(C.c_int64), dp]
     L.fhip_get_step.restype = ip
     L.fhip_force.argtypes = [vp, C.POINTER(FhipEnergy)]
+    L.fhip_force_static.argtypes = [C.POINTER(FhipParams), vp, vp, vp, vp, vp, vp, C.c_char_p, ip]
+    L.fhip_force_static.restype = ip
     L.fhip_md_prologue.argtypes = [vp, C.POINTER(FhipEnergy)]
     L.fhip_nose_poincare.argtypes = [vp, ip, C.POINTER(FhipEnergy)]
     L.fhip_leapfrog.argtypes = [vp, ip, ip, C.POINTER(FhipEnergy)]

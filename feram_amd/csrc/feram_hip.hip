@@ -638,6 +638,138 @@ int set_cells_impl(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, 
     return FHIP_OK;
 }
 
+// ---- the allocation-free force evaluation (fhip_force_static) ----------------
+// Every intermediate of the call lives in one fixed device array: the packed
+// spectra, the L(k) table in the library's k order, the FFT and k-vector
+// tables, the per-workgroup partial sums, the scalars and the cell codes.
+// Each device holds its own copy (loaded with the code object) until the
+// process ends; calls on one device share it, so overlapping calls race.
+constexpr size_t kStaticScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array: hipGetSymbolAddress once per
+// device, with that device current.  The array's name is a host-side stand-in
+// and is never passed to a kernel.
+int scratch_address(void **out, std::string &err)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return FHIP_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return FHIP_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return FHIP_OK;
+}
+
+// Carves 256-byte-aligned pieces out of the scratch array (or only counts
+// their size when base is null).
+struct Carve {
+    unsigned char *base;
+    size_t used = 0;
+    template <typename P>
+    P *take(size_t count)
+    {
+        used = (used + 255) & ~(size_t)255;
+        P *r = base ? reinterpret_cast<P *>(base + used) : nullptr;
+        used += count * sizeof(P);
+        return r;
+    }
+};
+
+struct StaticTables {
+    cpx<double> *twx, *twy, *twz, *csx, *csy, *csz;
+    double *kvx, *kvy, *kvz;
+    int Lx, Ly, Lz, Kx;
+    double a0;
+};
+
+// FFT twiddles e^{-2 pi i j / n}, cos/sin tables e^{+2 pi i j / n} and the k
+// components of each axis (the tables fhip_create uploads from the host).
+static __global__ __launch_bounds__(NT) void k_static_tables(StaticTables t)
+{
+    const int j = blockIdx.x * NT + threadIdx.x;
+    const double M2PI = 6.28318530717958647693;
+    auto put = [&](int n, cpx<double> *tw, cpx<double> *cs) {
+        if (j >= n) return;
+        double sn, cs_;
+        sincos(M2PI * (double)j / (double)n, &sn, &cs_);
+        tw[j] = cpx<double>{cs_, -sn};
+        cs[j] = cpx<double>{cs_, sn};
+    };
+    put(t.Lx, t.twx, t.csx);
+    put(t.Ly, t.twy, t.csy);
+    put(t.Lz, t.twz, t.csz);
+    if (j < t.Kx) t.kvx[j] = kcomp(j, t.Lx, t.a0, false);
+    if (j < t.Ly) t.kvy[j] = kcomp(j, t.Ly, t.a0, true);
+    if (j < t.Lz) t.kvz[j] = kcomp(j, t.Lz, t.a0, true);
+}
+
+// Cell codes (species 1, counted), the device scalars of a fresh context
+// (s = 1, E_add = the static external field), and L(k) from the caller's
+// c%long_range layout (3,3,0:Lx/2,0:Ly+padding_y-1,0:Lz-1) into the library's
+// k order, 6 upper-triangle words per point.
+static __global__ __launch_bounds__(NT) void k_static_init(Consts c, unsigned char *code, Scalars *sc,
+                                                           double Ex, double Ey, double Ez,
+                                                           const double *__restrict__ lr, int padding_y,
+                                                           double *__restrict__ Ltab)
+{
+    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (i < (size_t)c.N) code[i] = 8;
+    if (i == 0) {
+        Scalars z{};
+        z.s = 1.0;
+        z.i_step = 1.0;
+        z.scale_p = 1.0;
+        z.scale_pw = 1.0;
+        z.E_add[0] = Ex;
+        z.E_add[1] = Ey;
+        z.E_add[2] = Ez;
+        *sc = z;
+    }
+    const int Kx = c.Mx + 1;
+    if (i < (size_t)Kx * c.Ly * c.Lz) {
+        const int x = (int)(i % Kx);
+        const int y = (int)((i / Kx) % c.Ly);
+        const int z = (int)(i / ((size_t)Kx * c.Ly));
+        const size_t k = kindex(c.Mx, c.Ly, c.Lz, c.Mh, x, y, z);
+        const size_t base = 9 * (x + (size_t)Kx * (y + (size_t)(c.Ly + padding_y) * z));
+        const int ia[6] = {0, 1, 2, 1, 2, 0}, ib[6] = {0, 1, 2, 2, 0, 1};
+        for (int m = 0; m < 6; ++m) Ltab[k * 6 + m] = lr[base + ia[m] + 3 * ib[m]];
+    }
+}
+
+static __global__ void k_static_out(const Scalars *__restrict__ sc, double *energy, double *strain)
+{
+    const int t = threadIdx.x;
+    if (energy && t < 13) energy[t] = sc->en[t];
+    if (strain && t < 6) strain[t] = sc->strain[t];
+}
+
+int static_fail(char *err, int errlen, int code, const std::string &msg)
+{
+    if (err && errlen > 0) {
+        const size_t n = std::min((size_t)errlen - 1, msg.size());
+        std::memcpy(err, msg.data(), n);
+        err[n] = 0;
+    }
+    return code;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -1050,6 +1182,101 @@ int fhip_force(fhip_ctx *ctx, fhip_energy *e)
     RC(force_pipeline(ctx, SK_FORCE, true));
     RC(pull_scalars(ctx));
     energies_out(ctx, e);
+    return FHIP_OK;
+}
+
+int fhip_force_static(const fhip_params *p, const double *long_range, const double *u, double *dVddi,
+                      fhip_energy *energy, double *strain, void *stream, char *err, int errlen)
+{
+    if (!p || !long_range || !u || !dVddi || !energy)
+        return static_fail(err, errlen, FHIP_ERR_ARG, "null argument");
+    // a context record on the stack: its device pointers all point into the
+    // scratch array, and nothing on it allocates device memory
+    fhip_ctx ctx;
+    int rc = check_params(&ctx, p);
+    if (rc != FHIP_OK) return static_fail(err, errlen, rc, ctx.err);
+    if (p->precision != FHIP_FP64 || p->mass_acou[0] > 0.0 || p->modulation_constant != 0.0 ||
+        (p->bulk_or_film != FHIP_BULK && p->bulk_or_film != FHIP_E110))
+        return static_fail(err, errlen, FHIP_ERR_UNSUPPORTED,
+                           "fhip_force_static: FP64, bulk or e110, optimised acoustic strain, no modulation only");
+    ctx.p = *p;
+    ctx.Lx = p->Lx; ctx.Ly = p->Ly; ctx.Lz = p->Lz;
+    ctx.Kx = p->Lx / 2 + 1;
+    ctx.Mx = p->Lx / 2;
+    ctx.Mh = (size_t)ctx.Mx * p->Ly * p->Lz;
+    ctx.N = p->Lx * p->Ly * p->Lz;
+    ctx.Kh = ctx.Kx * p->Ly * p->Lz;
+    ctx.Ks = ctx.Mh;
+    ctx.stream = (hipStream_t)stream;
+    fill_consts(&ctx);
+    ctx.nslots = 9;
+    ctx.xblocks = (ctx.Ly * ctx.Lz + tile_rows(ctx.Lx) - 1) / tile_rows(ctx.Lx);
+    ctx.xfblocks = (ctx.Ly * ctx.Lz + xfwd_tile_rows(ctx.Lx) - 1) / xfwd_tile_rows(ctx.Lx);
+    ctx.yblocks[0] = (ctx.Mx * ctx.Lz + line_tile_rows<double>(ctx.Ly) - 1) / line_tile_rows<double>(ctx.Ly);
+    ctx.zblocks[0] = (ctx.Mx * ctx.Ly + ztile_rows<double>(ctx.Lz) - 1) / ztile_rows<double>(ctx.Lz);
+    ctx.kblocks = (ctx.N + NT - 1) / NT;
+    ctx.maxblocks = std::max(std::max(ctx.xblocks, ctx.zblocks[0]), ctx.kblocks);
+    ctx.precision = FHIP_FP64;
+
+    // the pieces of the scratch array, counted first
+    StaticTables t{};
+    cpx<double> *spec = nullptr;
+    auto carve = [&](Carve &cv) {
+        spec = cv.take<cpx<double>>(9 * ctx.Ks);
+        ctx.Ltab = cv.take<double>(6 * (size_t)ctx.Kh);
+        ctx.partials = cv.take<double>((size_t)P_NSLOT * ctx.maxblocks);
+        ctx.dsc = cv.take<Scalars>(1);
+        ctx.code = cv.take<unsigned char>(ctx.N);
+        t.twx = cv.take<cpx<double>>(ctx.Lx);
+        t.twy = cv.take<cpx<double>>(ctx.Ly);
+        t.twz = cv.take<cpx<double>>(ctx.Lz);
+        t.csx = cv.take<cpx<double>>(ctx.Lx);
+        t.csy = cv.take<cpx<double>>(ctx.Ly);
+        t.csz = cv.take<cpx<double>>(ctx.Lz);
+        t.kvx = cv.take<double>(ctx.Kx);
+        t.kvy = cv.take<double>(ctx.Ly);
+        t.kvz = cv.take<double>(ctx.Lz);
+    };
+    Carve count{nullptr};
+    carve(count);
+    if (count.used > kStaticScratchBytes)
+        return static_fail(err, errlen, FHIP_ERR_UNSUPPORTED,
+                           "fhip_force_static: the supercell needs " + std::to_string(count.used) +
+                               " bytes of scratch, more than the " + std::to_string(kStaticScratchBytes) +
+                               " the library holds per device (N = Lx*Ly*Lz up to about 110592)");
+    void *base = nullptr;
+    std::string msg;
+    rc = scratch_address(&base, msg);
+    if (rc != FHIP_OK) return static_fail(err, errlen, rc, msg);
+    Carve cv{(unsigned char *)base};
+    carve(cv);
+    for (int q = 0; q < MAX_SLOTS; ++q) ctx.dset.sp.s[q] = q < 9 ? spec + (size_t)q * ctx.Ks : nullptr;
+    ctx.dset.Lt = ctx.Ltab;
+    ctx.dset.twx = t.twx; ctx.dset.twy = t.twy; ctx.dset.twz = t.twz;
+    ctx.csx = t.csx; ctx.csy = t.csy; ctx.csz = t.csz;
+    ctx.kvx = t.kvx; ctx.kvy = t.kvy; ctx.kvz = t.kvz;
+    ctx.f.code = ctx.code;
+    const size_t N = ctx.N;
+    for (int q = 0; q < 3; ++q) {
+        ctx.f.u[q] = const_cast<double *>(u) + q * N;
+        ctx.f.F[q] = dVddi + q * N;
+    }
+    t.Lx = ctx.Lx; t.Ly = ctx.Ly; t.Lz = ctx.Lz; t.Kx = ctx.Kx; t.a0 = p->a0;
+    hipStream_t s = ctx.stream;
+    const int lmax = std::max(std::max(ctx.Lx, ctx.Ly), ctx.Lz);
+    k_static_tables<<<(lmax + NT - 1) / NT, NT, 0, s>>>(t);
+    const size_t ninit = std::max(N, (size_t)ctx.Kh);
+    const double *E = p->E_field_always_zero ? nullptr : p->external_E_field;
+    k_static_init<<<(unsigned)((ninit + NT - 1) / NT), NT, 0, s>>>(ctx.c, ctx.code, ctx.dsc, E ? E[0] : 0.0,
+                                                                 E ? E[1] : 0.0, E ? E[2] : 0.0, long_range,
+                                                                 p->padding_y, ctx.Ltab);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return static_fail(err, errlen, FHIP_ERR_HIP, std::string("set-up launch: ") + hipGetErrorString(e));
+    rc = force_pipeline(&ctx, SK_FORCE, true);
+    if (rc != FHIP_OK) return static_fail(err, errlen, rc, ctx.err);
+    k_static_out<<<1, 64, 0, s>>>(ctx.dsc, (double *)energy, strain);
+    e = hipGetLastError();
+    if (e != hipSuccess) return static_fail(err, errlen, FHIP_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
     return FHIP_OK;
 }
 

@@ -172,6 +172,32 @@ int fhip_get_layer_energies(fhip_ctx *ctx, double *e_dipo, double *e_acou, int a
 int fhip_set_kelvin(fhip_ctx *ctx, double kelvin);
 
 int fhip_force(fhip_ctx *ctx, fhip_energy *e);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the force evaluation of fhip_force (mirror_force_mirror)
+ * without a context, on caller-owned device arrays.  It calls no hipMalloc,
+ * hipFree or synchronisation: it only enqueues kernels on `stream` (a
+ * hipStream_t; NULL is the default stream).  Every intermediate (spectra,
+ * tables, partial sums, scalars) lives in a fixed 16 MiB device array of the
+ * library, one per device, held until the process ends; the supercell must
+ * fit it (9 FP64 spectra, the L(k) table and the cell codes: N = Lx Ly Lz up
+ * to about 110592, e.g. 48^3), larger ones are rejected with
+ * FHIP_ERR_UNSUPPORTED.  Calls that overlap on one device race on that array.
+ *   p          FP64, bulk or e110, acoustic displacement optimised
+ *              (mass_acou <= 0), no modulation; every cell species 1, counted;
+ *   long_range device array, c%long_range in the layout of fhip_set_long_range
+ *              (fhip_get_long_range of a context with the same parameters);
+ *   u          device array u[a*N + x + Lx*(y + Ly*z)] (fhip_put_soa's layout);
+ *   dVddi      device array, same layout, receives dV/du;
+ *   energy     device pointer, receives the Energy_type of the evaluation
+ *              (kinetic terms 0, as for a fresh context);
+ *   strain     device double[6] or NULL, receives c%strain;
+ *   err        host buffer of errlen bytes (or NULL) for the message of a
+ *              failed call.
+ * The first call on a device resolves the array's address there; make it
+ * before capturing the call into a graph. */
+int fhip_force_static(const fhip_params *p, const double *long_range, const double *u, double *dVddi,
+                      fhip_energy *energy, double *strain, void *stream, char *err, int errlen);
 int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e);
 /* nsteps consecutive steps (>= 1); e receives the energies after the last one. */
 int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e);
