@@ -116,31 +116,17 @@ def errors_vs(first, ref) -> dict:
             "p_rel_err_step2": rel(first["p2"], ref["p2"])}
 
 
-def copy_rate(sync_extra) -> float | None:
-    """GB/s of a 1 GiB device-to-device copy (read + write bytes over the
-    copy time), timed with events on torch's stream, the stream it runs on."""
-    try:
-        import torch
-        if not torch.cuda.is_available():
-            return None
-    except Exception:
-        return None
-    n = (1 << 30) // 8
-    x = torch.ones(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 10
-    a.record()
-    for _ in range(reps):
-        y.copy_(x)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / reps
-    del x, y
-    torch.cuda.empty_cache()
-    return 2 * n * 8 / (ms * 1e-3) / 1e9
+def copy_rate(nbytes: int = 1 << 30, reps: int = 10):
+    """GB/s (read + write bytes over the time per copy) of the library's plain
+    16-byte-word copy kernel between two device buffers (fhip_copy_rate)."""
+    import ctypes as C
+    from feram_amd import _lib
+    L = _lib.load()
+    gbs, us = C.c_double(), C.c_double()
+    rc = L.fhip_copy_rate(int(nbytes), int(reps), C.byref(gbs), C.byref(us))
+    if rc != 0:
+        raise RuntimeError(f"fhip_copy_rate: status {rc}")
+    return gbs.value, us.value
 
 
 def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
@@ -475,18 +461,14 @@ def main():
     ap.add_argument("--scan-q", type=float, default=1.0,
                     help="Q_Nose of the warm-start scan (the reference's 64^3 md benchmark uses 1.0)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (5 launches) to calibrate "
-                         "the FETCH_SIZE / WRITE_SIZE counters, then exit")
+                    help="only run the library's plain device copy of MIB MiB (5 timed launches after 2 "
+                         "of warm-up) to calibrate the FETCH_SIZE / WRITE_SIZE counters, then exit")
     args = ap.parse_args()
     if args.copy_calibration > 0:
-        import torch
-        n = args.copy_calibration * (1 << 20) // 8
-        x = torch.ones(n, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch_read": n * 8, "copy_bytes_per_launch_write": n * 8}))
+        nbytes = args.copy_calibration * (1 << 20)
+        gbs, us = copy_rate(nbytes, 5)
+        print(json.dumps({"copy_bytes_per_launch_read": nbytes, "copy_bytes_per_launch_write": nbytes,
+                          "kernel": "k_copy16", "GBs": gbs, "us": us}))
         return
     if args.gpus != 1:
         print(f"bench.py: the step is measured on one GPU; --gpus {args.gpus} is not run "
@@ -494,15 +476,8 @@ def main():
               "with one all-to-all per transform direction)", file=sys.stderr)
         sys.exit(2)
 
-    try:
-        import torch
-        have_torch = torch.cuda.is_available()
-    except Exception:
-        have_torch = False
-
     def sync_extra():
-        if have_torch:
-            torch.cuda.synchronize()
+        pass
 
     if args.workload == "replicas":
         bench_replicas(args, sync_extra)
@@ -558,7 +533,7 @@ def main():
     if os.path.exists(tr_path):
         with open(tr_path) as f:
             traffic = json.load(f).get(f"{L}:{head_name}:{dom}")
-    copy_gbs = copy_rate(sync_extra)
+    copy_gbs, _ = copy_rate()
 
     cpu = None
     if not args.no_cpu:

@@ -760,6 +760,26 @@ static __global__ void k_static_out(const Scalars *__restrict__ sc, double *ener
     if (strain && t < 6) strain[t] = sc->strain[t];
 }
 
+// ---- measurement: a plain device copy ---------------------------------------
+// 16-byte words from src to dst, 4 per thread, each load and store
+// instruction of a workgroup covering NT consecutive words.
+static __global__ __launch_bounds__(NT) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                      size_t n)
+{
+    const size_t base = (size_t)blockIdx.x * 4 * NT + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = base + (size_t)k * NT;
+        if (i < n) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = base + (size_t)k * NT;
+        if (i < n) dst[i] = v[k];
+    }
+}
+
 int static_fail(char *err, int errlen, int code, const std::string &msg)
 {
     if (err && errlen > 0) {
@@ -1278,6 +1298,44 @@ int fhip_force_static(const fhip_params *p, const double *long_range, const doub
     e = hipGetLastError();
     if (e != hipSuccess) return static_fail(err, errlen, FHIP_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
     return FHIP_OK;
+}
+
+int fhip_copy_rate(int64_t bytes, int reps, double *gbs, double *us_per_copy)
+{
+    if (bytes < 16 || reps < 1 || !gbs) return FHIP_ERR_ARG;
+    const size_t n = (size_t)bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = FHIP_ERR_HIP;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+        rc = FHIP_ERR_ALLOC;
+    } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+               hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+               hipMemsetAsync(a, 0x3c, n * 16, st) == hipSuccess) {
+        const unsigned blocks = (unsigned)((n + 4 * (size_t)NT - 1) / (4 * (size_t)NT));
+        bool ok = true;
+        for (int i = 0; i < 2 + reps && ok; ++i) {
+            if (i == 2) ok = hipEventRecord(e0, st) == hipSuccess;
+            k_copy16<<<blocks, NT, 0, st>>>((const uint4 *)a, (uint4 *)b, n);
+            ok = ok && hipGetLastError() == hipSuccess;
+        }
+        float ms = 0.0f;
+        if (ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.0f) {
+            const double t = 1e-3 * ms / reps;
+            *gbs = 2.0 * 16.0 * (double)n / t / 1e9;
+            if (us_per_copy) *us_per_copy = 1e6 * t;
+            rc = FHIP_OK;
+        }
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e)

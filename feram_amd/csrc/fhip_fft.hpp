@@ -131,15 +131,20 @@ __device__ __forceinline__ void dft(cpx<T>* v)
 // passes, fewer trips through LDS), else 4, 2, 3, 5.
 template <int N, int TB = 1, int NT = 1>
 struct radix_of {
-    template <int NS>
-    static constexpr int at()
+    // radix of a pass that leaves the factor REM of the length to transform
+    template <int REM>
+    static constexpr int of_rem()
     {
-        constexpr int rem = N / NS;
-        if constexpr (rem % 16 == 0 && (N / 16) * TB >= NT) return 16;
-        if constexpr (rem % 8 == 0 && (N / 8) * TB >= NT) return 8;
-        return (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : (rem % 5 == 0) ? 5 : 0;
+        if constexpr (REM % 16 == 0 && (N / 16) * TB >= NT) return 16;
+        if constexpr (REM % 8 == 0 && (N / 8) * TB >= NT) return 8;
+        return (REM % 4 == 0) ? 4 : (REM % 2 == 0) ? 2 : (REM % 3 == 0) ? 3 : (REM % 5 == 0) ? 5 : 0;
     }
+    template <int NS>
+    static constexpr int at() { return of_rem<N / NS>(); }
 };
+
+// Radices with an in-register DFT.
+constexpr bool dft_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16; }
 
 template <typename T, int N, int TB, int NT, int R, int NS, int SIGN, int ST>
 __device__ __forceinline__ void stockham_pass(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
@@ -194,6 +199,20 @@ __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* __restrict__ 
         static_assert(R > 1, "FFT length must factor into 2, 3 and 5");
         stockham_pass<T, N, TB, NT, R, NS, SIGN, ST>(buf, tw, tid);
         fft_lds<T, N, TB, NT, SIGN, ST, NS * R>(buf, tw, tid);
+    }
+}
+
+// The Stockham passes of a transform from stride NS up to (not including)
+// stride NSEND: the middle passes of a transform whose first and last passes
+// run in registers (see zpass_body).
+template <typename T, int N, int TB, int NT, int SIGN, int ST, int NS, int NSEND>
+__device__ __forceinline__ void fft_lds_span(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid)
+{
+    if constexpr (NS < NSEND) {
+        constexpr int R = radix_of<N, TB, NT>::template of_rem<NSEND / NS>();
+        static_assert(R > 1 && (NSEND / NS) % R == 0, "span must factor into 2, 3 and 5");
+        stockham_pass<T, N, TB, NT, R, NS, SIGN, ST>(buf, tw, tid);
+        fft_lds_span<T, N, TB, NT, SIGN, ST, NS * R, NSEND>(buf, tw, tid);
     }
 }
 

@@ -1015,49 +1015,109 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
                 }
             }
         }
-        // element tid + i*NT of the tile in line order (t fastest) for the
-        // k-space stage
-        auto line_i = [&](int i, int& t, int& b) { Map::line(tid, i, t, b); };
+        // The first and the last pass of every z transform run in registers
+        // where the tile allows: a thread's PPT points of its pencil b,
+        // t0 + i S (S = NT / TB), are the inputs of one radix-PPT butterfly of
+        // the first Stockham pass (stride 1) and the outputs of one of the
+        // last (stride LZ / PPT), so the tile goes from HBM to the k-space
+        // stage and back through LDS only for the middle passes; v[.][i] then
+        // holds the point (t0 + i S, b).  Otherwise the whole transform runs
+        // in LDS and the k-space stage reads the tile in line order.
+        constexpr int S = NT / TB;
+        constexpr bool REG = NT % TB == 0 && EL % NT == 0 && dft_radix(PPT) && LZ % (PPT * PPT) == 0;
+        const int zt0 = tid / TB, zb = tid % TB;
+        // the point (t, b) that iteration i of this thread holds after the forward transforms
+        auto pt_i = [&](int i, int& t, int& b) {
+            if constexpr (REG) Map::hbm(tid, i, t, b);
+            else Map::line(tid, i, t, b);
+        };
+        if constexpr (REG) {
 #pragma unroll
-        for (int fb = 0; fb < NF; fb += FB) {
+            for (int fb = 0; fb < NF; fb += FB) {
 #pragma unroll
-            for (int f = 0; f < FB; ++f) {
+                for (int f = 0; f < FB; ++f) {
+                    cpx<T> wv[PPT];
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    const int e = tid + i * NT;
-                    if (EL % NT == 0 || e < EL) {
-                        int lds; size_t off;
-                        hbm_i(i, lds, off);
-                        buf[lds + f * TB] = v[fb + f][i];
+                    for (int q = 0; q < PPT; ++q) wv[q] = v[fb + f][q];
+                    dft<PPT, -1>(wv);
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) buf[(PPT * zt0 + q) * STB + f * TB + zb] = wv[q];
+                }
+                __syncthreads();
+                fft_lds_span<T, LZ, FB * TB, NT, -1, STB, PPT, LZ / PPT>(buf, tws, tid);
+#pragma unroll
+                for (int f = 0; f < FB; ++f) {
+                    cpx<T> wv[PPT];
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) wv[q] = buf[(zt0 + q * S) * STB + f * TB + zb];
+#pragma unroll
+                    for (int q = 1; q < PPT; ++q) wv[q] = cmul(wv[q], tws[zt0 * q]);
+                    dft<PPT, -1>(wv);
+                    const int ord = lord[zb];
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
+                        v[fb + f][q] = wv[q];
+                        if (ord >= 0) side[(ord * LZ + zt0 + q * S) * NF + fb + f] = wv[q];
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
-            fft_lds<T, LZ, FB * TB, NT, -1, STB>(buf, tws, tid);
+            // kx = 0 pencils: R pencils (ky = 0, Ly/2) carry C = P + iQ; split
+            // into P(t), t < Lz/2, and Q(Lz - t), pairwise in the side buffer
+            constexpr int npair = (LZ - 1) / 2;
+            const int npen = nsp / LZ;
+            for (int e = tid; e < npen * npair * NF; e += NT) {
+                const int fi = e % NF, rest = e / NF, t = 1 + rest % npair, ord = rest / npair;
+                const int ky = lky[b0 + ord * c.Mx];
+                if (!(ky == 0 || 2 * ky == c.Ly)) continue;
+                cpx<T>* lo = side + (ord * LZ + t) * NF + fi;
+                cpx<T>* hi = side + (ord * LZ + LZ - t) * NF + fi;
+                const cpx<T> x = *lo, m = *hi;
+                *lo = split_pq(x, m, true);
+                *hi = split_pq(m, x, false);
+            }
+        } else {
 #pragma unroll
-            for (int f = 0; f < FB; ++f) {
-                const int fi = fb + f;
+            for (int fb = 0; fb < NF; fb += FB) {
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    const int e = tid + i * NT;
-                    if (EL % NT == 0 || e < EL) {
-                        int t, b;
-                        line_i(i, t, b);
-                        const cpx<T> x = buf[t * STB + f * TB + b];
-                        const int ord = lord[b];
-                        if (ord < 0) {
-                            v[fi][i] = x;
-                        } else {
-                            // kx = 0 pencil: R pencils (ky = 0, Ly/2) are split into P and Q here
-                            const int ky = lky[b];
-                            const bool rp = (ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ;
-                            side[(ord * LZ + t) * NF + fi] =
-                                rp ? split_pq(x, buf[(LZ - t) * STB + f * TB + b], 2 * t < LZ) : x;
+                for (int f = 0; f < FB; ++f) {
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        const int e = tid + i * NT;
+                        if (EL % NT == 0 || e < EL) {
+                            int lds; size_t off;
+                            hbm_i(i, lds, off);
+                            buf[lds + f * TB] = v[fb + f][i];
                         }
                     }
                 }
+                __syncthreads();
+                fft_lds<T, LZ, FB * TB, NT, -1, STB>(buf, tws, tid);
+#pragma unroll
+                for (int f = 0; f < FB; ++f) {
+                    const int fi = fb + f;
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        const int e = tid + i * NT;
+                        if (EL % NT == 0 || e < EL) {
+                            int t, b;
+                            pt_i(i, t, b);
+                            const cpx<T> x = buf[t * STB + f * TB + b];
+                            const int ord = lord[b];
+                            if (ord < 0) {
+                                v[fi][i] = x;
+                            } else {
+                                // kx = 0 pencil: R pencils (ky = 0, Ly/2) are split into P and Q here
+                                const int ky = lky[b];
+                                const bool rp = (ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ;
+                                side[(ord * LZ + t) * NF + fi] =
+                                    rp ? split_pq(x, buf[(LZ - t) * STB + f * TB + b], 2 * t < LZ) : x;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
 
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
@@ -1067,7 +1127,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             const int e = tid + i * NT;
             if (!(EL % NT == 0 || e < EL)) continue;
             int t, b;
-            line_i(i, t, b);
+            pt_i(i, t, b);
             const int kx = lkx[b];
             if (kx <= 0) continue;
             const int ky = lky[b];
@@ -1120,46 +1180,84 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             }
         }
         __syncthreads();
-        // inverse transforms
+        // inverse transforms; kx = 0 pencils take their values from the side
+        // buffer, R pencils joining P(t) (t < Lz/2) and Q(t) (held at Lz - t) back
+        auto inv_in = [&](int fi, int i, int t, int b) -> cpx<T> {
+            const int ord = lord[b];
+            if (ord < 0) return v[fi][i];
+            const int ky = lky[b];
+            cpx<T> x = side[(ord * LZ + t) * NF + fi];
+            if ((ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ)
+                x = join_pq(x, side[(ord * LZ + LZ - t) * NF + fi], 2 * t < LZ);
+            return x;
+        };
+        if constexpr (REG) {
 #pragma unroll
-        for (int fb = 0; fb < NF; fb += FB) {
+            for (int fb = 0; fb < NF; fb += FB) {
 #pragma unroll
-            for (int f = 0; f < FB; ++f) {
-                const int fi = fb + f;
+                for (int f = 0; f < FB; ++f) {
+                    cpx<T> wv[PPT];
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    const int e = tid + i * NT;
-                    if (EL % NT == 0 || e < EL) {
-                        int t, b;
-                        line_i(i, t, b);
-                        const int ord = lord[b];
-                        cpx<T> x = v[fi][i];
-                        if (ord >= 0) {
-                            // R pencils: join P(t) (t < Lz/2) and Q(t) (held at Lz - t) back
-                            const int ky = lky[b];
-                            x = side[(ord * LZ + t) * NF + fi];
-                            if ((ky == 0 || 2 * ky == c.Ly) && t != 0 && 2 * t != LZ)
-                                x = join_pq(x, side[(ord * LZ + LZ - t) * NF + fi], 2 * t < LZ);
-                        }
-                        buf[t * STB + f * TB + b] = x;
-                    }
+                    for (int q = 0; q < PPT; ++q) wv[q] = inv_in(fb + f, q, zt0 + q * S, zb);
+                    dft<PPT, +1>(wv);
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) buf[(PPT * zt0 + q) * STB + f * TB + zb] = wv[q];
                 }
-            }
-            __syncthreads();
-            fft_lds<T, LZ, FB * TB, NT, +1, STB>(buf, tws, tid);
+                __syncthreads();
+                fft_lds_span<T, LZ, FB * TB, NT, +1, STB, PPT, LZ / PPT>(buf, tws, tid);
 #pragma unroll
-            for (int f = 0; f < FB; ++f) {
-                cpx<T>* __restrict__ dst = sp.s[F0 + fb + f];
+                for (int f = 0; f < FB; ++f) {
+                    cpx<T>* __restrict__ dst = sp.s[F0 + fb + f];
+                    cpx<T> wv[PPT];
 #pragma unroll
-                for (int i = 0; i < PPT; ++i) {
-                    const int e = tid + i * NT;
-                    if (EL % NT == 0 || e < EL) {
+                    for (int q = 0; q < PPT; ++q) wv[q] = buf[(zt0 + q * S) * STB + f * TB + zb];
+#pragma unroll
+                    for (int q = 1; q < PPT; ++q) {
+                        cpx<T> tw = tws[zt0 * q];
+                        tw.y = -tw.y;
+                        wv[q] = cmul(wv[q], tw);
+                    }
+                    dft<PPT, +1>(wv);
+#pragma unroll
+                    for (int q = 0; q < PPT; ++q) {
                         int lds; size_t off;
-                        if (hbm_i(i, lds, off)) dst[off] = buf[lds + f * TB];
+                        if (hbm_i(q, lds, off)) dst[off] = wv[q];
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
+        } else {
+#pragma unroll
+            for (int fb = 0; fb < NF; fb += FB) {
+#pragma unroll
+                for (int f = 0; f < FB; ++f) {
+                    const int fi = fb + f;
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        const int e = tid + i * NT;
+                        if (EL % NT == 0 || e < EL) {
+                            int t, b;
+                            pt_i(i, t, b);
+                            buf[t * STB + f * TB + b] = inv_in(fi, i, t, b);
+                        }
+                    }
+                }
+                __syncthreads();
+                fft_lds<T, LZ, FB * TB, NT, +1, STB>(buf, tws, tid);
+#pragma unroll
+                for (int f = 0; f < FB; ++f) {
+                    cpx<T>* __restrict__ dst = sp.s[F0 + fb + f];
+#pragma unroll
+                    for (int i = 0; i < PPT; ++i) {
+                        const int e = tid + i * NT;
+                        if (EL % NT == 0 || e < EL) {
+                            int lds; size_t off;
+                            if (hbm_i(i, lds, off)) dst[off] = buf[lds + f * TB];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
         }
         const int nb = a.maxblocks;
         if constexpr (GROUP == ZG_DD) {
@@ -1488,12 +1586,278 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
     }
 }
 
+// xinv of a step (NF = 9 inverse fields, 12 with acoustic MD): the field
+// pairs are transformed PB at a time side by side in LDS (PB * RB lines, so a
+// 128-point transform takes radix-8 passes with every thread busy) and each
+// transformed value is folded into the cell's force as it comes out of LDS:
+// only the force, the cell's state and one batch of spectra are live in
+// registers (FP64 128^3: 3 waves per SIMD where holding every field's values
+// takes 2).  The force is the sum of the same terms as dVddi-dipole-dipole.F,
+// dVddi_inho_coupling.F, dVddi-unharmonic.F, mirror_force_mirror.F:39-65 and
+// dVddi-homo-coupling.F, the inhomogeneous coupling added field by field.
+template <typename T, int LX, int NF>
+__device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f, const Spectra<T>& sp,
+                                               const XinvArgs& a, const Scalars* __restrict__ sc,
+                                               const cpx<T>* twx, int bx)
+{
+    static_assert(NF == 9 || NF == 12, "step fields");
+    constexpr int RB = tile_of<LX>::value;          // rows per workgroup
+    constexpr int PB = 2;                           // field pairs per LDS transform
+    constexpr int NL = PB * RB;                     // lines of a full batch
+    constexpr int ST = NL + FHIP_XPAD;
+    constexpr int CELLS = RB * LX;
+    constexpr int CPT = (CELLS + NT - 1) / NT;
+    constexpr int MX = LX / 2;
+    constexpr int HP = RB * MX;
+    constexpr int HPT = (HP + NT - 1) / NT;
+    constexpr int NPAIR = (NF + 1) / 2;
+    constexpr int NBATCH = (NPAIR + PB - 1) / PB;
+    __shared__ cpx<T> buf[LX * ST];
+    __shared__ cpx<T> tws[LX];
+    __shared__ double red[NT];
+    const int tid = threadIdx.x;
+    const int nrows = c.Ly * c.Lz;
+    const int row0 = bx * RB;
+    for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
+    const bool kick = a.mode == XI_NP || a.mode == XI_LF || a.mode == XI_VS;
+
+    // the cells' state: u and the codes now (the folds need u), the momenta
+    // the kick reads before the last transform
+    bool live[CPT];
+    size_t cell[CPT];
+    double u[CPT][3], F[CPT][3], pp[CPT][3];
+    unsigned char code[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        const int e = tid + i * NT;
+        const int r = e / LX, x = e - r * LX;
+        live[i] = (CELLS % NT == 0 || e < CELLS) && row0 + r < nrows;
+        cell[i] = (size_t)(row0 + r) * LX + x;
+        code[i] = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) u[i][q] = F[i][q] = pp[i][q] = 0.0;
+        if (live[i]) {
+            code[i] = f.code[cell[i]];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) u[i][q] = f.u[q][cell[i]];
+        }
+    }
+    double g[NF == 12 ? CPT : 1][3];                // acoustic dV/dw (dVdac.F:56-62)
+
+    cpx<T> h[PB][2][HPT];
+    auto load_batch = [&](int b) {
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int fi = 2 * (b * PB + p) + k;
+#pragma unroll
+                for (int i = 0; i < HPT; ++i) {
+                    const int e = tid + i * NT;
+                    h[p][k][i] = cpx<T>{T(0), T(0)};
+                    if (fi >= NF || (HP % NT != 0 && e >= HP)) continue;
+                    if (row0 + e / MX < nrows) h[p][k][i] = sp.s[fi][(size_t)row0 * MX + e];
+                }
+            }
+        }
+    };
+    // value v of inverse field fi at cell i, normalised by 1/N
+    auto fold = [&](int i, int fi, double v) {
+        v = v * c.Ninv;
+        if (fi < 3) {
+            F[i][fi] = F[i][fi] + v;                                        // dVddi-dipole-dipole.F:34-38
+        } else if (fi < 9) {
+            // eta_jj(R) / N through homogeneous_coupling_2 (dVddi_inho_coupling.F:54-82)
+            const int jj = fi - 3;
+            double e[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) e[m] = v * c.hc2[jj + 6 * m];
+            F[i][0] = F[i][0] + (2 * e[0] * u[i][0] + e[4] * u[i][2] + e[5] * u[i][1]);
+            F[i][1] = F[i][1] + (2 * e[1] * u[i][1] + e[3] * u[i][2] + e[5] * u[i][0]);
+            F[i][2] = F[i][2] + (2 * e[2] * u[i][2] + e[3] * u[i][1] + e[4] * u[i][0]);
+        } else if constexpr (NF == 12) {
+            g[i][fi - 9] = v;
+        }
+    };
+
+    load_batch(0);
+#pragma unroll
+    for (int b = 0; b < NBATCH; ++b) {
+        const int npb = (b * PB + PB <= NPAIR) ? PB : NPAIR - b * PB;   // pairs in this batch
+        // Z = A + i B of each pair with both half spectra completed Hermitian;
+        // kx = 0 holds the real A[0] + i A[Lx/2]
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+            if (p >= npb) break;
+            const bool two = 2 * (b * PB + p) + 1 < NF;
+#pragma unroll
+            for (int i = 0; i < HPT; ++i) {
+                const int e = tid + i * NT;
+                if (HP % NT != 0 && e >= HP) continue;
+                const int r = e / MX, kx = e - r * MX, line = p * RB + r;
+                const cpx<T> A = h[p][0][i], B = two ? h[p][1][i] : cpx<T>{T(0), T(0)};
+                if (kx == 0) {
+                    buf[line] = cpx<T>{A.x, B.x};
+                    buf[MX * ST + line] = cpx<T>{A.y, B.y};
+                } else {
+                    buf[kx * ST + line] = cpx<T>{A.x - B.y, A.y + B.x};
+                    buf[(LX - kx) * ST + line] = cpx<T>{A.x + B.y, B.x - A.y};
+                }
+            }
+        }
+        // the next batch's spectra (or, before the last transform, the
+        // momenta) are requested while this batch transforms
+        if (b + 1 < NBATCH) {
+            load_batch(b + 1);
+        } else if (kick) {
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                if (!live[i]) continue;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) pp[i][q] = (a.mode == XI_NP ? f.pk[q] : f.p[q])[cell[i]];
+            }
+        }
+        __syncthreads();
+        if (npb == PB) fft_lds<T, LX, NL, NT, +1, ST>(buf, tws, tid);
+        else fft_lds<T, LX, RB, NT, +1, ST>(buf, tws, tid);
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) {
+            const int e = tid + i * NT;
+            if (CELLS % NT != 0 && e >= CELLS) continue;
+            const int r = e / LX, x = e - r * LX;
+#pragma unroll
+            for (int p = 0; p < PB; ++p) {
+                if (p >= npb) break;
+                const int f0 = 2 * (b * PB + p);
+                const cpx<T> z = buf[x * ST + p * RB + r];
+                fold(i, f0, (double)z.x);
+                if (f0 + 1 < NF) fold(i, f0 + 1, (double)z.y);
+            }
+        }
+        __syncthreads();
+    }
+
+    double eunh = 0.0, ede = 0.0, ke1 = 0.0, ke2 = 0.0, kea = 0.0;
+    const bool film = c.mir_lo < c.Lz;
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+        if (!live[i]) continue;
+        const int e = tid + i * NT;
+        const int r = e / LX, x = e - r * LX;
+        const size_t cc = cell[i];
+        const int s = code[i] & 3;
+        const bool counted = (code[i] & 8) != 0;
+        // film/epit: the dipole force and kick of a mirrored cell come from
+        // its source cell (mirror_force_mirror.F:80-84, k_mirror_kick)
+        const bool mirrored = film && row0 + r >= c.mir_lo * c.Ly;
+        const double minv = (code[i] & 4) ? 0.0 : c.minv[s];
+        double* Fi = F[i];
+        const double* ui = u[i];
+        // on-site anharmonic terms (dVddi-unharmonic.F:34-65)
+        const double d0 = ui[0] * ui[0], d1 = ui[1] * ui[1], d2 = ui[2] * ui[2];
+        const double d[3] = {d0, d1, d2};
+        const double ss = d0 + d1 + d2;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int q1 = (q + 1) % 3, q2 = (q + 2) % 3;
+            Fi[q] = Fi[q] + ui[q] * 2 * (c.kappa2[s] - c.kappa2[0]);
+            Fi[q] = Fi[q] + ui[q] * (c.fa2g[s] * ss - c.tg[s] * d[q]);
+            Fi[q] = Fi[q] + ui[q] * (c.sk[s] * (ss * ss) + c.k4x8[s] * (ss * ss * ss)
+                                     - c.tk2[s] * (d[q] * d[q]) - c.fk[s] * d[q1] * d[q2]);
+        }
+        eunh += (c.kappa2[s] - c.kappa2[0]) * ss + c.alpha[s] * (ss * ss)
+              + c.gamma[s] * (d0 * d1 + d1 * d2 + d2 * d0) + c.k1[s] * (ss * ss * ss)
+              + c.k2[s] * ((d0 * d0) * (d1 + d2) + (d1 * d1) * (d2 + d0) + (d2 * d2) * (d0 + d1))
+              + c.k3[s] * d0 * d1 * d2 + c.k4[s] * ((ss * ss) * (ss * ss));
+        // external and local field (mirror_force_mirror.F:39-65)
+        if (c.efield) {
+            double E[3];
+            if (mirrored && sc->e_mir != 0.0) {                     // initialize_E_field.F:68-71
+                const int rr = row0 + r, iz = rr / c.Ly, iy = rr - iz * c.Ly;
+                const size_t cs = ((size_t)(c.mir_src - iz) * c.Ly + iy) * LX + x;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double Es = c.has_eloc ? f.eloc[q][cs] + sc->E_add[q] : sc->E_add[q];
+                    E[q] = q < 2 ? -Es : Es;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Fi[q] = Fi[q] - c.Zstar * E[q];
+            ede += c.Zstar * (ui[0] * E[0] + ui[1] * E[1] + ui[2] * E[2]);
+        }
+        // homogeneous coupling (dVddi-homo-coupling.F:28-44)
+        {
+            const double* eh = sc->e_homo;
+            Fi[0] = Fi[0] + 2 * eh[0] * ui[0] + eh[4] * ui[2] + eh[5] * ui[1];
+            Fi[1] = Fi[1] + 2 * eh[1] * ui[1] + eh[3] * ui[2] + eh[5] * ui[0];
+            Fi[2] = Fi[2] + 2 * eh[2] * ui[2] + eh[3] * ui[1] + eh[4] * ui[0];
+        }
+        // kicks
+        if (mirrored) {
+        } else if (a.mode == XI_NP) {
+            const double dt2s = c.dt_2 * sc->s;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double p1 = pp[i][q] - dt2s * Fi[q];                 // nose-poincare.F:66-71
+                const double p2 = p1 - dt2s * Fi[q];                       // nose-poincare.F:24-29 (next step)
+                if (counted) { ke1 += p1 * p1 * minv; ke2 += p2 * p2 * minv; }
+                f.pk[q][cc] = p2;
+                if (a.materialize) f.p[q][cc] = p1;
+            }
+        } else if (a.mode == XI_LF) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double pq = pp[i][q] - c.dt * Fi[q];                 // leapfrog.F:134-139
+                if (counted) ke1 += pq * pq * minv;
+                f.p[q][cc] = pq;
+            }
+        } else if (a.mode == XI_VS) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double pq = counted ? pp[i][q] - c.dt * Fi[q] : 0.0;      // velocity_scaling.F:62-76
+                if (counted) ke1 += pq * pq * minv;
+                f.p[q][cc] = pq;
+            }
+        }
+        if ((a.materialize || a.mode == XI_FORCE || film) && !mirrored) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) f.F[q][cc] = Fi[q];
+        }
+        if constexpr (NF == 12) {
+            const double mwinv = c.mwinv[s];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (a.mode == XI_LF || a.mode == XI_VS) {
+                    const double pq = f.pw[q][cc] - c.dt * g[i][q];
+                    kea += pq * pq * mwinv;
+                    f.pw[q][cc] = pq;
+                }
+                if (a.materialize || a.mode == XI_FORCE) f.dVdw[q][cc] = g[i][q];
+            }
+        }
+    }
+    const int nb = a.maxblocks;
+    double r[5] = {eunh, ede, ke1, ke2, kea};
+    block_sums<5>(r, red);
+    if (tid == 0) {
+        a.partials[P_EUNH * nb + bx] = r[0];
+        a.partials[P_EDE * nb + bx] = r[1];
+        a.partials[P_KE1 * nb + bx] = r[2];
+        a.partials[P_KE2 * nb + bx] = r[3];
+        a.partials[P_KEA * nb + bx] = r[4];
+    }
+}
+
 template <typename T, int LX, int NF>
 __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, XinvArgs a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* __restrict__ twx)
 {
-    xinv_body<T, LX, NF>(c, f, sp, a, sc, twx, blockIdx.x);
+    if constexpr (NF >= 9) xinv_step_body<T, LX, NF>(c, f, sp, a, sc, twx, blockIdx.x);
+    else xinv_body<T, LX, NF>(c, f, sp, a, sc, twx, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2354,7 +2718,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NF == 12 ? 2
     const Rep<T>& r = reps[blockIdx.z];
     XinvArgs a = r.ia;
     a.materialize = materialize;
-    xinv_body<T, LX, NF>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
+    if constexpr (NF >= 9) xinv_step_body<T, LX, NF>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
+    else xinv_body<T, LX, NF>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
 }
 
 template <typename T>

@@ -315,6 +315,12 @@ int fhip_profile(fhip_ctx *ctx, int enable);
 int fhip_use_graph(fhip_ctx *ctx, int enable);
 int fhip_kernel_stats(fhip_ctx *ctx, int n, const char **names, double *total_ms, int64_t *launches);
 int fhip_synchronize(fhip_ctx *ctx);
+/* No counterpart in the reference (measurement): the attainable HBM rate of
+ * the current device, a copy of `bytes` (a whole number of 16-byte words)
+ * between two device buffers, timed with HIP events over reps launches after
+ * two of warm-up; gbs = (bytes read + bytes written) / time per copy.  The
+ * two buffers are allocated and freed by the call. */
+int fhip_copy_rate(int64_t bytes, int reps, double *gbs, double *us_per_copy);
 /* Bytes of device memory the context holds. */
 int64_t fhip_device_bytes(const fhip_ctx *ctx);
 

@@ -120,6 +120,32 @@ static void check_group(const char* what, size_t pool_elems, int side_nf)
     }
     if (GROUP == ZG_PLAIN) return;
 
+    // register first / last passes (zpass_body's REG path): thread (t0, b)
+    // writes its first-pass outputs to rows PPT*t0 + q and reads its last-pass
+    // inputs from rows t0 + q*S, with twiddles tws[t0*q]
+    constexpr int S = NT / TB;
+    constexpr bool REG = NT % TB == 0 && EL % NT == 0 && dft_radix(PPT) && LZ % (PPT * PPT) == 0;
+    if constexpr (REG) {
+        std::vector<int> seen_w(EL, 0), seen_r(EL, 0);
+        for (int tid = 0; tid < g_threads; ++tid) {
+            const int t0 = tid / TB, b = tid % TB;
+            for (int q = 0; q < PPT; ++q) {
+                const int tw_ = PPT * t0 + q, tr = t0 + q * S;
+                CHECK(tw_ < LZ && tr < LZ && t0 * q < LZ, "%s LZ=%d register pass rows t0=%d q=%d", what, LZ, t0, q);
+                seen_w[tw_ * TB + b] += 1;
+                seen_r[tr * TB + b] += 1;
+                for (int f = 0; f < FB; ++f) {
+                    CHECK((size_t)(tw_ * STB + f * TB + b) < pool_elems, "%s register first-pass slot", what);
+                    CHECK((size_t)(tr * STB + f * TB + b) < pool_elems, "%s register last-pass slot", what);
+                }
+                int th, bh;
+                Map::hbm(tid, q, th, bh);
+                CHECK(th == tr && bh == b, "%s register points differ from the HBM map", what);
+            }
+        }
+        for (int e = 0; e < EL; ++e) CHECK(seen_w[e] == 1 && seen_r[e] == 1, "%s register pass coverage", what);
+    }
+
     // side buffer and global offsets for every compiled Lx, Ly
     std::vector<int> lord;
     for (int Lx : kSizes) {
@@ -168,6 +194,17 @@ static void check_group(const char* what, size_t pool_elems, int side_nf)
                             CHECK(kk < Mh && kk < Kh, "%s k index", what);
                         }
                     }
+                // register path: the R-pencil split pairs (t, LZ - t) in the side buffer
+                if constexpr (REG) {
+                    constexpr int npair = (LZ - 1) / 2;
+                    const int npen = nsp / LZ;
+                    for (int e = 0; e < npen * npair * NF; ++e) {
+                        const int fi = e % NF, rest = e / NF, t = 1 + rest % npair, ord = rest / npair;
+                        CHECK(b0 + ord * Mx < TB, "%s split pencil", what);
+                        CHECK((size_t)((ord * LZ + t) * NF + fi) < side_elems, "%s split lo", what);
+                        CHECK((size_t)((ord * LZ + LZ - t) * NF + fi) < side_elems && LZ - t > t, "%s split hi", what);
+                    }
+                }
                 // kx = 0 evaluations: j < nsp over the threads, first half at side + nsp*NF
                 for (int j = 0; j < nsp; ++j)
                     for (int q = 0; q < NF; ++q) {
@@ -193,8 +230,10 @@ static void check_edge()
     // static LDS of the paired kernel fits the 64 KB static limit
     CHECK(sizeof(ZLds<T, LZ, zpair_buf_size<T, LZ, ZG_STRAIN>()>) <= 64 * 1024, "static LDS strain LZ=%d", LZ);
     CHECK(sizeof(ZLds<T, LZ, zpair_buf_size<T, LZ, ZG_ACOU>()>) <= 64 * 1024, "static LDS acou LZ=%d", LZ);
-    std::printf("z pass %s LZ=%3d TB=%2d PPT=%d: in bounds\n", sizeof(T) == 8 ? "fp64" : "fp32", LZ,
-                ZTile<T, LZ>::TB, ZTile<T, LZ>::PPT);
+    constexpr int TB = ZTile<T, LZ>::TB, PPT = ZTile<T, LZ>::PPT;
+    constexpr bool REG = NT % TB == 0 && (LZ * TB) % NT == 0 && dft_radix(PPT) && LZ % (PPT * PPT) == 0;
+    std::printf("z pass %s LZ=%3d TB=%2d PPT=%d%s: in bounds\n", sizeof(T) == 8 ? "fp64" : "fp32", LZ, TB, PPT,
+                REG ? " (register first/last passes)" : "");
 }
 
 int main(int argc, char** argv)

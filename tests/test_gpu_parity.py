@@ -62,7 +62,7 @@ def test_long_range_table(oracle_mod, L):
     assert _normwise(lg, lo) <= FORCE_TOL
 
 
-@pytest.mark.parametrize("L", [8, 12, 16, (8, 12, 16), (16, 6, 24), 24])
+@pytest.mark.parametrize("L", [8, 12, 16, (8, 12, 16), (16, 6, 24), 24, (8, 8, 256), (12, 6, 128), (6, 16, 64)])
 def test_force_and_energies(oracle_mod, L):
     p = batio3_zvr(L)
     o, g = _pair(oracle_mod, p)
@@ -331,7 +331,7 @@ def _system32(p):
     return System(p, precision=1)
 
 
-@pytest.mark.parametrize("L", [12, 16, (8, 12, 16), 32])
+@pytest.mark.parametrize("L", [12, 16, (8, 12, 16), 32, (8, 8, 256), (12, 6, 128), (6, 16, 64)])
 def test_fp32_force_and_energies(oracle_mod, L):
     p = batio3_zvr(L)
     o = oracle_mod.Oracle(p)
