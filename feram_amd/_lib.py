@@ -28,7 +28,8 @@ EXPORTS = ("fhip_create", "fhip_destroy", "fhip_last_error", "fhip_set_long_rang
            "fhip_set_E_schedule", "fhip_get_step", "fhip_force", "fhip_force_static", "fhip_md_prologue",
            "fhip_nose_poincare",
            "fhip_leapfrog", "fhip_velocity_scaling", "fhip_profile", "fhip_use_graph", "fhip_kernel_stats",
-           "fhip_synchronize", "fhip_copy_rate", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
+           "fhip_synchronize", "fhip_copy_rate", "fhip_checkpoint_bytes", "fhip_checkpoint_save",
+           "fhip_checkpoint_load", "fhip_device_bytes", "fhip_avg_reset", "fhip_avg_add", "fhip_avg_auto",
            "fhip_avg_get", "fhip_batch_create", "fhip_batch_destroy", "fhip_batch_last_error",
            "fhip_batch_step", "fhip_batch_graph", "fhip_batch_tune", "fhip_batch_plan", "fhip_batch_size",
            "fhip_batch_classes",
@@ -134,6 +135,10 @@ def load():
     L.fhip_use_graph.argtypes = [vp, ip]
     L.fhip_kernel_stats.argtypes = [vp, ip, C.POINTER(C.c_char_p), dp, C.POINTER(C.c_int64)]
     L.fhip_synchronize.argtypes = [vp]
+    L.fhip_checkpoint_bytes.argtypes = [vp]
+    L.fhip_checkpoint_bytes.restype = C.c_int64
+    L.fhip_checkpoint_save.argtypes = [vp, vp, C.c_int64]
+    L.fhip_checkpoint_load.argtypes = [vp, vp, C.c_int64]
     L.fhip_copy_rate.argtypes = [C.c_int64, ip, dp, dp]
     L.fhip_copy_rate.restype = ip
     L.fhip_device_bytes.argtypes = [vp]

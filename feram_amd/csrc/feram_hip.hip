@@ -790,6 +790,72 @@ int static_fail(char *err, int errlen, int code, const std::string &msg)
     return code;
 }
 
+// ---- binary checkpoint -------------------------------------------------------
+// Header, then the sections in this order: the device scalars; u, p, dV/du
+// (FP64, 3N each); w, p_w, dV/dw with acoustic MD; the optimised w(k) when it
+// is materialised; the averaging sums, histograms and dipoRavg when present;
+// the heat-source layer energies when present.
+constexpr char kCkptMagic[8] = {'F', 'H', 'I', 'P', 'C', 'K', 'P', 'T'};
+constexpr uint32_t kCkptVersion = 1;
+enum { CK_AVG = 1, CK_HIST = 2, CK_RAVG = 4, CK_ACOUK = 8, CK_LAYERS = 16, CK_AVG_AUTO = 32 };
+struct CkptHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    int32_t Lx, Ly, Lz, method, precision, acoustic, geometry, flags;
+    int64_t scalars_bytes, total_bytes;
+};
+
+struct CkptSection {
+    void *dev;
+    size_t bytes;
+};
+
+std::vector<CkptSection> ckpt_sections(fhip_ctx *ctx, int flags)
+{
+    std::vector<CkptSection> v;
+    const size_t N = ctx->N;
+    v.push_back({ctx->dsc, sizeof(Scalars)});
+    v.push_back({ctx->f.u[0], 3 * N * sizeof(double)});
+    v.push_back({ctx->f.p[0], 3 * N * sizeof(double)});
+    v.push_back({ctx->f.F[0], 3 * N * sizeof(double)});
+    if (ctx->c.acoustic) {
+        v.push_back({ctx->f.w[0], 3 * N * sizeof(double)});
+        v.push_back({ctx->f.pw[0], 3 * N * sizeof(double)});
+        v.push_back({ctx->f.dVdw[0], 3 * N * sizeof(double)});
+    }
+    if (flags & CK_ACOUK) v.push_back({ctx->acouK, 3 * (size_t)ctx->Kh * sizeof(cpx<double>)});
+    if (flags & CK_AVG) v.push_back({ctx->davg, sizeof(AvgAcc)});
+    if (flags & CK_HIST) {
+        v.push_back({ctx->dist1, 3 * (size_t)DIST_W * sizeof(unsigned long long)});
+        v.push_back({ctx->dist3, (size_t)DIST_W * DIST_W * DIST_W * sizeof(int)});
+    }
+    if (flags & CK_RAVG) v.push_back({ctx->f.uavg[0], 3 * N * sizeof(double)});
+    if (flags & CK_LAYERS) {
+        v.push_back({ctx->el, 2 * (size_t)ctx->Lz * sizeof(double)});
+        v.push_back({ctx->el_sum, 2 * (size_t)ctx->Lz * sizeof(double)});
+    }
+    return v;
+}
+
+int ckpt_flags(const fhip_ctx *ctx)
+{
+    int fl = 0;
+    if (ctx->acouK_valid && !ctx->c.acoustic) fl |= CK_ACOUK;
+    if (ctx->davg) fl |= CK_AVG;
+    if (ctx->davg && ctx->avg_hist && ctx->dist1) fl |= CK_HIST;
+    if (ctx->davg && ctx->avg_ravg && ctx->f.uavg[0]) fl |= CK_RAVG;
+    if (ctx->el && ctx->el_sum) fl |= CK_LAYERS;
+    if (ctx->avg_auto) fl |= CK_AVG_AUTO;
+    return fl;
+}
+
+size_t ckpt_total(fhip_ctx *ctx, int flags)
+{
+    size_t t = sizeof(CkptHeader);
+    for (const auto &sct : ckpt_sections(ctx, flags)) t += sct.bytes;
+    return t;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -1336,6 +1402,73 @@ int fhip_copy_rate(int64_t bytes, int reps, double *gbs, double *us_per_copy)
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return rc;
+}
+
+int64_t fhip_checkpoint_bytes(fhip_ctx *ctx)
+{
+    if (!ctx) return FHIP_ERR_ARG;
+    return (int64_t)ckpt_total(ctx, ckpt_flags(ctx));
+}
+
+int fhip_checkpoint_save(fhip_ctx *ctx, void *buf, int64_t bytes)
+{
+    if (!ctx || !buf) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    const int flags = ckpt_flags(ctx);
+    const size_t total = ckpt_total(ctx, flags);
+    if (bytes < (int64_t)total) return fail(ctx, FHIP_ERR_ARG, "checkpoint buffer too small (fhip_checkpoint_bytes)");
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, sizeof(h.magic));
+    h.version = kCkptVersion;
+    h.header_bytes = sizeof(CkptHeader);
+    h.Lx = ctx->Lx; h.Ly = ctx->Ly; h.Lz = ctx->Lz;
+    h.method = ctx->p.method; h.precision = ctx->precision; h.acoustic = ctx->c.acoustic;
+    h.geometry = ctx->p.bulk_or_film; h.flags = flags;
+    h.scalars_bytes = sizeof(Scalars);
+    h.total_bytes = (int64_t)total;
+    unsigned char *out = (unsigned char *)buf;
+    std::memcpy(out, &h, sizeof(h));
+    size_t off = sizeof(h);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (const auto &sct : ckpt_sections(ctx, flags)) {
+        HIPCHK(hipMemcpy(out + off, sct.dev, sct.bytes, hipMemcpyDeviceToHost));
+        off += sct.bytes;
+    }
+    return FHIP_OK;
+}
+
+int fhip_checkpoint_load(fhip_ctx *ctx, const void *buf, int64_t bytes)
+{
+    if (!ctx || !buf) return fail(ctx, FHIP_ERR_ARG, "null argument");
+    if (bytes < (int64_t)sizeof(CkptHeader)) return fail(ctx, FHIP_ERR_ARG, "checkpoint too short");
+    CkptHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, kCkptMagic, sizeof(h.magic)) != 0 || h.version != kCkptVersion ||
+        h.header_bytes != sizeof(CkptHeader) || h.scalars_bytes != (int64_t)sizeof(Scalars))
+        return fail(ctx, FHIP_ERR_ARG, "not a checkpoint of this library version");
+    if (h.Lx != ctx->Lx || h.Ly != ctx->Ly || h.Lz != ctx->Lz || h.method != ctx->p.method ||
+        h.precision != ctx->precision || h.acoustic != ctx->c.acoustic || h.geometry != ctx->p.bulk_or_film)
+        return fail(ctx, FHIP_ERR_ARG, "checkpoint of another supercell, method, precision or geometry");
+    if ((h.flags & CK_LAYERS) && !(ctx->el && ctx->el_sum))
+        return fail(ctx, FHIP_ERR_STATE, "checkpoint holds heat-source layers: call fhip_set_heat_source first");
+    // averaging buffers as the saving context had them
+    if (h.flags & CK_AVG) {
+        const int opts = ((h.flags & CK_HIST) ? FHIP_AVG_HISTOGRAMS : 0) | ((h.flags & CK_RAVG) ? FHIP_AVG_DIPO_R_AVG : 0);
+        RC(fhip_avg_reset(ctx, opts));
+    }
+    const size_t total = ckpt_total(ctx, h.flags & ~CK_AVG_AUTO);
+    if (h.total_bytes != (int64_t)total || bytes < (int64_t)total)
+        return fail(ctx, FHIP_ERR_ARG, "checkpoint size does not match its sections");
+    const unsigned char *in = (const unsigned char *)buf;
+    size_t off = sizeof(h);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (const auto &sct : ckpt_sections(ctx, h.flags)) {
+        HIPCHK(hipMemcpy(sct.dev, in + off, sct.bytes, hipMemcpyHostToDevice));
+        off += sct.bytes;
+    }
+    std::memcpy(ctx->hsc, in + sizeof(h), sizeof(Scalars));
+    ctx->acouK_valid = (h.flags & CK_ACOUK) != 0;
+    ctx->avg_auto = (h.flags & CK_AVG_AUTO) != 0 && ctx->davg;
+    return FHIP_OK;
 }
 
 int fhip_md_prologue(fhip_ctx *ctx, fhip_energy *e)

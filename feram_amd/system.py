@@ -229,6 +229,31 @@ class System:
         self._check(self._L.fhip_set_kelvin(self.ctx, float(kelvin)), "set_kelvin")
         self.p = self.p.copy(kelvin=float(kelvin))
 
+    # ---- exact resume ---------------------------------------------------------
+    def checkpoint(self) -> bytes:
+        """The context's state as the library's binary checkpoint
+        (fhip_checkpoint_save): loading it into a System made with the same
+        parameters resumes bitwise."""
+        n = int(self._L.fhip_checkpoint_bytes(self.ctx))
+        if n < 0:
+            raise FhipError(f"checkpoint_bytes: {_lib.ERRORS.get(n, n)}")
+        buf = (C.c_ubyte * n)()
+        self._check(self._L.fhip_checkpoint_save(self.ctx, buf, n), "checkpoint_save")
+        return bytes(buf)
+
+    def restore(self, data: bytes) -> None:
+        """Load a checkpoint written by checkpoint() (fhip_checkpoint_load)."""
+        buf = (C.c_ubyte * len(data)).from_buffer_copy(data)
+        self._check(self._L.fhip_checkpoint_load(self.ctx, buf, len(data)), "checkpoint_load")
+
+    def save_checkpoint(self, path) -> None:
+        with open(path, "wb") as f:
+            f.write(self.checkpoint())
+
+    def load_checkpoint(self, path) -> None:
+        with open(path, "rb") as f:
+            self.restore(f.read())
+
     # ---- the step ------------------------------------------------------------
     def mirror_force_mirror(self) -> dict:
         e = _lib.FhipEnergy()

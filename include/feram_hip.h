@@ -204,6 +204,24 @@ int fhip_nose_poincare(fhip_ctx *ctx, int nsteps, fhip_energy *e);
 int fhip_leapfrog(fhip_ctx *ctx, int i_step, int nsteps, fhip_energy *e);
 int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e);
 
+/* Binary checkpoint for exact resume.  The reference's restart is the .coord
+ * text written by write_system (coord_module.F:472-483: u in f10.6, p/s and
+ * the forces in e14.6) and read back by initialize-dipoR.F:22-38, which loses
+ * the state beyond those digits; this keeps every double a later step call
+ * reads: the thermostat and integrator scalars (s, pi, H0, the drift factor,
+ * pending velocity scaling, the E-field schedule and its step index, strain,
+ * energies), u, p, dV/du (and w, p_w, dV/dw with acoustic MD), the optimised
+ * w(k) when materialised, the averaging sums, histograms and dipoRavg, and
+ * the heat-source layer energies.  Loading into a context created with the
+ * same parameters (and cells, modulation and heat source set as before) and
+ * stepping gives bitwise what the saving context's next calls give.  The
+ * buffer is host memory of fhip_checkpoint_bytes() bytes; the format is the
+ * library's (a header with the supercell, method, precision and geometry,
+ * checked on load, then the raw sections). */
+int64_t fhip_checkpoint_bytes(fhip_ctx *ctx);
+int fhip_checkpoint_save(fhip_ctx *ctx, void *buf, int64_t bytes);
+int fhip_checkpoint_load(fhip_ctx *ctx, const void *buf, int64_t bytes);
+
 /* Average_type (average_module.F:17-30) as the sums add_to_Average builds
  * (average_module.F:57-128), before divide_and_write_Average divides them. */
 typedef struct fhip_average {
