@@ -126,5 +126,49 @@ def build(verbose: bool = True) -> str:
         print(f"[feram_amd.build] built {LIB} in {time.time() - t0:.1f} s", flush=True)
     return LIB
 
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN, "_build")
+FORTRAN_DRIVER = os.path.join(ROOT, "tests", "host", "_build", "fortran_np")
+
+
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = True):
+    """The ISO_C_BINDING module and the context module (feram_amd/fortran/
+    _build: objects and .mod files for a feram build to link), and the
+    Fortran test driver of the device step (tests/host/fortran_np.f90)
+    linked against libferam_hip.so.  Returns the driver's path, or None
+    without a Fortran compiler."""
+    fc = flang()
+    if fc is None:
+        if verbose:
+            print("[feram_amd.build] no Fortran compiler: Fortran module not built", flush=True)
+        return None
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    os.makedirs(os.path.dirname(FORTRAN_DRIVER), exist_ok=True)
+    objs = []
+    for name in ("feram_hip_binding", "feram_gpu"):
+        obj = os.path.join(FORTRAN_BUILD, name + ".o")
+        cmd = [fc, "-O2", "-module-dir", FORTRAN_BUILD, "-c", os.path.join(FORTRAN, name + ".f90"), "-o", obj]
+        if verbose:
+            print("[feram_amd.build]", " ".join(cmd), flush=True)
+        subprocess.run(cmd, cwd=ROOT, check=True)
+        objs.append(obj)
+    # the driver finds the library next to the package wherever the tree is
+    rel = os.path.relpath(HERE, os.path.dirname(FORTRAN_DRIVER))
+    cmd = [fc, "-O2", "-module-dir", FORTRAN_BUILD, os.path.join(ROOT, "tests", "host", "fortran_np.f90")] + objs + \
+          [f"-L{HERE}", "-lferam_hip", f"-Wl,-rpath,$ORIGIN/{rel}", "-o", FORTRAN_DRIVER]
+    if verbose:
+        print("[feram_amd.build]", " ".join(cmd), flush=True)
+    subprocess.run(cmd, cwd=ROOT, check=True)
+    return FORTRAN_DRIVER
+
+
 if __name__ == "__main__":
     build()
+    build_fortran()

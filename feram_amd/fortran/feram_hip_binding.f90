@@ -1,10 +1,10 @@
 ! feram_hip_binding.f90 -- ISO_C_BINDING interface to libferam_hip.so.
 !
-! The module declares the C ABI of include/feram_hip.h for Fortran callers.
-! A feram build that runs its step on the GPU links this module and the
-! library, and replaces the bodies of nose_poincare, leapfrog,
-! velocity_scaling and mirror_force_mirror with calls into it (see the
-! fhip_* wrappers below; file:line citations are into feram's src/).
+! The module declares the C ABI of include/feram_hip.h for Fortran callers
+! (interfaces only).  feram_gpu.f90 holds the device context and the step
+! calls on top of it, and replacements/ the four units a feram build links in
+! place of nose-poincare.F, leapfrog.F, velocity_scaling.F and
+! mirror_force_mirror.F (file:line citations are into feram's src/).
 module feram_hip_binding
   use, intrinsic :: iso_c_binding
   implicit none
