@@ -22,7 +22,7 @@ SOURCES = [os.path.join(CSRC, "feram_hip.hip"), os.path.join(CSRC, "fhip_batch.h
            os.path.join(CSRC, "fhip_mlp.hip"), os.path.join(CSRC, "fhip_stages.hip")]
 # fhip_stages.hip holds the per-edge launchers and is compiled once per group
 # of edges (FHIP_STAGE_GROUP), so the edges build in parallel
-STAGE_GROUPS = 9
+STAGE_GROUPS = 13
 HEADERS = [os.path.join(CSRC, "fhip_fft.hpp"), os.path.join(CSRC, "fhip_kernels.hpp"),
            os.path.join(CSRC, "fhip_host.hpp"), os.path.join(ROOT, "include", "feram_hip.h")]
 ARCH = os.environ.get("FHIP_ARCH", "gfx950")

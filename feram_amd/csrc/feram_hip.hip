@@ -564,7 +564,7 @@ int check_params(fhip_ctx *ctx, const fhip_params *p)
     if (p->Lx % 2 != 0) return fail(ctx, FHIP_ERR_UNSUPPORTED, "Lx must be even");
     if (!size_supported(p->Lx) || !size_supported(p->Ly) || !size_supported(p->Lz))
         return fail(ctx, FHIP_ERR_UNSUPPORTED,
-                    "supercell edges must be one of 4 6 8 12 16 24 32 48 64 96 128 192 256");
+                    "supercell edges must be one of " FHIP_SIZES_TEXT);
     if (p->method < 0 || p->method > 2) return fail(ctx, FHIP_ERR_ARG, "method must be md, vs or lf");
     if (p->bulk_or_film < 0 || p->bulk_or_film > 3) return fail(ctx, FHIP_ERR_ARG, "bad geometry");
     if (p->gap_id < 0 || p->gap_id > 2) return fail(ctx, FHIP_ERR_ARG, "gap_id must be 0, 1 or 2");

@@ -13,8 +13,12 @@
 
 using namespace fhip;
 
-// Supercell edges with a compiled FFT length.
-#define FHIP_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+// Supercell edges with a compiled FFT length: products of 2, 3 and 5 (the
+// edges of the reference's examples up to 256, including the odd 81 and 243
+// of its gap_id = 1 films).
+#define FHIP_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(60) X(64) X(80) X(81) \
+    X(90) X(96) X(100) X(128) X(192) X(243) X(256)
+#define FHIP_SIZES_TEXT "4 6 8 12 16 20 24 32 40 48 60 64 80 81 90 96 100 128 192 243 256"
 
 namespace fhip_host {
 
@@ -280,4 +284,4 @@ template <typename T, int n> void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s,
 
 // Edges of the batched instantiations.  Supercells above 64 per edge fill the
 // device alone and are stepped with the per-context calls.
-#define FHIP_BATCH_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
+#define FHIP_BATCH_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(60) X(64)

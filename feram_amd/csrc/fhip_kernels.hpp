@@ -815,6 +815,14 @@ __device__ __forceinline__ cpx<KT> ev_out(cpx<KT> acc, cpx<KT> o, int md, KT hal
     return {acc.x, o.x};
 }
 
+// The dipole-dipole group evaluates both points of every pair: the reference
+// applies L(k) at each point of the kx = 0 and kx = Lx/2 planes of its half
+// spectrum from that point's own table entry (dVddi-dipole-dipole.F:40-50),
+// and the table is symmetric only up to the asymmetric image ranges of the
+// real-space Ewald sum (dipole-dipole-long-range.F:41-91), so L(-k) is not
+// L(k) to rounding on anisotropic supercells; the c2r keeps the average.
+// The strain group needs the second evaluation only where the partner's
+// folded k vector is not exactly -k.
 template <bool DD>
 __device__ __forceinline__ KPlan kplan(const Consts& c, int s, int t)
 {
@@ -822,19 +830,19 @@ __device__ __forceinline__ KPlan kplan(const Consts& c, int s, int t)
     const int tp = (t == 0) ? 0 : Lz - t;          // kz of the mirror partner
     KPlan p{1, 0, s, t, EV_N, 0, 0, 0, EV_C, 1.0, -1, 0};
     if (s != 0 && 2 * s < Ly) {                    // P pencil: (0, s, t)
-        if (!DD && 2 * t == Lz) { p.nev = 2; p.y1 = Ly - s; p.z1 = t; p.wm = 0.5; }
+        if (DD || 2 * t == Lz) { p.nev = 2; p.y1 = Ly - s; p.z1 = tp; p.wm = 0.5; }
         else { p.yp = Ly - s; p.zp = tp; }
     } else if (2 * s > Ly) {                       // Q pencil: (Lx/2, Ly - s, t)
         p.x0 = Mx; p.y0 = Ly - s;
-        if (!DD) { p.nev = 2; p.x1 = Mx; p.y1 = s; p.z1 = tp; p.wm = 0.5; }
+        p.nev = 2; p.x1 = Mx; p.y1 = s; p.z1 = tp; p.wm = 0.5;
     } else if (t == 0 || 2 * t == Lz) {            // R pencil, real P and Q
         p.nev = 2; p.m0 = EV_RP; p.m1 = EV_RQ; p.x1 = Mx; p.y1 = s; p.z1 = t; p.wm = 0.5;
     } else if (2 * t < Lz) {                       // R pencil, P(t): (0, s, t)
-        if (!DD && s != 0) { p.nev = 2; p.y1 = s; p.z1 = tp; p.wm = 0.5; }
+        if (DD || s != 0) { p.nev = 2; p.y1 = s; p.z1 = tp; p.wm = 0.5; }
         else { p.yp = s; p.zp = tp; }
     } else {                                       // R pencil, Q(Lz - t): (Lx/2, s, Lz - t)
         p.x0 = Mx; p.z0 = Lz - t;
-        if (!DD) { p.nev = 2; p.x1 = Mx; p.y1 = s; p.z1 = t; p.wm = 0.5; }
+        p.nev = 2; p.x1 = Mx; p.y1 = s; p.z1 = t; p.wm = 0.5;
     }
     return p;
 }

@@ -1,8 +1,8 @@
 // fhip_stages.hip -- launchers of the kernels whose FFT length is a template
 // parameter, for one group of supercell edges.
 //
-// The file is compiled once per group (FHIP_STAGE_GROUP = 0 .. 8), each
-// object holding the device code of its edges only, so the thirteen edges
+// The file is compiled once per group (FHIP_STAGE_GROUP = 0 .. 12), each
+// object holding the device code of its edges only, so the edges
 // compile in parallel translation units; feram_hip.hip and fhip_batch.hip
 // dispatch on the edge and call the instantiation of that edge.
 #include "fhip_host.hpp"
@@ -14,23 +14,38 @@ using namespace fhip_host;
 #error "compile with -DFHIP_STAGE_GROUP=<group>"
 #endif
 
+// groups of edges (the batched ones, up to 64, also get the batch launchers)
 #if FHIP_STAGE_GROUP == 0
 #define FHIP_GROUP_SIZES(X) X(4) X(6)
+#define FHIP_GROUP_BATCH(X) X(4) X(6)
 #elif FHIP_STAGE_GROUP == 1
 #define FHIP_GROUP_SIZES(X) X(8) X(12)
+#define FHIP_GROUP_BATCH(X) X(8) X(12)
 #elif FHIP_STAGE_GROUP == 2
-#define FHIP_GROUP_SIZES(X) X(16) X(24)
+#define FHIP_GROUP_SIZES(X) X(16) X(20)
+#define FHIP_GROUP_BATCH(X) X(16) X(20)
 #elif FHIP_STAGE_GROUP == 3
-#define FHIP_GROUP_SIZES(X) X(32) X(48)
+#define FHIP_GROUP_SIZES(X) X(24) X(32)
+#define FHIP_GROUP_BATCH(X) X(24) X(32)
 #elif FHIP_STAGE_GROUP == 4
-#define FHIP_GROUP_SIZES(X) X(64)
+#define FHIP_GROUP_SIZES(X) X(40) X(48)
+#define FHIP_GROUP_BATCH(X) X(40) X(48)
 #elif FHIP_STAGE_GROUP == 5
-#define FHIP_GROUP_SIZES(X) X(96)
+#define FHIP_GROUP_SIZES(X) X(60) X(64)
+#define FHIP_GROUP_BATCH(X) X(60) X(64)
 #elif FHIP_STAGE_GROUP == 6
-#define FHIP_GROUP_SIZES(X) X(128)
+#define FHIP_GROUP_SIZES(X) X(80) X(81)
 #elif FHIP_STAGE_GROUP == 7
-#define FHIP_GROUP_SIZES(X) X(192)
+#define FHIP_GROUP_SIZES(X) X(90) X(96)
 #elif FHIP_STAGE_GROUP == 8
+#define FHIP_GROUP_SIZES(X) X(100)
+#elif FHIP_STAGE_GROUP == 9
+#define FHIP_GROUP_SIZES(X) X(128)
+#elif FHIP_STAGE_GROUP == 10
+#define FHIP_GROUP_SIZES(X) X(192)
+#elif FHIP_STAGE_GROUP == 11
+#define FHIP_GROUP_SIZES(X) X(243)
+#elif FHIP_STAGE_GROUP == 12
 #define FHIP_GROUP_SIZES(X) X(256)
 #else
 #error "unknown FHIP_STAGE_GROUP"
@@ -171,11 +186,11 @@ void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize)
     FHIP_INST_T(float, n)
 FHIP_GROUP_SIZES(FHIP_INST)
 
-#if FHIP_STAGE_GROUP <= 4          // edges up to 64 are batched
+#ifdef FHIP_GROUP_BATCH
 #define FHIP_INSTB(n)        \
     FHIP_INST_B(double, n)   \
     FHIP_INST_B(float, n)
-FHIP_GROUP_SIZES(FHIP_INSTB)
+FHIP_GROUP_BATCH(FHIP_INSTB)
 #endif
 
 }  // namespace fhip_stage

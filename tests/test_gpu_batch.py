@@ -27,6 +27,7 @@ SWEEP = [  # (edges, precision, kelvin, E_z in V/A)
     ((12, 12, 12), 0, 450.0, 0.0005),
     ((8, 8, 8), 1, 250.0, 0.0),
     ((16, 8, 12), 0, 200.0, 0.0),
+    ((20, 40, 60), 0, 350.0, 0.0005),
 ]
 
 
@@ -71,7 +72,7 @@ def compare(single, batched, energies_s, energies_b):
 
 def test_np_batch_matches_individual_steps():
     from feram_amd.system import Batch
-    print("[gpu] batch md: 5 replicas, 4 size classes, 7 steps", flush=True)
+    print("[gpu] batch md: 6 replicas, 5 size classes, 7 steps", flush=True)
     single, batched = pair("md")
     es = []
     for s in single:
@@ -82,7 +83,7 @@ def test_np_batch_matches_individual_steps():
     for s in single:
         es.append(s.nose_poincare(nsteps))
     with Batch(batched) as bt:
-        assert bt.classes == 4
+        assert bt.classes == 5
         eb = bt.step(nsteps)
     compare(single, batched, es, eb)
     # a second call continues from the state the first left

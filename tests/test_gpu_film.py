@@ -68,7 +68,9 @@ def _check_energies(eg, eo, tol):
 
 
 CASES = [("epit", 0, "md", (8, 6, 16)), ("film", 0, "lf", (8, 8, 12)), ("epit", 2, "vs", (8, 6, 16)),
-         ("film", 2, "md", (6, 8, 8)), ("epit", 0, "lf", (12, 8, 8))]
+         ("film", 2, "md", (6, 8, 8)), ("epit", 0, "lf", (12, 8, 8)),
+         # gap_id 1 needs an odd Lz (param_module.F:475-477): the 16x16x81 examples'
+         ("epit", 1, "md", (16, 16, 81)), ("film", 1, "vs", (8, 8, 81)), ("epit", 1, "lf", (8, 12, 243))]
 
 
 @pytest.mark.parametrize("geometry,gap,method,L", CASES)

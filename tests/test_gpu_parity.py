@@ -62,7 +62,8 @@ def test_long_range_table(oracle_mod, L):
     assert _normwise(lg, lo) <= FORCE_TOL
 
 
-@pytest.mark.parametrize("L", [8, 12, 16, (8, 12, 16), (16, 6, 24), 24, (8, 8, 256), (12, 6, 128), (6, 16, 64)])
+@pytest.mark.parametrize("L", [8, 12, 16, (8, 12, 16), (16, 6, 24), 24, (8, 8, 256), (12, 6, 128), (6, 16, 64),
+                               (16, 16, 80), (32, 20, 60), (40, 90, 12), (100, 16, 8), (16, 16, 81), (8, 20, 243)])
 def test_force_and_energies(oracle_mod, L):
     p = batio3_zvr(L)
     o, g = _pair(oracle_mod, p)
@@ -122,9 +123,9 @@ def test_acoustic_force(oracle_mod):
     _check_energies(eg, eo)
 
 
-@pytest.mark.parametrize("method", ["md", "lf", "vs"])
-def test_trajectory(oracle_mod, method):
-    p = batio3_zvr(12, method=method, kelvin=300.0)
+@pytest.mark.parametrize("method,L", [("md", 12), ("lf", 12), ("vs", 12), ("md", 40), ("vs", (20, 16, 81))])
+def test_trajectory(oracle_mod, method, L):
+    p = batio3_zvr(L, method=method, kelvin=300.0)
     o, g = _pair(oracle_mod, p)
     u = _random(p, 21)
     o.field("dipoR")[...] = u
@@ -331,7 +332,8 @@ def _system32(p):
     return System(p, precision=1)
 
 
-@pytest.mark.parametrize("L", [12, 16, (8, 12, 16), 32, (8, 8, 256), (12, 6, 128), (6, 16, 64)])
+@pytest.mark.parametrize("L", [12, 16, (8, 12, 16), 32, (8, 8, 256), (12, 6, 128), (6, 16, 64), (16, 16, 80),
+                               (40, 90, 12), (16, 16, 81)])
 def test_fp32_force_and_energies(oracle_mod, L):
     p = batio3_zvr(L)
     o = oracle_mod.Oracle(p)

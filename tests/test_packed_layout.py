@@ -9,7 +9,11 @@ same plan as kplan in fhip_kernels.hpp) and must reproduce what the reference
 computes on its full half spectrum: rfftn, a per-point kernel, and a c2r that
 keeps the Hermitian part (FFTW's rule, reproduced by numpy's irfft).  The
 kernels are chosen so that a dropped partner evaluation, a wrong partner index
-or a swapped P/Q slot changes the result.
+or a swapped P/Q slot changes the result: the strain kernel is a function of
+the folded k vector that is not Hermitian where the partner is not exactly
+-k, and the dipole-dipole kernel is a table over the stored points that is
+not symmetric under k -> -k (as the reference's L(k) is not exactly, on
+anisotropic supercells, from the asymmetric image ranges of its Ewald sum).
 """
 import numpy as np
 import pytest
@@ -26,13 +30,22 @@ def kvec(kx, ky, kz, Lx, Ly, Lz):
     return np.array([kcomp(kx, Lx, False), kcomp(ky, Ly, True), kcomp(kz, Lz, True)])
 
 
-def g_strain(k):
-    # odd imaginary part: Hermitian-preserving only where the partner is exactly -k
-    return 1 + k[0] ** 2 + 0.3 * k[1] * k[2] + 1j * (k[0] + 0.5 * k[1] - k[2] + 0.2 * k[0] * k[1] * k[2])
+def strain_kernel(shape):
+    """Kernel of the folded k vector with an odd imaginary part: Hermitian-
+    preserving only where the partner is exactly -k."""
+    Lz, Ly, Lx = shape
+
+    def g(p):
+        k = kvec(*p, Lx, Ly, Lz)
+        return 1 + k[0] ** 2 + 0.3 * k[1] * k[2] + 1j * (k[0] + 0.5 * k[1] - k[2] + 0.2 * k[0] * k[1] * k[2])
+    return g
 
 
-def g_dd(k):
-    return 1 + k[0] ** 2 + k[1] ** 2 + 0.7 * k[2] ** 2
+def dd_kernel(shape, seed=11):
+    """A real table over the half-spectrum points, not symmetric under k -> -k."""
+    Lz, Ly, Lx = shape
+    table = 1 + np.random.default_rng(seed).uniform(0.0, 1.0, size=(Lx // 2 + 1, Ly, Lz))
+    return lambda p: table[p]
 
 
 def reference(a, g):
@@ -43,7 +56,7 @@ def reference(a, g):
     for kz in range(Lz):
         for ky in range(Ly):
             for kx in range(Lx // 2 + 1):
-                gk = g(kvec(kx, ky, kz, Lx, Ly, Lz))
+                gk = g((kx, ky, kz))
                 O[kz, ky, kx] = gk * H[kz, ky, kx]
                 wgt = 1.0 if kx in (0, Lx // 2) else 2.0
                 E += wgt * np.real(np.conj(H[kz, ky, kx]) * gk * H[kz, ky, kx])
@@ -60,25 +73,23 @@ def join_pq(v, m, lower):
 
 
 def kplan(s, t, Lx, Ly, Lz, dd):
-    """Evaluations of the element (kx = 0 pencil, slot s, index t): list of (point, mode), weight."""
+    """Evaluations of the element (kx = 0 pencil, slot s, index t): list of (point, mode), weight.
+    The dipole-dipole group evaluates both points of every pair; the strain
+    group only where the partner's folded k vector is not exactly -k."""
     Mx = Lx // 2
     tp = 0 if t == 0 else Lz - t
     if s != 0 and 2 * s < Ly:
-        if not dd and 2 * t == Lz:
-            return [((0, s, t), 'N'), ((0, Ly - s, t), 'C')], 0.5
+        if dd or 2 * t == Lz:
+            return [((0, s, t), 'N'), ((0, Ly - s, tp), 'C')], 0.5
         return [((0, s, t), 'N')], 1.0
     if 2 * s > Ly:
-        if dd:
-            return [((Mx, Ly - s, t), 'N')], 1.0
         return [((Mx, Ly - s, t), 'N'), ((Mx, s, tp), 'C')], 0.5
     if t == 0 or 2 * t == Lz:
         return [((0, s, t), 'RP'), ((Mx, s, t), 'RQ')], 0.5
     if 2 * t < Lz:
-        if not dd and s != 0:
+        if dd or s != 0:
             return [((0, s, t), 'N'), ((0, s, tp), 'C')], 0.5
         return [((0, s, t), 'N')], 1.0
-    if dd:
-        return [((Mx, s, Lz - t), 'N')], 1.0
     return [((Mx, s, Lz - t), 'N'), ((Mx, s, t), 'C')], 0.5
 
 
@@ -94,7 +105,8 @@ def packed_pipeline(a, g, dd, plan_fn=kplan):
         if 2 * t != Ly:
             S[:, t, 0] = split_pq(C[:, t], C[:, Ly - t], 2 * t < Ly)
     S = np.fft.fft(S, axis=0)
-    for s in (0, Ly // 2):                                         # R pencils
+    rpen = (0, Ly // 2) if Ly % 2 == 0 else (0,)                    # R pencils
+    for s in rpen:
         R = S[:, s, 0].copy()
         for t in range(1, Lz):
             if 2 * t != Lz:
@@ -106,7 +118,7 @@ def packed_pipeline(a, g, dd, plan_fn=kplan):
             for kx in range(Mx):
                 x = S[t, s, kx]
                 if kx != 0:
-                    gk = g(kvec(kx, s, t, Lx, Ly, Lz))
+                    gk = g((kx, s, t))
                     Out[t, s, kx] = gk * x
                     E += 2.0 * np.real(np.conj(x) * gk * x)
                     continue
@@ -115,7 +127,7 @@ def packed_pipeline(a, g, dd, plan_fn=kplan):
                 out = 0j
                 for (p, md) in plan:
                     xe = {'N': x, 'C': np.conj(x), 'RP': x.real + 0j, 'RQ': x.imag + 0j}[md]
-                    o = g(kvec(*p, Lx, Ly, Lz)) * xe
+                    o = g(p) * xe
                     E += 2.0 * wm * np.real(np.conj(xe) * o)
                     if md == 'N':
                         out += half * o
@@ -126,7 +138,7 @@ def packed_pipeline(a, g, dd, plan_fn=kplan):
                     else:
                         out = out.real + 1j * o.real
                 Out[t, s, kx] = out
-    for s in (0, Ly // 2):
+    for s in rpen:
         R = Out[:, s, 0].copy()
         for t in range(1, Lz):
             if 2 * t != Lz:
@@ -144,25 +156,36 @@ def packed_pipeline(a, g, dd, plan_fn=kplan):
     return np.fft.irfft(Z, n=Lx, axis=2), E
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 12, 8), (4, 4, 4), (8, 6, 12), (12, 8, 16)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 12, 8), (4, 4, 4), (8, 6, 12), (12, 8, 16), (9, 6, 8), (8, 9, 6)])
 @pytest.mark.parametrize("dd", [False, True])
 def test_packed_pipeline_matches_half_spectrum(shape, dd):
     a = np.random.default_rng(7).standard_normal(shape)
-    g = g_dd if dd else g_strain
+    g = dd_kernel(shape) if dd else strain_kernel(shape)
     ref, Eref = reference(a, g)
     got, E = packed_pipeline(a, g, dd)
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
     assert abs(E - Eref) <= 1e-13 * abs(Eref)
 
 
+def _first_only(s, t, Lx, Ly, Lz, dd):
+    plan, _ = kplan(s, t, Lx, Ly, Lz, dd)
+    return plan[:1], 1.0
+
+
 def test_single_evaluation_of_the_nyquist_plane_is_wrong():
     """Evaluating the kx = Lx/2 points once (no partner) must fail: the check has teeth."""
     a = np.random.default_rng(3).standard_normal((8, 8, 8))
-    ref, _ = reference(a, g_strain)
+    g = strain_kernel(a.shape)
+    ref, _ = reference(a, g)
+    got, _ = packed_pipeline(a, g, False, plan_fn=_first_only)
+    assert np.max(np.abs(got - ref)) > 1e-6 * np.max(np.abs(ref))
 
-    def first_only(s, t, Lx, Ly, Lz, dd):
-        plan, _ = kplan(s, t, Lx, Ly, Lz, dd)
-        return plan[:1], 1.0
 
-    got, _ = packed_pipeline(a, g_strain, False, plan_fn=first_only)
+def test_single_dipole_evaluation_misses_the_table_asymmetry():
+    """One evaluation per kx = 0 / Lx/2 pair with the point's own table value
+    (a plan that assumes L(-k) = L(k)) misses the reference's result."""
+    a = np.random.default_rng(5).standard_normal((8, 6, 12))
+    g = dd_kernel(a.shape)
+    ref, _ = reference(a, g)
+    got, _ = packed_pipeline(a, g, True, plan_fn=_first_only)
     assert np.max(np.abs(got - ref)) > 1e-6 * np.max(np.abs(ref))

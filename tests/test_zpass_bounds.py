@@ -9,6 +9,7 @@ enumerated for a 512-thread launch of the NT = 256 body run past the tile
 than the body is compiled for is what puts a z kernel out of bounds.
 """
 import os
+import re
 import subprocess
 
 import pytest
@@ -33,7 +34,9 @@ def test_zpass_indices_in_bounds_at_nt(checker):
     r = subprocess.run([checker], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "checks passed (NT = 256, 256 threads per workgroup)" in r.stdout
-    assert r.stdout.count("in bounds") == 2 * 13
+    with open(HEADERS[2]) as f:
+        sizes = re.search(r"#define FHIP_SIZES\(X\)(.*?)\n#define", f.read(), re.S).group(1)
+    assert r.stdout.count("in bounds") == 2 * len(re.findall(r"X\(\d+\)", sizes))
 
 
 def test_oversized_launch_leaves_the_tile(checker):
