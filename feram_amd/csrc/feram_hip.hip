@@ -115,8 +115,10 @@ int run_zpair(fhip_ctx *ctx, const ZArgs<T> &a)
     if (shm > 96 * 1024)
         return fail(ctx, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
     switch (ctx->Lz) {
-#define FHIP_CASE(n) \
-    case n: return fhip_stage::zpair<T, n, G2>(ctx, a, shm);
+#define FHIP_CASE(n)                                                                       \
+    case n:                                                                                \
+        return ctx->c.zsp ? fhip_stage::zpair<T, n, G2, ZG_DDZ>(ctx, a, shm)               \
+                          : fhip_stage::zpair<T, n, G2, ZG_DD>(ctx, a, shm);
         FHIP_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -153,7 +155,7 @@ int avg_add(fhip_ctx *ctx)
 }
 
 // ---- one force evaluation (mirror_force_mirror) inside a step ---------------
-int nforward(const fhip_ctx *ctx) { return ctx->c.acoustic ? 12 : 9; }
+int nforward(const fhip_ctx *ctx) { return 9 + (ctx->c.acoustic ? 3 : 0) + (ctx->c.zsp ? 3 : 0); }
 
 enum StepKind { SK_FORCE = 0, SK_NP = 1, SK_LF = 2, SK_VS = 3 };
 
@@ -383,6 +385,10 @@ void fill_consts(fhip_ctx *ctx)
     c.a0 = p.a0;
     c.Zstar = p.Z_star[0];
     c.Z2 = p.Z_star[0] * p.Z_star[0];
+    for (int s = 0; s < 3; ++s) c.Zs[s] = p.Z_star[s];
+    // species with different Z*: Z*(R) u gets its own transforms
+    c.zsp = (p.Z_star[1] != p.Z_star[0] || p.Z_star[2] != p.Z_star[0]) ? 1 : 0;
+    c.zu_slot = c.acoustic ? 12 : 9;
     for (int s = 0; s < 3; ++s) {
         c.kappa2[s] = p.P_kappa2[s];
         c.fa2g[s] = 4 * p.P_alpha[s] + 2 * p.P_gamma[s];
@@ -603,8 +609,6 @@ int set_cells_impl(fhip_ctx *ctx, const int32_t *relaxor, const int32_t *sum_p, 
                 const size_t hi = padded_index(ctx, 0, 1, x, y, z);
                 int s = relaxor ? relaxor[hi] - 1 : 0;
                 if (s < 0 || s > 2) return fail(ctx, FHIP_ERR_ARG, "relaxor species must be 1..3");
-                if (p.Z_star[s] != p.Z_star[0])
-                    return fail(ctx, FHIP_ERR_UNSUPPORTED, "species with different Z_star are not supported");
                 bool counted = sum_p ? sum_p[hi] != 0 : true;
                 bool frozen = false;
                 if (minv) {
@@ -905,7 +909,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     ctx->Ks = ctx->Mh;
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     fill_consts(ctx);
-    ctx->nslots = ctx->c.acoustic ? 12 : 9;
+    ctx->nslots = nforward(ctx);
     ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
     ctx->xfblocks = (ctx->Ly * ctx->Lz + xfwd_tile_rows(ctx->Lx) - 1) / xfwd_tile_rows(ctx->Lx);
     for (int t = 0; t < 2; ++t) {
@@ -1282,9 +1286,11 @@ int fhip_force_static(const fhip_params *p, const double *long_range, const doub
     int rc = check_params(&ctx, p);
     if (rc != FHIP_OK) return static_fail(err, errlen, rc, ctx.err);
     if (p->precision != FHIP_FP64 || p->mass_acou[0] > 0.0 || p->modulation_constant != 0.0 ||
-        (p->bulk_or_film != FHIP_BULK && p->bulk_or_film != FHIP_E110))
+        (p->bulk_or_film != FHIP_BULK && p->bulk_or_film != FHIP_E110) || p->Z_star[1] != p->Z_star[0] ||
+        p->Z_star[2] != p->Z_star[0])
         return static_fail(err, errlen, FHIP_ERR_UNSUPPORTED,
-                           "fhip_force_static: FP64, bulk or e110, optimised acoustic strain, no modulation only");
+                           "fhip_force_static: FP64, bulk or e110, optimised acoustic strain, no modulation, "
+                           "one Z* only");
     ctx.p = *p;
     ctx.Lx = p->Lx; ctx.Ly = p->Ly; ctx.Lz = p->Lz;
     ctx.Kx = p->Lx / 2 + 1;

@@ -511,7 +511,10 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
             return bfail(b, FHIP_ERR_UNSUPPORTED, "every replica of a batch uses the same method");
         if (!batch_size_ok(c->Lx) || !batch_size_ok(c->Ly) || !batch_size_ok(c->Lz))
             return bfail(b, FHIP_ERR_UNSUPPORTED,
-                         "batched supercell edges are 4 6 8 12 16 24 32 48 64 (larger ones fill the device alone)");
+                         "batched supercell edges are 4 6 8 12 16 20 24 32 40 48 60 64 (larger ones fill the device "
+                         "alone)");
+        if (c->c.zsp)
+            return bfail(b, FHIP_ERR_UNSUPPORTED, "species with different Z* are stepped by the per-context calls");
         b->ctxs.push_back(c);
         BClass *k = nullptr;
         for (auto &q : b->classes)

@@ -271,7 +271,7 @@ template <typename T, int n> int xfwd(fhip_ctx *ctx, const XfwdArgs &a);
 template <typename T, int n> int xinv(fhip_ctx *ctx, const XinvArgs &a, int nf);
 template <typename T, int n> int ypass(fhip_ctx *ctx, int sign, int slot0, int nf, hipStream_t s);
 template <typename T, int n> int zplain(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t s, size_t shm);
-template <typename T, int n, int G2> int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm);
+template <typename T, int n, int G2, int DDG> int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm);
 
 // replica batches: one launch over every replica of a size class
 template <typename T, int n> void b_xfwd(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int phase, int mode);

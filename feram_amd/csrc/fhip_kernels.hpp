@@ -15,6 +15,10 @@
 //   0..2  u_x,u_y,u_z        -> (S + Z*^2 L) u   (dipole-dipole force before 1/N)
 //   3..8  uu_xx..uu_xy       -> eta(k) = k (x) w(k)  (inhomogeneous strain)
 //   9..11 w (acoustic MD)    -> dV/dw(k)
+// With species of different Z* the dipole-dipole group splits as the
+// reference does: u -> S u in slots 0..2 and Z*(R) u -> L (Z* u) in the three
+// slots from zu_slot (9, or 12 with acoustic MD), multiplied by Z*(R) after
+// the inverse transform.
 // One step is: xfwd (drift + r2c along x) -> y forward -> z pass (forward,
 // k-space stage, inverse) -> scalar stage -> y inverse -> xinv (c2r along x,
 // local forces, kick).  Reductions go through per-block partial sums that a
@@ -25,7 +29,7 @@
 namespace fhip {
 
 constexpr int NT = 256;          // threads per workgroup in every kernel
-constexpr int MAX_SLOTS = 12;    // spectral slots
+constexpr int MAX_SLOTS = 15;    // spectral slots
 enum PartialSlot {
     P_ESR = 0, P_ELR, P_EIS, P_EIC, P_EIM, P_EUNH, P_EDE, P_KE1, P_KE2, P_KEA, P_KEPK,
     P_AVG,                       // 12 slots: sums of u, u^2, Z* u, (Z* u)^2 over the counted cells
@@ -41,7 +45,10 @@ struct Consts {
     int has_eloc;        // per-cell local field present
     int modulation;      // modulation_constant != 0 with a .modulation field
     int mir_lo, mir_src; // film/epit: layers iz >= mir_lo mirror layer mir_src - iz (mir_lo = Lz: none)
+    int zsp;             // species with different Z*: Z*(R) u has its own transforms (dipoFFT.F:35-44)
+    int zu_slot;         // spectral slot of (Z* u)_x when zsp
     double Ninv, dt, dt_2, a0, Z2, Zstar;
+    double Zs[3];        // Z* per species (read_relaxor.F:35)
     double kappa2[3], fa2g[3], tg[3], sk[3], tk2[3], fk[3], k4x8[3];
     double alpha[3], gamma[3], k1[3], k2[3], k3[3], k4[3];
     double minv[3], mwinv[3];
@@ -258,7 +265,7 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
     if (a.phase == XP_MIRROR && row0 + RB <= mrow) return;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
-    double u[CPT][3], w[CPT][3];
+    double u[CPT][3], w[CPT][3], zs[CPT];
     bool ok[CPT];
     size_t cell[CPT];
     // every cell's code, displacement and momentum are loaded before any
@@ -292,6 +299,7 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
         const int e = tid + i * NT;
         const int r = e / LX, x = e - r * LX;
         if (a.mode == XF_GENERIC) continue;
+        zs[i] = 0.0;
         if (!ok[i]) {
             u[i][0] = u[i][1] = u[i][2] = 0.0;
             w[i][0] = w[i][1] = w[i][2] = 0.0;
@@ -300,6 +308,7 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
         const size_t cc = cell[i];
         const unsigned char code = pre ? cpre[i] : f.code[cc];
         const int spc = code & 3;
+        zs[i] = c.Zs[spc];
         const double minv = (code & 4) ? 0.0 : c.minv[spc];
         if (a.phase == XP_MIRROR) {
             const int rr = row0 + r, iz = rr / c.Ly, iy = rr - iz * c.Ly;
@@ -381,9 +390,12 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
                         const double ua = (m == 0 || m == 5) ? u[i][0] : (m == 1 || m == 3) ? u[i][1] : u[i][2];
                         const double ub = (m == 0 || m == 4) ? u[i][0] : (m == 1 || m == 5) ? u[i][1] : u[i][2];
                         v = ua * ub;
-                    } else {
+                    } else if (c.acoustic && fi < 12) {
                         const int q = fi - 9;
                         v = q == 0 ? w[i][0] : (q == 1 ? w[i][1] : w[i][2]);
+                    } else {                                             // Z_u_R (dipoFFT.F:35-43)
+                        const int q = fi - c.zu_slot;
+                        v = zs[i] * (q == 0 ? u[i][0] : (q == 1 ? u[i][1] : u[i][2]));
                     }
                     if (h == 0) va = v; else vb = v;
                 }
@@ -560,7 +572,7 @@ __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0
 // kx = Lx/2 plane, ky = Ly/2, kz = Lz/2) the strain stage is evaluated at
 // both and averaged, elsewhere once with the partner's energy share.
 // ---------------------------------------------------------------------------
-enum ZGroup { ZG_PLAIN = 0, ZG_DD = 1, ZG_STRAIN = 2, ZG_ACOU = 3 };
+enum ZGroup { ZG_PLAIN = 0, ZG_DD = 1, ZG_STRAIN = 2, ZG_ACOU = 3, ZG_DDZ = 4 };
 
 template <typename T>
 struct ZArgs {
@@ -645,6 +657,44 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
             es += X[q].x * fs[q].x + X[q].y * fs[q].y;
             el += X[q].x * fl[q].x + X[q].y * fl[q].y;
             put(q, cpx<KT>{fs[q].x + fl[q].x, fs[q].y + fl[q].y});
+        }
+        acc0 += 2.0 * wm * (double)es;
+        acc1 += 2.0 * wm * (double)el;
+    } else if constexpr (GROUP == ZG_DDZ) {
+        // species with different Z*: S(k) u^ and L(k) (Z* u)^ separately
+        // (dVddi-dipole-dipole.F:20-30, 40-50), E_sr and E_lr by Parseval
+        const cpx<double> ex = a.csx[kx], ey = a.csy[ky], ez = a.csz[kz];
+        const KT cx = KT(ex.x), cy = KT(ey.x), cz = KT(ez.x), sx = KT(ex.y), sy = KT(ey.y), szz = KT(ez.y);
+        const KT j1 = KT(c.j[0]), j2 = KT(c.j[1]), j3 = KT(c.j[2]), j4 = KT(c.j[3]), j5 = KT(c.j[4]),
+                 j6 = KT(c.j[5]), j7 = KT(c.j[6]);
+        const KT k2t = KT(2.0 * c.kappa2[0]);
+        const KT cxyz = cx * cy * cz;
+        KT S[6];
+        S[0] = k2t + 2 * j2 * cx + 2 * j1 * (cy + cz) + 4 * j3 * cx * (cy + cz) + 4 * j4 * cy * cz + 8 * j6 * cxyz;
+        S[1] = k2t + 2 * j2 * cy + 2 * j1 * (cz + cx) + 4 * j3 * cy * (cz + cx) + 4 * j4 * cz * cx + 8 * j6 * cxyz;
+        S[2] = k2t + 2 * j2 * cz + 2 * j1 * (cx + cy) + 4 * j3 * cz * (cx + cy) + 4 * j4 * cx * cy + 8 * j6 * cxyz;
+        S[3] = -4 * j5 * sy * szz - 8 * j7 * sy * szz * cx;
+        S[4] = -4 * j5 * szz * sx - 8 * j7 * szz * sx * cy;
+        S[5] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz;
+        KT L[6];
+        const T* lt = a.Ltab + kk * 6;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) L[m] = KT(lt[m]);
+        auto sym = [&](const KT (&M)[6], const cpx<KT>* v, cpx<KT>* o) {
+            o[0] = {M[0] * v[0].x + M[5] * v[1].x + M[4] * v[2].x, M[0] * v[0].y + M[5] * v[1].y + M[4] * v[2].y};
+            o[1] = {M[5] * v[0].x + M[1] * v[1].x + M[3] * v[2].x, M[5] * v[0].y + M[1] * v[1].y + M[3] * v[2].y};
+            o[2] = {M[4] * v[0].x + M[3] * v[1].x + M[2] * v[2].x, M[4] * v[0].y + M[3] * v[1].y + M[2] * v[2].y};
+        };
+        cpx<KT> fs[3], fl[3];
+        sym(S, X, fs);
+        sym(L, X + 3, fl);
+        KT es = KT(0), el = KT(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            es += X[q].x * fs[q].x + X[q].y * fs[q].y;
+            el += X[3 + q].x * fl[q].x + X[3 + q].y * fl[q].y;
+            put(q, fs[q]);
+            put(3 + q, fl[q]);
         }
         acc0 += 2.0 * wm * (double)es;
         acc1 += 2.0 * wm * (double)el;
@@ -914,8 +964,8 @@ struct ZMap {
 // barriers) and the LDS row stride that follows.
 template <int GROUP>
 struct ZGroupShape {
-    static constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
-    static constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_PLAIN) ? 0 : 3;
+    static constexpr int NF = (GROUP == ZG_DD) ? 3 : (GROUP == ZG_STRAIN || GROUP == ZG_DDZ ? 6 : (GROUP == ZG_ACOU ? 9 : 1));
+    static constexpr int F0 = (GROUP == ZG_DD || GROUP == ZG_DDZ || GROUP == ZG_PLAIN) ? 0 : 3;
     static constexpr int FB = (GROUP == ZG_PLAIN) ? 1 : (GROUP == ZG_STRAIN ? ZSTRAIN_FB : 3);
 };
 
@@ -954,6 +1004,9 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
     const int nlines = c.Mx * c.Ly;
     const size_t zstride = (size_t)nlines;
     for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
+    // spectral slot of the group's field fi (the split dipole-dipole group
+    // reads u^ from slots 0..2 and (Z* u)^ from zu_slot on)
+    auto slot_of = [&](int fi) { return (GROUP == ZG_DDZ && fi >= 3) ? c.zu_slot + fi - 3 : F0 + fi; };
 
     using Map = ZMap<NT, TB, LZ>;
     // element e of the tile in HBM order: LDS slot and HBM offset
@@ -1012,7 +1065,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
         // field by field through LDS and the results stay in registers
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi) {
-            const cpx<T>* __restrict__ src = sp.s[F0 + fi];
+            const cpx<T>* __restrict__ src = sp.s[slot_of(fi)];
 #pragma unroll
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
@@ -1155,7 +1208,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
 #pragma unroll 1
         for (int j = tid; j < nsp; j += NT) {
             const int ord = j / LZ, t = j - ord * LZ;
-            const KPlan pl = kplan<GROUP == ZG_DD>(c, lky[b0 + ord * c.Mx], t);
+            const KPlan pl = kplan<GROUP == ZG_DD || GROUP == ZG_DDZ>(c, lky[b0 + ord * c.Mx], t);
             cpx<KT> X[NF], O[NF];
 #pragma unroll
             for (int q = 0; q < NF; ++q) X[q] = ev_in(cpx<KT>{KT(side[j * NF + q].x), KT(side[j * NF + q].y)}, pl.m0);
@@ -1173,7 +1226,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
 #pragma unroll 1
         for (int j = tid; j < nsp; j += NT) {
             const int ord = j / LZ, t = j - ord * LZ;
-            const KPlan pl = kplan<GROUP == ZG_DD>(c, lky[b0 + ord * c.Mx], t);
+            const KPlan pl = kplan<GROUP == ZG_DD || GROUP == ZG_DDZ>(c, lky[b0 + ord * c.Mx], t);
             if (pl.nev < 2) continue;
             cpx<KT> X[NF], O[NF];
 #pragma unroll
@@ -1215,7 +1268,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
                 fft_lds_span<T, LZ, FB * TB, NT, +1, STB, PPT, LZ / PPT>(buf, tws, tid);
 #pragma unroll
                 for (int f = 0; f < FB; ++f) {
-                    cpx<T>* __restrict__ dst = sp.s[F0 + fb + f];
+                    cpx<T>* __restrict__ dst = sp.s[slot_of(fb + f)];
                     cpx<T> wv[PPT];
 #pragma unroll
                     for (int q = 0; q < PPT; ++q) wv[q] = buf[(zt0 + q * S) * STB + f * TB + zb];
@@ -1254,7 +1307,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
                 fft_lds<T, LZ, FB * TB, NT, +1, STB>(buf, tws, tid);
 #pragma unroll
                 for (int f = 0; f < FB; ++f) {
-                    cpx<T>* __restrict__ dst = sp.s[F0 + fb + f];
+                    cpx<T>* __restrict__ dst = sp.s[slot_of(fb + f)];
 #pragma unroll
                     for (int i = 0; i < PPT; ++i) {
                         const int e = tid + i * NT;
@@ -1268,7 +1321,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             }
         }
         const int nb = a.maxblocks;
-        if constexpr (GROUP == ZG_DD) {
+        if constexpr (GROUP == ZG_DD || GROUP == ZG_DDZ) {
             double r[2] = {acc0, acc1};
             block_sums<2>(r, red);
             if (tid == 0) {
@@ -1309,7 +1362,7 @@ constexpr int zpair_buf_size()
     return zbuf_size<T, LZ, ZG_DD>() > zbuf_size<T, LZ, G2>() ? zbuf_size<T, LZ, ZG_DD>() : zbuf_size<T, LZ, G2>();
 }
 
-template <typename T, int LZ, int G2>
+template <typename T, int LZ, int G2, int DDG = ZG_DD>
 __device__ __forceinline__ void zpair_body(const Consts& c, const Spectra<T>& sp, const ZArgs<T>& a, Scalars* sc,
                                            const cpx<T>* twz, int bx)
 {
@@ -1317,19 +1370,20 @@ __device__ __forceinline__ void zpair_body(const Consts& c, const Spectra<T>& sp
     if (bx & 1) {
         zpass_body<T, LZ, G2>(c, sp, a, sc, twz, bx, lds);
         __syncthreads();
-        zpass_body<T, LZ, ZG_DD>(c, sp, a, sc, twz, bx, lds);
+        zpass_body<T, LZ, DDG>(c, sp, a, sc, twz, bx, lds);
     } else {
-        zpass_body<T, LZ, ZG_DD>(c, sp, a, sc, twz, bx, lds);
+        zpass_body<T, LZ, DDG>(c, sp, a, sc, twz, bx, lds);
         __syncthreads();
         zpass_body<T, LZ, G2>(c, sp, a, sc, twz, bx, lds);
     }
 }
 
-template <typename T, int LZ, int G2>
+// DDG: ZG_DD (one Z*, folded into L(k)) or ZG_DDZ (species with different Z*)
+template <typename T, int LZ, int G2, int DDG = ZG_DD>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(G2 == ZG_ACOU ? 1 : 2)))
 void k_zpair(Consts c, Spectra<T> sp, ZArgs<T> a, Scalars* sc, const cpx<T>* __restrict__ twz)
 {
-    zpair_body<T, LZ, G2>(c, sp, a, sc, twz, blockIdx.x);
+    zpair_body<T, LZ, G2, DDG>(c, sp, a, sc, twz, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1346,6 +1400,8 @@ struct XinvArgs {
     double* gen[12];             // XI_GENERIC: real outputs (unnormalised c2r)
 };
 
+// c2r along x of NF spectral slots into real device fields (XI_GENERIC,
+// set-up and output work), unnormalised: two fields per complex transform.
 template <typename T, int LX, int NF>
 __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XinvArgs& a,
                                              const Scalars* __restrict__ sc,
@@ -1354,75 +1410,25 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
     constexpr int RB = tile_of<LX>::value;
     constexpr int ST = RB + FHIP_XPAD;
     constexpr int CELLS = RB * LX;
-    constexpr int CPT = (CELLS + NT - 1) / NT;
     constexpr int MX = LX / 2;
+    constexpr int HP = RB * MX;
     __shared__ cpx<T> buf[LX * ST];
     __shared__ cpx<T> tws[LX];
-    __shared__ double red[NT];
     const int tid = threadIdx.x;
     const int nrows = c.Ly * c.Lz;
     const int row0 = bx * RB;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
-
-    // packed half-spectrum points of this block's rows: all NF fields are
-    // requested from HBM up front, so that one load latency covers the block
-    constexpr int HP = RB * MX;
-    constexpr int HPT = (HP + NT - 1) / NT;
-    cpx<T> h[NF][HPT];
-#pragma unroll
-    for (int fi = 0; fi < NF; ++fi) {
-#pragma unroll
-        for (int i = 0; i < HPT; ++i) {
-            const int e = tid + i * NT;
-            h[fi][i] = cpx<T>{T(0), T(0)};
-            if (HP % NT != 0 && e >= HP) continue;
-            const int r = e / MX;
-            if (row0 + r < nrows) h[fi][i] = sp.s[fi][(size_t)row0 * MX + e];
-        }
-    }
-
-    T acc[CPT][NF];
-    // FP64 spectra: the cells' displacement, the momenta the kick reads and
-    // the cell codes are requested before the last transform, which covers
-    // their latency (FP64 128^3: 94 -> 91 us).  FP32 reads them in the cell
-    // loop: the 48 extra registers would cost a wave per SIMD there
-    // (FP32 128^3: 58 -> 71 us with the prefetch).
-    constexpr bool PREF = NF >= 9 && sizeof(T) == 8;
-    constexpr int CPL = PREF ? CPT : 1;
-    double up[CPL][3], pp[CPL][3];
-    unsigned char cp[CPL];
-    const bool kick = a.mode == XI_NP || a.mode == XI_LF || a.mode == XI_VS;
-#pragma unroll
     for (int f0 = 0; f0 < NF; f0 += 2) {
         const bool two = (f0 + 1 < NF);
-        if (PREF && f0 + 2 >= NF && a.mode != XI_GENERIC) {
-#pragma unroll
-            for (int i = 0; i < CPL; ++i) {
-                const int e = tid + i * NT;
-                const int r = e / LX, x = e - r * LX;
-                cp[i] = 0;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) up[i][q] = pp[i][q] = 0.0;
-                if ((CELLS % NT == 0 || e < CELLS) && row0 + r < nrows) {
-                    const size_t cc = (size_t)(row0 + r) * LX + x;
-                    cp[i] = f.code[cc];
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) up[i][q] = f.u[q][cc];
-                    if (kick) {
-#pragma unroll
-                        for (int q = 0; q < 3; ++q) pp[i][q] = (a.mode == XI_NP ? f.pk[q] : f.p[q])[cc];
-                    }
-                }
-            }
-        }
         // Z = A + i B with both half spectra completed Hermitian; kx = 0
-        // holds the real A[0] + i A[Lx/2].
-#pragma unroll
-        for (int i = 0; i < HPT; ++i) {
-            const int e = tid + i * NT;
-            if (HP % NT != 0 && e >= HP) continue;
+        // holds the real A[0] + i A[Lx/2]
+        for (int e = tid; e < HP; e += NT) {
             const int r = e / MX, kx = e - r * MX;
-            const cpx<T> A = h[f0][i], B = two ? h[f0 + 1][i] : cpx<T>{T(0), T(0)};
+            cpx<T> A{T(0), T(0)}, B{T(0), T(0)};
+            if (row0 + r < nrows) {
+                A = sp.s[f0][(size_t)row0 * MX + e];
+                if (two) B = sp.s[f0 + 1][(size_t)row0 * MX + e];
+            }
             if (kx == 0) {
                 buf[r] = cpx<T>{A.x, B.x};
                 buf[MX * ST + r] = cpx<T>{A.y, B.y};
@@ -1433,174 +1439,27 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
         }
         __syncthreads();
         fft_lds<T, LX, RB, NT, +1, ST>(buf, tws, tid);
-#pragma unroll
-        for (int i = 0; i < CPT; ++i) {
-            const int e = tid + i * NT;
-            if (CELLS % NT != 0 && e >= CELLS) continue;
+        for (int e = tid; e < CELLS; e += NT) {
             const int r = e / LX, x = e - r * LX;
+            if (row0 + r >= nrows) continue;
+            const size_t cc = (size_t)(row0 + r) * LX + x;
             const cpx<T> z = buf[x * ST + r];
-            acc[i][f0] = z.x;
-            if (two) acc[i][f0 + 1] = z.y;
+            a.gen[f0][cc] = (double)z.x;
+            if (two) a.gen[f0 + 1][cc] = (double)z.y;
         }
         __syncthreads();
     }
-
-    double eunh = 0.0, ede = 0.0, ke1 = 0.0, ke2 = 0.0, kea = 0.0;
-#pragma unroll
-    for (int i = 0; i < CPT; ++i) {
-        const int e = tid + i * NT;
-        if (CELLS % NT != 0 && e >= CELLS) continue;
-        const int r = e / LX, x = e - r * LX;
-        if (row0 + r >= nrows) continue;
-        const size_t cc = (size_t)(row0 + r) * LX + x;
-        if (a.mode == XI_GENERIC) {
-#pragma unroll
-            for (int q = 0; q < NF; ++q) a.gen[q][cc] = (double)acc[i][q];
-            continue;
-        }
-        if constexpr (NF >= 9) {
-            // the cell's state: prefetched (FP64) or loaded here (FP32)
-            auto u_of = [&](int q) -> double { if constexpr (PREF) return up[i][q]; else return f.u[q][cc]; };
-            auto p_of = [&](int q) -> double {
-                if constexpr (PREF) return pp[i][q]; else return (a.mode == XI_NP ? f.pk[q] : f.p[q])[cc];
-            };
-            unsigned char code;
-            if constexpr (PREF) code = cp[i]; else code = f.code[cc];
-            const int s = code & 3;
-            const bool counted = (code & 8) != 0;
-            // film/epit: the dipole force and kick of a mirrored cell come from
-            // its source cell (mirror_force_mirror.F:80-84, k_mirror_kick)
-            const bool film = c.mir_lo < c.Lz;
-            const bool mirrored = film && row0 + r >= c.mir_lo * c.Ly;
-            const double minv = (code & 4) ? 0.0 : c.minv[s];
-            double u[3], F[3];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                u[q] = u_of(q);
-                F[q] = (double)acc[i][q] * c.Ninv;                    // dVddi-dipole-dipole.F:34-38
-            }
-            // on-site anharmonic terms (dVddi-unharmonic.F:34-65)
-            const double d0 = u[0] * u[0], d1 = u[1] * u[1], d2 = u[2] * u[2];
-            const double d[3] = {d0, d1, d2};
-            const double ss = d0 + d1 + d2;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const int q1 = (q + 1) % 3, q2 = (q + 2) % 3;
-                F[q] = F[q] + u[q] * 2 * (c.kappa2[s] - c.kappa2[0]);
-                F[q] = F[q] + u[q] * (c.fa2g[s] * ss - c.tg[s] * d[q]);
-                F[q] = F[q] + u[q] * (c.sk[s] * (ss * ss) + c.k4x8[s] * (ss * ss * ss)
-                                      - c.tk2[s] * (d[q] * d[q]) - c.fk[s] * d[q1] * d[q2]);
-            }
-            eunh += (c.kappa2[s] - c.kappa2[0]) * ss + c.alpha[s] * (ss * ss)
-                  + c.gamma[s] * (d0 * d1 + d1 * d2 + d2 * d0) + c.k1[s] * (ss * ss * ss)
-                  + c.k2[s] * ((d0 * d0) * (d1 + d2) + (d1 * d1) * (d2 + d0) + (d2 * d2) * (d0 + d1))
-                  + c.k3[s] * d0 * d1 * d2 + c.k4[s] * ((ss * ss) * (ss * ss));
-            // external and local field (mirror_force_mirror.F:39-65)
-            if (c.efield) {
-                double E[3];
-                if (mirrored && sc->e_mir != 0.0) {                     // initialize_E_field.F:68-71
-                    const int rr = row0 + r, iz = rr / c.Ly, iy = rr - iz * c.Ly;
-                    const size_t cs = ((size_t)(c.mir_src - iz) * c.Ly + iy) * LX + x;
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) {
-                        const double Es = c.has_eloc ? f.eloc[q][cs] + sc->E_add[q] : sc->E_add[q];
-                        E[q] = q < 2 ? -Es : Es;
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
-                }
-#pragma unroll
-                for (int q = 0; q < 3; ++q) F[q] = F[q] - c.Zstar * E[q];
-                ede += c.Zstar * (u[0] * E[0] + u[1] * E[1] + u[2] * E[2]);
-            }
-            // homogeneous coupling (dVddi-homo-coupling.F:28-44)
-            {
-                const double* eh = sc->e_homo;
-                F[0] = F[0] + 2 * eh[0] * u[0] + eh[4] * u[2] + eh[5] * u[1];
-                F[1] = F[1] + 2 * eh[1] * u[1] + eh[3] * u[2] + eh[5] * u[0];
-                F[2] = F[2] + 2 * eh[2] * u[2] + eh[3] * u[1] + eh[4] * u[0];
-            }
-            // inhomogeneous coupling (dVddi_inho_coupling.F:54-82)
-            {
-                double ei[6];
-#pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    double sum = 0.0;
-#pragma unroll
-                    for (int jj = 0; jj < 6; ++jj) sum = sum + (double)acc[i][3 + jj] * c.Ninv * c.hc2[jj + 6 * m];
-                    ei[m] = sum;
-                }
-                F[0] = F[0] + 2 * ei[0] * u[0] + ei[4] * u[2] + ei[5] * u[1];
-                F[1] = F[1] + 2 * ei[1] * u[1] + ei[3] * u[2] + ei[5] * u[0];
-                F[2] = F[2] + 2 * ei[2] * u[2] + ei[3] * u[1] + ei[4] * u[0];
-            }
-            // kicks
-            if (mirrored) {
-            } else if (a.mode == XI_NP) {
-                const double dt2s = c.dt_2 * sc->s;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double p1 = p_of(q) - dt2s * F[q];                 // nose-poincare.F:66-71
-                    const double p2 = p1 - dt2s * F[q];                      // nose-poincare.F:24-29 (next step)
-                    if (counted) { ke1 += p1 * p1 * minv; ke2 += p2 * p2 * minv; }
-                    f.pk[q][cc] = p2;
-                    if (a.materialize) f.p[q][cc] = p1;
-                }
-            } else if (a.mode == XI_LF) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double pq = p_of(q) - c.dt * F[q];                 // leapfrog.F:134-139
-                    if (counted) ke1 += pq * pq * minv;
-                    f.p[q][cc] = pq;
-                }
-            } else if (a.mode == XI_VS) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double pq = counted ? p_of(q) - c.dt * F[q] : 0.0;      // velocity_scaling.F:62-76
-                    if (counted) ke1 += pq * pq * minv;
-                    f.p[q][cc] = pq;
-                }
-            }
-            if ((a.materialize || a.mode == XI_FORCE || film) && !mirrored) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) f.F[q][cc] = F[q];
-            }
-            if constexpr (NF == 12) {
-                const double mwinv = c.mwinv[s];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double g = (double)acc[i][9 + q] * c.Ninv;          // dVdac.F:56-62
-                    if (a.mode == XI_LF || a.mode == XI_VS) {
-                        const double pq = f.pw[q][cc] - c.dt * g;
-                        kea += pq * pq * mwinv;
-                        f.pw[q][cc] = pq;
-                    }
-                    if (a.materialize || a.mode == XI_FORCE) f.dVdw[q][cc] = g;
-                }
-            }
-        }
-    }
-    if (a.mode == XI_GENERIC) return;
-    const int nb = a.maxblocks;
-    double r[5] = {eunh, ede, ke1, ke2, kea};
-    block_sums<5>(r, red);
-    if (tid == 0) {
-        a.partials[P_EUNH * nb + bx] = r[0];
-        a.partials[P_EDE * nb + bx] = r[1];
-        a.partials[P_KE1 * nb + bx] = r[2];
-        a.partials[P_KE2 * nb + bx] = r[3];
-        a.partials[P_KEA * nb + bx] = r[4];
-    }
 }
 
-// xinv of a step (NF = 9 inverse fields, 12 with acoustic MD): the field
+// xinv of a step (NF = 9 inverse fields, plus 3 with acoustic MD and 3 with
+// species of different Z*): the field
 // pairs are transformed PB at a time side by side in LDS (PB * RB lines, so a
 // 128-point transform takes radix-8 passes with every thread busy) and each
 // transformed value is folded into the cell's force as it comes out of LDS:
 // only the force, the cell's state and one batch of spectra are live in
-// registers (FP64 128^3: 3 waves per SIMD where holding every field's values
-// takes 2).  The force is the sum of the same terms as dVddi-dipole-dipole.F,
+// registers, and the next batch's spectra load while this one transforms
+// (FP64 128^3: 90 -> 82 us against holding every field's values).  The force
+// is the sum of the same terms as dVddi-dipole-dipole.F,
 // dVddi_inho_coupling.F, dVddi-unharmonic.F, mirror_force_mirror.F:39-65 and
 // dVddi-homo-coupling.F, the inhomogeneous coupling added field by field.
 template <typename T, int LX, int NF>
@@ -1608,7 +1467,7 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
                                                const XinvArgs& a, const Scalars* __restrict__ sc,
                                                const cpx<T>* twx, int bx)
 {
-    static_assert(NF == 9 || NF == 12, "step fields");
+    static_assert(NF == 9 || NF == 12 || NF == 15, "step fields");
     constexpr int RB = tile_of<LX>::value;          // rows per workgroup
     constexpr int PB = 2;                           // field pairs per LDS transform
     constexpr int NL = PB * RB;                     // lines of a full batch
@@ -1650,7 +1509,7 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
             for (int q = 0; q < 3; ++q) u[i][q] = f.u[q][cell[i]];
         }
     }
-    double g[NF == 12 ? CPT : 1][3];                // acoustic dV/dw (dVdac.F:56-62)
+    double g[NF >= 12 ? CPT : 1][3];                // acoustic dV/dw (dVdac.F:56-62)
 
     cpx<T> h[PB][2][HPT];
     auto load_batch = [&](int b) {
@@ -1674,6 +1533,10 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
         v = v * c.Ninv;
         if (fi < 3) {
             F[i][fi] = F[i][fi] + v;                                        // dVddi-dipole-dipole.F:34-38
+        } else if (fi >= 9 && c.zsp && fi >= c.zu_slot) {
+            // long-range force of species with different Z*: Z*(R) (L (Z* u)^)(R) / N
+            // (dVddi-dipole-dipole.F:52-62)
+            F[i][fi - c.zu_slot] = F[i][fi - c.zu_slot] + c.Zs[code[i] & 3] * v;
         } else if (fi < 9) {
             // eta_jj(R) / N through homogeneous_coupling_2 (dVddi_inho_coupling.F:54-82)
             const int jj = fi - 3;
@@ -1683,7 +1546,7 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
             F[i][0] = F[i][0] + (2 * e[0] * u[i][0] + e[4] * u[i][2] + e[5] * u[i][1]);
             F[i][1] = F[i][1] + (2 * e[1] * u[i][1] + e[3] * u[i][2] + e[5] * u[i][0]);
             F[i][2] = F[i][2] + (2 * e[2] * u[i][2] + e[3] * u[i][1] + e[4] * u[i][0]);
-        } else if constexpr (NF == 12) {
+        } else if constexpr (NF >= 12) {
             g[i][fi - 9] = v;
         }
     };
@@ -1792,9 +1655,10 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
 #pragma unroll
                 for (int q = 0; q < 3; ++q) E[q] = c.has_eloc ? f.eloc[q][cc] + sc->E_add[q] : sc->E_add[q];
             }
+            const double zs = c.Zs[s];
 #pragma unroll
-            for (int q = 0; q < 3; ++q) Fi[q] = Fi[q] - c.Zstar * E[q];
-            ede += c.Zstar * (ui[0] * E[0] + ui[1] * E[1] + ui[2] * E[2]);
+            for (int q = 0; q < 3; ++q) Fi[q] = Fi[q] - zs * E[q];
+            ede += zs * (ui[0] * E[0] + ui[1] * E[1] + ui[2] * E[2]);
         }
         // homogeneous coupling (dVddi-homo-coupling.F:28-44)
         {
@@ -1834,7 +1698,8 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
 #pragma unroll
             for (int q = 0; q < 3; ++q) f.F[q][cc] = Fi[q];
         }
-        if constexpr (NF == 12) {
+        if constexpr (NF >= 12) {
+            if (!c.acoustic) continue;
             const double mwinv = c.mwinv[s];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
@@ -2377,11 +2242,12 @@ __device__ __forceinline__ void avg_cells_body(const Consts& c, const Fields& f,
     }
     if (cc < (size_t)c.N && (f.code[cc] & 8)) {
         double u[3];
+        const double zs = c.Zs[f.code[cc] & 3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) u[q] = f.u[q][cc];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const double zu = c.Zstar * u[q];                   // Z_u_R (dipoFFT.F:35-43)
+            const double zu = zs * u[q];                        // Z_u_R (dipoFFT.F:35-43)
             v[q] = u[q];                                        // average_module.F:94-95
             v[3 + q] = u[q] * u[q];
             v[6 + q] = zu;                                      // average_module.F:115-116

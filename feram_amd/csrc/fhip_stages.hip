@@ -76,6 +76,7 @@ int xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
     const cpx<T> *tw = tset<T>(ctx).twx;
     const int nb = ctx->xblocks;
     hipStream_t s = ctx->stream;
+    if (nf == 15) return launch(ctx, K_XINV, [&] { k_xinv<T, n, 15><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
     if (nf == 12) return launch(ctx, K_XINV, [&] { k_xinv<T, n, 12><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
     if (nf == 9) return launch(ctx, K_XINV, [&] { k_xinv<T, n, 9><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
     return launch(ctx, K_XINV, [&] { k_xinv<T, n, 3><<<nb, NT, 0, s>>>(c, f, sp, a, sc, tw); });
@@ -106,7 +107,7 @@ int zplain(fhip_ctx *ctx, const ZArgs<T> &a, hipStream_t s, size_t shm)
     return launch(ctx, K_SETUP, [&] { k_zpass<T, n, ZG_PLAIN><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); }, s);
 }
 
-template <typename T, int n, int G2>
+template <typename T, int n, int G2, int DDG>
 int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm)
 {
     Consts c = ctx->c;
@@ -116,8 +117,8 @@ int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm)
     const int nb = ctx->zblocks[tix<T>()];
     hipStream_t s = ctx->stream;
     const ZArgs<T> za = a;
-    RC(ensure_lds(ctx, (const void *)k_zpair<T, n, G2>, shm));
-    return launch(ctx, K_ZPAIR, [&] { k_zpair<T, n, G2><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
+    RC(ensure_lds(ctx, (const void *)k_zpair<T, n, G2, DDG>, shm));
+    return launch(ctx, K_ZPAIR, [&] { k_zpair<T, n, G2, DDG><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
 }
 
 // ---- replica batches -------------------------------------------------------
@@ -170,8 +171,10 @@ void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize)
     template int xinv<T, n>(fhip_ctx *, const XinvArgs &, int);                             \
     template int ypass<T, n>(fhip_ctx *, int, int, int, hipStream_t);                       \
     template int zplain<T, n>(fhip_ctx *, const ZArgs<T> &, hipStream_t, size_t);           \
-    template int zpair<T, n, ZG_STRAIN>(fhip_ctx *, const ZArgs<T> &, size_t);              \
-    template int zpair<T, n, ZG_ACOU>(fhip_ctx *, const ZArgs<T> &, size_t);
+    template int zpair<T, n, ZG_STRAIN, ZG_DD>(fhip_ctx *, const ZArgs<T> &, size_t);       \
+    template int zpair<T, n, ZG_ACOU, ZG_DD>(fhip_ctx *, const ZArgs<T> &, size_t);         \
+    template int zpair<T, n, ZG_STRAIN, ZG_DDZ>(fhip_ctx *, const ZArgs<T> &, size_t);      \
+    template int zpair<T, n, ZG_ACOU, ZG_DDZ>(fhip_ctx *, const ZArgs<T> &, size_t);
 #define FHIP_INST_B(T, n)                                                                   \
     template void b_xfwd<T, n>(const Rep<T> *, dim3, hipStream_t, bool, int, int);          \
     template void b_ypass<T, n>(const Rep<T> *, dim3, hipStream_t, bool, int);              \

@@ -396,3 +396,68 @@ def test_fp32_modulation_energies():
     e = g.md_prologue()
     assert e["inho_strain"] / p.N == pytest.approx(0.000072004971, rel=1e-6)
     assert e["inho_modulation"] / p.N == pytest.approx(-0.000144009942, rel=1e-6)
+
+
+ZSTAR_SPECIES = [9.956, 7.5, 12.25]
+
+
+def _species_pair(oracle_mod, p, seed, precision=0):
+    """Oracle and device with relaxor species of different Z* (read_relaxor.F:35)."""
+    from feram_amd.system import System
+    rng = np.random.default_rng(seed)
+    relax = rng.integers(1, 4, size=(p.Lz, p.Ly, p.Lx)).astype(np.int32)
+    o = oracle_mod.Oracle(p)
+    o.relaxor[...] = relax
+    o.field("Z_starR")[...] = np.array(p.Z_star)[relax - 1]
+    for s in range(3):
+        o.field("mass_dipo_inv")[relax == s + 1] = 1.0 / p.mass_dipo[s]
+    g = System(p, precision=precision)
+    g.set_cells(relaxor=relax)
+    u = _random(p, seed + 1)
+    o.field("dipoR")[...] = u
+    g.put("dipoR", u)
+    if p.mass_acou[0] > 0:
+        w = 0.02 * rng.standard_normal(u.shape)
+        o.field("acouR")[...] = w
+        g.put("acouR", w)
+    return o, g
+
+
+@pytest.mark.parametrize("L,method,acoustic", [((12, 8, 16), "md", False), ((16, 16, 80), "vs", False),
+                                               ((8, 12, 8), "lf", True)])
+def test_species_zstar_force_and_trajectory(oracle_mod, L, method, acoustic):
+    """Species of different Z*: u and Z*(R) u transformed separately, Z*(R) on
+    the long-range force and on the field term, as the reference does."""
+    p = batio3_zvr(L, method=method, kelvin=300.0, acoustic_mass_amu=46.44 if acoustic else None)
+    p.Z_star = list(ZSTAR_SPECIES)
+    p.mass_dipo = [p.mass_dipo[0], 1.3 * p.mass_dipo[0], 0.7 * p.mass_dipo[0]]
+    p.external_E_field = [0.0008, -0.0005, 0.0015]
+    p.E_field_always_zero = False
+    print(f"[gpu] species Z* L={L} {method} acoustic={acoustic}", flush=True)
+    o, g = _species_pair(oracle_mod, p, 61)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FORCE_TOL
+    _check_energies(eg, eo)
+    if acoustic:
+        assert _normwise(g.get("dVdacR"), o.field("dVdacR")) <= FORCE_TOL
+    e_o = o.md_prologue()
+    g.md_prologue()
+    for _ in range(6):
+        o.step(e_o)
+    eg = g.step(6)
+    for name in ("dipoR", "dipoP", "dVddi"):
+        assert _normwise(g.get(name), o.field(name)) <= TRAJ_TOL, name
+    _check_energies(eg, e_o.as_dict(), TRAJ_TOL)
+    g.close()
+
+
+def test_species_zstar_fp32(oracle_mod):
+    p = batio3_zvr((16, 12, 16))
+    p.Z_star = list(ZSTAR_SPECIES)
+    o, g = _species_pair(oracle_mod, p, 62, precision=1)
+    eo = o.mirror_force_mirror().as_dict()
+    eg = g.mirror_force_mirror()
+    assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FP32_TOL
+    _check_energies(eg, eo, tol=FP32_TOL)
+    g.close()
