@@ -572,6 +572,26 @@ __global__ __launch_bounds__(NT) void k_ypass(Consts c, Spectra<T> sp, int slot0
 // kx = Lx/2 plane, ky = Ly/2, kz = Lz/2) the strain stage is evaluated at
 // both and averaged, elsewhere once with the partner's energy share.
 // ---------------------------------------------------------------------------
+// Loads of data that the step reads once and does not need again soon: the
+// L(k) table (read once per step) and the spectra the x-inverse pass consumes
+// (rewritten by the next step before anyone reads them).  In FP64 the rest of
+// the step's working set (u, the pre-kicked p and the nine spectra, 250 MB at
+// 128^3) is about the size of the device's 256 MB last-level cache, so these
+// loads are nontemporal and leave that set cached (FP64 128^3: x passes
+// 4-5 us faster each).  FP32 state fits the cache whole and keeps ordinary
+// loads, which are faster there.
+template <typename T>
+__device__ __forceinline__ T once_load(const T* p)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> once_load(const cpx<T>* p)
+{
+    return cpx<T>{once_load(&p->x), once_load(&p->y)};
+}
+
 enum ZGroup { ZG_PLAIN = 0, ZG_DD = 1, ZG_STRAIN = 2, ZG_ACOU = 3, ZG_DDZ = 4 };
 
 template <typename T>
@@ -641,7 +661,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         const T* lt = a.Ltab + kk * 6;
         const KT Z2 = KT(c.Z2);
 #pragma unroll
-        for (int m = 0; m < 6; ++m) L[m] = KT(lt[m]) * Z2;
+        for (int m = 0; m < 6; ++m) L[m] = KT(once_load(lt + m)) * Z2;
         const cpx<KT> ux = X[0], uy = X[1], uz = X[2];
         // F = (S + Z^2 L) u ; energies via Parseval
         cpx<KT> fs[3], fl[3];
@@ -679,7 +699,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         KT L[6];
         const T* lt = a.Ltab + kk * 6;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) L[m] = KT(lt[m]);
+        for (int m = 0; m < 6; ++m) L[m] = KT(once_load(lt + m));
         auto sym = [&](const KT (&M)[6], const cpx<KT>* v, cpx<KT>* o) {
             o[0] = {M[0] * v[0].x + M[5] * v[1].x + M[4] * v[2].x, M[0] * v[0].y + M[5] * v[1].y + M[4] * v[2].y};
             o[1] = {M[5] * v[0].x + M[1] * v[1].x + M[3] * v[2].x, M[5] * v[0].y + M[1] * v[1].y + M[3] * v[2].y};
@@ -1523,7 +1543,7 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
                     const int e = tid + i * NT;
                     h[p][k][i] = cpx<T>{T(0), T(0)};
                     if (fi >= NF || (HP % NT != 0 && e >= HP)) continue;
-                    if (row0 + e / MX < nrows) h[p][k][i] = sp.s[fi][(size_t)row0 * MX + e];
+                    if (row0 + e / MX < nrows) h[p][k][i] = once_load(sp.s[fi] + (size_t)row0 * MX + e);
                 }
             }
         }
