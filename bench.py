@@ -21,9 +21,9 @@ the same prologue and two steps from the same doubles in the CPU-baseline leg.
 
 ``roofline`` is the dominant kernel's algorithmic HBM bytes per launch over
 its average launch time, taken with HIP events on the library's stream in a
-second pass of the same K steps right after the timed one; ``roofline.
-copy_GBs`` is a 1 GiB device copy timed in the same run (the attainable HBM
-rate on this box).  ``cpu_baseline`` times the CPU FP64 oracle (OpenMP over
+separate pass of K steps (direct launches) between the warm-up and the timed
+steps; ``roofline.copy_GBs`` is a 1 GiB device copy timed in the same run (the
+attainable HBM rate on this box).  ``cpu_baseline`` times the CPU FP64 oracle (OpenMP over
 the host threads) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -138,13 +138,20 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
     # the first two steps, compared with the oracle's (the CPU-baseline leg)
     e2 = g.nose_poincare(2)
     first = {"e0": e0, "F0": F0, "e2": e2, "u2": g.get("dipoR"), "p2": g.get("dipoP"), "F2": g.get("dVddi")}
-    if warmup > 0:
-        g.nose_poincare(warmup)
 
     def sync():
         g.synchronize()
         sync_extra()
 
+    if warmup > 0:
+        g.nose_poincare(warmup)
+    # per-kernel times: K steps with HIP events around every launch (direct
+    # launches), between the warm-up and the timed steps
+    g.profile(True)
+    g.nose_poincare(steps)
+    sync()
+    stats = g.kernel_stats()
+    g.profile(False)
     sync()
     t0 = time.perf_counter()
     e = g.nose_poincare(steps)
@@ -152,11 +159,6 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
     dt = time.perf_counter() - t0
     if not all(np.isfinite(v) for v in e.values()):
         raise RuntimeError(f"non-finite energies after the timed steps: {e}")
-    g.profile(True)
-    g.nose_poincare(steps)
-    sync()
-    stats = g.kernel_stats()
-    g.profile(False)
     L = p.Lx
     N, Kh = p.N, (L // 2 + 1) * L * L
     ws = 4 if precision == 1 else 8
@@ -494,7 +496,7 @@ def main():
         if args.size == 128:
             args.size = 64
         if args.scan_steps == 4000:
-            args.scan_steps = 6000
+            args.scan_steps = 10000
         bench_warmstart(args, sync_extra)
         return
 

@@ -236,12 +236,19 @@ int force_pipeline(fhip_ctx *ctx, int kind, bool materialize, int next_step = 0)
     return force_pipeline_t<double>(ctx, kind, materialize, next_step);
 }
 
+// Middle steps are replayed from graphs of 16, 4 and 1 steps: the kernels of
+// one graph run back to back, and a graph launch costs ~9 us of idle device
+// time at its boundary (128^3), so a long call is mostly 16-step launches.
+constexpr int kGraphSteps[3] = {16, 4, 1};
+
 void drop_step_graph(fhip_ctx *ctx)
 {
-    if (ctx->sgexec) hipGraphExecDestroy(ctx->sgexec);
-    if (ctx->sgraph) hipGraphDestroy(ctx->sgraph);
-    ctx->sgexec = nullptr;
-    ctx->sgraph = nullptr;
+    for (int j = 0; j < 3; ++j) {
+        if (ctx->sgexec[j]) hipGraphExecDestroy(ctx->sgexec[j]);
+        if (ctx->sgraph[j]) hipGraphDestroy(ctx->sgraph[j]);
+        ctx->sgexec[j] = nullptr;
+        ctx->sgraph[j] = nullptr;
+    }
     ctx->skey.clear();
 }
 
@@ -275,8 +282,9 @@ int one_step(fhip_ctx *ctx, int kind, bool materialize, int next_step)
 }
 
 // count steps that are not the last of a call (nothing materialised): with two
-// or more, one such step is captured as a hipGraph (kernel nodes only) and
-// replayed; the capture is kept while the launch state stays the same.
+// or more, graphs of 16, 4 and 1 such steps (kernel nodes only) are captured
+// together and replayed, largest first; the captures are kept while the
+// launch state stays the same.
 int middle_steps(fhip_ctx *ctx, int kind, int next_step, int count)
 {
     if (count <= 0) return FHIP_OK;
@@ -285,22 +293,34 @@ int middle_steps(fhip_ctx *ctx, int kind, int next_step, int count)
         return FHIP_OK;
     }
     std::vector<unsigned char> key = step_key(ctx, kind, next_step);
-    if (!ctx->sgexec || key != ctx->skey) {
+    if (!ctx->sgexec[0] || key != ctx->skey) {
         drop_step_graph(ctx);
-        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = one_step(ctx, kind, false, next_step);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (rc != FHIP_OK) {
-            if (g) hipGraphDestroy(g);
-            return rc;
+        for (int j = 0; j < 3; ++j) {
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = FHIP_OK;
+            for (int i = 0; i < kGraphSteps[j] && rc == FHIP_OK; ++i) rc = one_step(ctx, kind, false, next_step);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc != FHIP_OK) {
+                if (g) hipGraphDestroy(g);
+                drop_step_graph(ctx);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                drop_step_graph(ctx);
+                return fail(ctx, FHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            }
+            ctx->sgraph[j] = g;
+            const hipError_t ei = hipGraphInstantiate(&ctx->sgexec[j], g, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                drop_step_graph(ctx);
+                return fail(ctx, FHIP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            }
         }
-        if (e != hipSuccess) return fail(ctx, FHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        ctx->sgraph = g;
-        HIPCHK(hipGraphInstantiate(&ctx->sgexec, g, nullptr, nullptr, 0));
         ctx->skey = key;
     }
-    for (int i = 0; i < count; ++i) HIPCHK(hipGraphLaunch(ctx->sgexec, ctx->stream));
+    for (int j = 0; j < 3; ++j)
+        for (; count >= kGraphSteps[j]; count -= kGraphSteps[j]) HIPCHK(hipGraphLaunch(ctx->sgexec[j], ctx->stream));
     return FHIP_OK;
 }
 

@@ -63,9 +63,9 @@ struct fhip_batch {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr;
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<int> gkey;              // averaging flags of the captured graph
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};     // 16, 4 and 1 middle steps
+    hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+    std::vector<int> gkey;              // averaging flags and tiles of the captured graphs
     std::string err;
 };
 
@@ -337,33 +337,50 @@ std::vector<int> graph_key(const fhip_batch *b)
     return key;
 }
 
+constexpr int kBatchGraphSteps[3] = {16, 4, 1};
+
 void drop_graph(fhip_batch *b)
 {
-    if (b->gexec) hipGraphExecDestroy(b->gexec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
+    for (int j = 0; j < 3; ++j) {
+        if (b->gexec[j]) hipGraphExecDestroy(b->gexec[j]);
+        if (b->graph[j]) hipGraphDestroy(b->graph[j]);
+        b->gexec[j] = nullptr;
+        b->graph[j] = nullptr;
+    }
     b->gkey.clear();
 }
 
-// Capture one middle step (every class) into a graph, once per averaging
-// configuration.
+// Capture 16, 4 and 1 middle steps (every class) into three graphs, once per
+// averaging and tile configuration: a long call replays mostly 16-step
+// graphs, whose kernels run back to back (a graph launch leaves the device
+// idle for a few microseconds at its boundary).
 int ensure_graph(fhip_batch *b)
 {
     const std::vector<int> key = graph_key(b);
-    if (b->gexec && key == b->gkey) return FHIP_OK;
+    if (b->gexec[0] && key == b->gkey) return FHIP_OK;
     drop_graph(b);
-    BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = batch_phase(b, PH_STEP);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->stream, &g);
-    if (rc != FHIP_OK) {
-        if (g) hipGraphDestroy(g);
-        return rc;
+    for (int j = 0; j < 3; ++j) {
+        BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = FHIP_OK;
+        for (int i = 0; i < kBatchGraphSteps[j] && rc == FHIP_OK; ++i) rc = batch_phase(b, PH_STEP);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != FHIP_OK) {
+            if (g) hipGraphDestroy(g);
+            drop_graph(b);
+            return rc;
+        }
+        if (e != hipSuccess) {
+            drop_graph(b);
+            return bfail(b, FHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        }
+        b->graph[j] = g;
+        e = hipGraphInstantiate(&b->gexec[j], g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            drop_graph(b);
+            return bfail(b, FHIP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
     }
-    if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    b->graph = g;
-    BHIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
     b->gkey = key;
     return FHIP_OK;
 }
@@ -582,7 +599,10 @@ int fhip_batch_step(fhip_batch *b, int nsteps, fhip_energy *e)
     if (middle > 0) {
         if (b->use_graph) {
             RC(ensure_graph(b));
-            for (int i = 0; i < middle; ++i) BHIP(hipGraphLaunch(b->gexec, b->stream));
+            int left = middle;
+            for (int j = 0; j < 3; ++j)
+                for (; left >= kBatchGraphSteps[j]; left -= kBatchGraphSteps[j])
+                    BHIP(hipGraphLaunch(b->gexec[j], b->stream));
         } else {
             for (int i = 0; i < middle; ++i) RC(batch_phase(b, PH_STEP));
         }

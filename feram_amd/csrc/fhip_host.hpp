@@ -100,11 +100,11 @@ struct fhip_ctx {
     std::vector<int> nlayer_host;
     // measurement
     bool prof = false;
-    // graph of one middle step of a multi-step call, and the by-value launch
-    // state it was captured with
+    // graphs of 16, 4 and 1 middle steps of a multi-step call
+    // (kGraphSteps), and the by-value launch state they were captured with
     bool use_graph = true;
-    hipGraph_t sgraph = nullptr;
-    hipGraphExec_t sgexec = nullptr;
+    hipGraph_t sgraph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t sgexec[3] = {nullptr, nullptr, nullptr};
     std::vector<unsigned char> skey;
     std::vector<hipEvent_t> ev_pool;
     struct Rec { int id; hipEvent_t a, b; };

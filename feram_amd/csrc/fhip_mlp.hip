@@ -1,4 +1,4 @@
-// fhip_mlp.hip -- the learned warm start: a 3-128-128-12 multilayer perceptron
+// fhip_mlp.hip -- the learned warm start: a 3-128-128-13 multilayer perceptron
 // (GELU hidden layers) mapping (temperature, field, pressure) to the
 // equilibrium dipole statistics and homogeneous strain that plain MD reaches.
 // It is trained on equilibrated states that the MD itself produces and seeds
@@ -25,7 +25,7 @@
 
 namespace {
 
-constexpr int NIN = 3, NH = 128, NOUT = 12;
+constexpr int NIN = 3, NH = 128, NOUT = 13;
 constexpr int KIN = 32;                 // input width padded to one MFMA K step
 constexpr int MS = 16;                  // samples per workgroup (one M tile)
 constexpr int WL = 64;                  // lanes per workgroup (one wave)
@@ -41,8 +41,8 @@ constexpr int NPART = NPARAM + 1;       // per-workgroup partial: gradients, the
 // W-derived values at n = 16 t + l % 16, k = 32 s + 8 (l / 16) + j.
 constexpr int F1_T = NH / 16, F1_S = 1;          // W1 forward: 8 tiles x 1 step
 constexpr int F2_T = NH / 16, F2_S = NH / 32;    // W2 forward / backward: 8 x 4
-constexpr int F3_T = 1, F3_S = NH / 32;          // W3 forward: 1 tile (12 of 16) x 4 steps
-constexpr int B3_T = NH / 16, B3_S = 1;          // W3 backward: 8 tiles x 1 step (12 of 32)
+constexpr int F3_T = 1, F3_S = NH / 32;          // W3 forward: 1 tile (13 of 16) x 4 steps
+constexpr int B3_T = NH / 16, B3_S = 1;          // W3 backward: 8 tiles x 1 step (13 of 32)
 constexpr int PK = 64 * 8;                       // bf16 per (tile, step)
 constexpr int PF1 = 0, PF2 = PF1 + F1_T * F1_S * PK, PF3 = PF2 + F2_T * F2_S * PK,
               PB2 = PF3 + F3_T * F3_S * PK, PB3 = PB2 + F2_T * F2_S * PK, NPACK = PB3 + B3_T * B3_S * PK;
@@ -107,9 +107,9 @@ constexpr int LD_I = KIN + 8;           // of the 32-wide input and output-gradi
 
 struct MlpArgs {
     const float* x;          // [n][3] normalised inputs
-    const float* y;          // [n][12] normalised targets (training)
-    const float* wout;       // [12] loss weight per output
-    float* out;              // [n][12] predictions (forward only) or nullptr
+    const float* y;          // [n][13] normalised targets (training)
+    const float* wout;       // [13] loss weight per output
+    float* out;              // [n][13] predictions (forward only) or nullptr
     float* part;             // [nwg][NPART] gradients + loss (training) or nullptr
     const float* master;     // FP32 parameters (biases)
     const __bf16* pack;      // BF16 packings
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(WL) void k_mlp(MlpArgs a)
     }
     __syncthreads();
 
-    // layer 3 (12 outputs in a 16-wide tile)
+    // layer 3 (13 outputs in a 16-wide tile)
     f32x4 z3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < NH / 32; ++s) z3 = mfma(load_rows(A2, LD_H, s, l), load_pack(a.pack, PF3, F3_S, 0, s, l), z3);
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(WL) void k_mlp(MlpArgs a)
     __syncthreads();
 
     // weight gradients, K = the 16 samples: dW[m][n] = sum_i G[i][m] act[i][n]
-    // dW3 (m = outputs, 12 of 16; n = hidden)
+    // dW3 (m = outputs, 13 of 16; n = hidden)
     {
         const bf16x8 ga = load_cols(G3, LD_I, 0, l);
 #pragma unroll

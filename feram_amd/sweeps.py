@@ -17,6 +17,11 @@ import numpy as np
 from .params import Param
 
 
+# the nine potential-energy terms of Energy_type (energy_module.F:73-87)
+POTENTIAL = ("short_range", "long_range", "dipole_E_field", "unharmonic", "homo_strain", "homo_coupling",
+             "inho_strain", "inho_coupling", "inho_modulation")
+
+
 def phase_of(u_sorted, threshold: float = 0.04) -> str:
     """Name of the ferroelectric phase from the sorted |<u>| (ascending):
     R (three components polarised), O (two), T (one) or C (none)."""
@@ -33,7 +38,8 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
     returns per temperature and chain <u> (3), its sorted magnitudes and the
     averaged homogeneous strain (6), the chain means and standard errors of
     the sorted |<u>|, and with `variance` the per-component variance of u over
-    the cells of the final state of each temperature."""
+    the cells of the final state of each temperature; "pe" is the averaged
+    potential energy per cell [eV]."""
     from .params import batio3_zvr
     from .system import Batch, System
     temps = [float(t) for t in temps]
@@ -51,6 +57,7 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
         systems.append(g)
     u = np.zeros((len(temps), len(seeds), 3))
     var = np.zeros((len(temps), len(seeds), 3))
+    pe = np.zeros((len(temps), len(seeds)))
     strain = np.zeros((len(temps), len(seeds), 6))
     with Batch(systems) as bt:
         for it, T in enumerate(temps):
@@ -71,6 +78,7 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
                 g.avg_auto(False)
                 u[it, j] = np.array(a["u"]) / (a["n_add"] * g.avg_u_norm())
                 strain[it, j] = np.array(a["strain"]) / a["n_add"]
+                pe[it, j] = sum(a["energy"][t] for t in POTENTIAL) / (a["n_add"] * g.N)
                 if variance:
                     var[it, j] = g.get("dipoR").reshape(-1, 3).var(axis=0)
             if progress:
@@ -79,7 +87,7 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
         g.close()
     us = np.sort(np.abs(u), axis=2)
     err = us.std(axis=1, ddof=1) / np.sqrt(len(seeds)) if len(seeds) > 1 else np.full(us.shape[::2], np.nan)
-    out = {"temps": temps, "seeds": list(seeds), "u": u, "u_sorted": us, "strain": strain,
+    out = {"temps": temps, "seeds": list(seeds), "u": u, "u_sorted": us, "strain": strain, "pe": pe,
            "u_sorted_mean": us.mean(axis=1), "u_sorted_err": err,
            "phases": [[phase_of(us[i, j]) for j in range(len(seeds))] for i in range(len(temps))]}
     if variance:

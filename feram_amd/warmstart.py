@@ -10,10 +10,16 @@ uses it to start new runs near equilibrium (``WarmStart.initialise``).  The
 MD still decides when a run is equilibrated.
 
 Inputs: temperature [K], field E_z [V/A] (along z) and hydrostatic pressure
-[GPa].  Outputs (12): the sorted magnitudes of the mean dipole <u>
+[GPa].  Outputs (13): the sorted magnitudes of the mean dipole <u>
 (ascending, so the largest lies along z, the field axis), the per-component
-variance of u over the cells in the same order, and the homogeneous strain
-(xx yy zz yz zx xy) in that frame.
+variance of u over the cells in the same order, the homogeneous strain
+(xx yy zz yz zx xy) in that frame, and the potential energy per cell.
+
+A warm start draws the dipoles as the predicted mean plus Gaussian
+fluctuations with the spectrum of the quadratic dipole energy
+(``shaped_fluctuations``) and the predicted variances, and gives the momenta
+the kinetic energy that makes the total energy the predicted equilibrium one,
+so that neither the structure nor the thermostat has much to relax.
 """
 from __future__ import annotations
 
@@ -23,8 +29,9 @@ import numpy as np
 
 from . import _lib
 from .params import Param
+from .sweeps import POTENTIAL
 
-NIN, NOUT = 3, 12
+NIN, NOUT = 3, 13
 # inputs are mapped to about [-1, 1] over the trained ranges
 T_MID, T_HALF = 300.0, 150.0
 E_MID, E_HALF = 5e-4, 5e-4
@@ -41,7 +48,7 @@ def normalise_inputs(T, Ez, GPa) -> np.ndarray:
 
 
 class Mlp:
-    """The device network (fhip_mlp): 3-128-128-12, GELU, BF16 MFMA, FP32 master weights."""
+    """The device network (fhip_mlp): 3-128-128-13, GELU, BF16 MFMA, FP32 master weights."""
 
     def __init__(self, seed: int = 0):
         self._L = _lib.load()
@@ -114,13 +121,99 @@ class Mlp:
         return hist
 
 
+# ---- fluctuations with the spectrum of the quadratic dipole energy ---------------
+KELVIN_EV = 11604.45            # K per eV (define.h:33)
+
+
+def short_range_kernel(p: Param) -> np.ndarray:
+    """S(k) on numpy's rfftn grid [kz][ky][kx <= Lx/2][a][b]: the Fourier
+    transform of the j1..j7 neighbour couplings plus 2 kappa2 on the diagonal
+    (dipole-dipole-short-range.F:35-83), with c = cos, s = sin of 2 pi k / L."""
+    def cs(n, L):
+        a = 2 * np.pi * np.arange(n) / L
+        return np.cos(a), np.sin(a)
+    cx, sx = cs(p.Lx // 2 + 1, p.Lx)
+    cy, sy = cs(p.Ly, p.Ly)
+    cz, sz = cs(p.Lz, p.Lz)
+    cx, sx = cx[None, None, :], sx[None, None, :]
+    cy, sy = cy[None, :, None], sy[None, :, None]
+    cz, sz = cz[:, None, None], sz[:, None, None]
+    j1, j2, j3, j4, j5, j6, j7 = p.j
+    k2t = 2 * p.P_kappa2[0]
+    cxyz = cx * cy * cz
+    S = np.zeros((p.Lz, p.Ly, p.Lx // 2 + 1, 3, 3))
+    S[..., 0, 0] = k2t + 2 * j2 * cx + 2 * j1 * (cy + cz) + 4 * j3 * cx * (cy + cz) + 4 * j4 * cy * cz + 8 * j6 * cxyz
+    S[..., 1, 1] = k2t + 2 * j2 * cy + 2 * j1 * (cz + cx) + 4 * j3 * cy * (cz + cx) + 4 * j4 * cz * cx + 8 * j6 * cxyz
+    S[..., 2, 2] = k2t + 2 * j2 * cz + 2 * j1 * (cx + cy) + 4 * j3 * cz * (cx + cy) + 4 * j4 * cx * cy + 8 * j6 * cxyz
+    S[..., 1, 2] = S[..., 2, 1] = -4 * j5 * sy * sz - 8 * j7 * sy * sz * cx
+    S[..., 2, 0] = S[..., 0, 2] = -4 * j5 * sz * sx - 8 * j7 * sz * sx * cy
+    S[..., 0, 1] = S[..., 1, 0] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz
+    return S
+
+
+def dipole_kernel(p: Param, long_range: np.ndarray) -> np.ndarray:
+    """A(k) = S(k) + Z*^2 L(k) on the rfftn grid, the kernel of the quadratic
+    dipole energy E = (1/2N) sum_k u^(k)^H A(k) u^(k) (dVddi-dipole-dipole.F:
+    20-88, one Z* for all cells).  long_range: c%long_range in C order
+    (z, y[+padding], kx, b, a), as System.get_long_range returns it."""
+    L = np.asarray(long_range, float)[:, :p.Ly]
+    Lsym = 0.5 * (L + np.swapaxes(L, -1, -2))
+    return short_range_kernel(p) + p.Z_star[0] ** 2 * Lsym
+
+
+def shaped_fluctuations(A: np.ndarray, shape, T: float, var, rng) -> np.ndarray:
+    """Gaussian dipole fluctuations [z][y][x][3] with the equilibrium spectrum of
+    the quadratic energy A(k) + Delta (Delta: one on-site shift that stands for
+    the anharmonic and strain terms, set so that the mean per-component
+    variance over the cells is mean(var)), each component then scaled to its
+    own variance var[a].  With E = (1/2N) sum_k u^H (A + Delta) u^ the
+    equilibrium covariance of u^(k) = sum_R u(R) e^{-ikR} is N kT (A + Delta)^-1,
+    drawn here as sqrt(kT) (A + Delta)^(-1/2) w^(k) from the transform w^ of
+    white unit noise."""
+    Lz, Ly, Lx = shape
+    N = Lx * Ly * Lz
+    var = np.maximum(np.asarray(var, float), 1e-12)
+    kT = T / KELVIN_EV
+    lam, V = np.linalg.eigh(A)
+    wk = np.full(A.shape[:3], 2.0)                     # kx and -kx on the half grid
+    wk[:, :, 0] = 1.0
+    if Lx % 2 == 0:
+        wk[:, :, -1] = 1.0
+    lmin = lam.min()
+    target = var.mean()
+
+    def mean_var(delta):
+        return kT / N * np.sum(wk[..., None] / (lam + delta)) / 3
+
+    lo = -lmin + 1e-9 * max(abs(lmin), 1.0)
+    hi = max(1.0, 2 * abs(lmin))
+    while mean_var(hi) > target:
+        hi *= 2
+    for _ in range(100):                               # var(Delta) falls monotonically
+        mid = 0.5 * (lo + hi)
+        if mean_var(mid) > target:
+            lo = mid
+        else:
+            hi = mid
+    scale = np.sqrt(kT / (lam + hi))                   # (A + Delta)^(-1/2) sqrt(kT) in the eigenbasis
+    w = np.fft.rfftn(rng.standard_normal((3,) + tuple(shape)), axes=(1, 2, 3))
+    w = np.moveaxis(w, 0, -1)                          # [kz][ky][kx][a]
+    c = np.einsum("...ba,...b->...a", V, w)           # V^T w
+    u = np.einsum("...ab,...b->...a", V, c * scale)   # V diag(scale) V^T w
+    u = np.fft.irfftn(np.moveaxis(u, -1, 0), s=tuple(shape), axes=(1, 2, 3))
+    u = np.moveaxis(u, 0, -1)
+    u *= np.sqrt(var / u.reshape(-1, 3).var(axis=0))
+    return u
+
+
 # ---- canonical frame of an equilibrium state -------------------------------------
 _SHEAR = {(1, 2): 3, (2, 1): 3, (2, 0): 4, (0, 2): 4, (0, 1): 5, (1, 0): 5}
 
 
-def canonical(mean_u, var_u, strain):
+def canonical(mean_u, var_u, strain, pe):
     """Rotate (permute and reflect the axes of) an equilibrium state so that
-    |<u>| ascends from x to z and every <u>_a >= 0; returns the 12 targets."""
+    |<u>| ascends from x to z and every <u>_a >= 0; returns the 13 targets
+    (the potential energy per cell pe is frame-independent)."""
     mean_u, var_u, strain = np.asarray(mean_u), np.asarray(var_u), np.asarray(strain)
     perm = np.argsort(np.abs(mean_u), kind="stable")          # new axis a <- old axis perm[a]
     sign = np.where(mean_u >= 0, 1.0, -1.0)
@@ -131,7 +224,7 @@ def canonical(mean_u, var_u, strain):
     for (a, b), k in ((pair, _SHEAR[pair]) for pair in ((1, 2), (2, 0), (0, 1))):
         pa, pb = perm[a], perm[b]
         e[k] = strain[_SHEAR[(pa, pb)]] * sign[pa] * sign[pb]
-    return np.concatenate([m, v, e])
+    return np.concatenate([m, v, e, [float(pe)]])
 
 
 def equilibrium_samples(conditions, L: int = 12, n_therm: int = 3000, n_avg: int = 1000,
@@ -139,10 +232,10 @@ def equilibrium_samples(conditions, L: int = 12, n_therm: int = 3000, n_avg: int
                         batch_size: int = 256):
     """Equilibrate one replica per condition (T [K], E_z [V/A], GPa) with plain
     NP-MD from the reference's default start (init_dipo_avg 0.1, dev 0.02 per
-    component, p = 0), all replicas in one batch, and return the 12 canonical
+    component, p = 0), all replicas in one batch, and return the 13 canonical
     targets per replica: <u> from the device averages over the last n_avg steps,
-    the variance of u over the cells averaged over `snapshots` states, and the
-    averaged homogeneous strain."""
+    the variance of u over the cells averaged over `snapshots` states, the
+    averaged homogeneous strain and potential energy per cell."""
     from .params import batio3_zvr
     from .system import Batch, System
     conditions = np.asarray(conditions, dtype=float)
@@ -178,7 +271,8 @@ def equilibrium_samples(conditions, L: int = 12, n_therm: int = 3000, n_avg: int
             n = a["n_add"] * g.N
             mean_u = np.array(a["u"]) / n
             strain = np.array(a["strain"]) / a["n_add"]
-            out[c0 + i] = canonical(mean_u, var[i], strain)
+            pe = sum(a["energy"][t] for t in POTENTIAL) / (a["n_add"] * g.N)
+            out[c0 + i] = canonical(mean_u, var[i], strain, pe)
             g.close()
     return out
 
@@ -188,8 +282,8 @@ def chain_samples(temps, fields=(0.0, 2.5e-4, 5e-4, 1e-3), pressures=(-6.0, -5.0
                   progress=None):
     """Training data on the heating branch: one chained scan over `temps` per
     (field, pressure, seed), all chains in one batch (feram_amd.sweeps), and
-    the 12 canonical targets of every (chain, temperature).  Returns
-    (conditions [n][3] = T, E_z, GPa; targets [n][12])."""
+    the 13 canonical targets of every (chain, temperature).  Returns
+    (conditions [n][3] = T, E_z, GPa; targets [n][13])."""
     from .params import batio3_zvr
     from .sweeps import temperature_chain
     bases, conds = [], []
@@ -208,7 +302,7 @@ def chain_samples(temps, fields=(0.0, 2.5e-4, 5e-4, 1e-3), pressures=(-6.0, -5.0
     for i, T in enumerate(res["temps"]):
         for j, (Ez, GPa) in enumerate(conds):
             X.append((T, Ez, GPa))
-            Y.append(canonical(res["u"][i, j], res["variance"][i, j], res["strain"][i, j]))
+            Y.append(canonical(res["u"][i, j], res["variance"][i, j], res["strain"][i, j], res["pe"][i, j]))
     return np.array(X), np.array(Y)
 
 
@@ -219,6 +313,7 @@ class WarmStart:
         self.net = Mlp(seed)
         self.y_mean = np.zeros(NOUT)
         self.y_std = np.ones(NOUT)
+        self._kernels = {}
 
     def fit(self, conditions, targets, iters: int = 3000, lr: float = 2e-3) -> np.ndarray:
         x = normalise_inputs(*np.asarray(conditions, float).T)
@@ -232,28 +327,67 @@ class WarmStart:
         x = normalise_inputs(T, Ez, GPa).reshape(-1, NIN)
         return self.net.forward(x) * self.y_std + self.y_mean
 
+    def kernel(self, system) -> np.ndarray:
+        """A(k) = S(k) + Z*^2 L(k) of the system's supercell, from its device
+        L(k) table (cached per supercell and parameter set)."""
+        p = system.p
+        key = (p.Lx, p.Ly, p.Lz, tuple(p.j), p.P_kappa2[0], p.Z_star[0], p.epsilon_inf, p.a0, p.kappa)
+        if key not in self._kernels:
+            self._kernels[key] = dipole_kernel(p, system.get_long_range())
+        return self._kernels[key]
+
     def initialise(self, system, T: float, Ez: float, GPa: float, seed: int = 0,
-                   momenta: bool = True) -> np.ndarray:
-        """u(R) = <u> + sqrt(var) * N(0, 1) per component in the canonical frame
-        (largest mean along z, all means >= 0); with momenta, p from the
-        Maxwell-Boltzmann distribution at T (p_a ~ N(0, m k T)).  Returns the
-        prediction."""
+                   momenta: bool = True, shaped: bool = True, energy: bool = True) -> np.ndarray:
+        """u(R) = <u> + fluctuations in the canonical frame (largest mean along
+        z, all means >= 0): with `shaped`, Gaussian fluctuations with the
+        spectrum of the quadratic dipole energy (shaped_fluctuations) and the
+        predicted per-component variance, else white noise sqrt(var) N(0, 1).
+        With momenta, p from the Maxwell-Boltzmann distribution at T
+        (p_a ~ N(0, m k T)); with `energy` as well, p is then scaled so that
+        the kinetic energy per cell is the predicted equilibrium potential plus
+        3/2 kT minus the potential energy of the drawn u (one force
+        evaluation), within [1/4, 7/4] of 3/2 kT.  Returns the prediction."""
         pred = self.predict(T, Ez, GPa)[0]
         m, v = np.maximum(pred[:3], 0.0), np.maximum(pred[3:6], 0.0)
         p = system.p
         rng = np.random.default_rng(seed)
         shape = (p.Lz, p.Ly, p.Lx, 3)
-        system.put("dipoR", m + np.sqrt(v) * rng.standard_normal(shape))
+        if shaped:
+            system.put("dipoR", m + shaped_fluctuations(self.kernel(system), shape[:3], T, v, rng))
+        else:
+            system.put("dipoR", m + np.sqrt(v) * rng.standard_normal(shape))
         if momenta:
-            kT = T / 11604.45                                   # eV (define.h:33)
+            kT = T / KELVIN_EV
             mass = np.asarray(p.mass_dipo, float)
-            system.put("dipoP", np.sqrt(mass * kT) * rng.standard_normal(shape))
+            mom = np.sqrt(mass * kT) * rng.standard_normal(shape)
+            if energy:
+                e = system.mirror_force_mirror()
+                pe0 = sum(e[t] for t in POTENTIAL) / p.N
+                ke = float(np.sum(mom ** 2 / mass)) / (2 * p.N)
+                target = min(max(pred[12] + 1.5 * kT - pe0, 0.25 * 1.5 * kT), 1.75 * 1.5 * kT)
+                mom *= np.sqrt(target / ke)
+            system.put("dipoP", mom)
         return pred
 
 
 # ---- what the warm start gains ------------------------------------------------------
-POTENTIAL = ("short_range", "long_range", "dipole_E_field", "unharmonic", "homo_strain", "homo_coupling",
-             "inho_strain", "inho_coupling", "inho_modulation")
+def mser_index(series, batch: int = 5) -> int:
+    """Truncation point of the MSER-5 rule (K. P. White, "An effective
+    truncation heuristic for bias reduction in simulation output", Simulation
+    69 (1997) 323-334): over batch means of `batch` records, the number d of
+    leading batches (d at most half of them) whose removal minimises the
+    squared standard error of the mean of the rest, sum_{i>d} (y_i - ybar_d)^2
+    / (n - d)^2.  Returned in records (d * batch)."""
+    s = np.asarray(series, float)
+    n = len(s) // batch
+    y = s[:n * batch].reshape(n, batch).mean(axis=1)
+    best, best_d = None, 0
+    for d in range(0, n // 2 + 1):
+        r = y[d:]
+        v = float(((r - r.mean()) ** 2).sum()) / len(r) ** 2
+        if best is None or v < best:
+            best, best_d = v, d
+    return best_d * batch
 
 
 def equilibration_index(series, tail: float = 0.5, window: int = 5, band: float = 4.0) -> int:
@@ -271,13 +405,18 @@ def equilibration_index(series, tail: float = 0.5, window: int = 5, band: float 
 
 def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, record: int = 10,
                  Ez: float = 0.0, GPa: float = -5.0, precision: int = 0, seed: int = 1,
-                 base: Param | None = None, blocks: int = 5, Q_Nose: float | None = None) -> dict:
+                 base: Param | None = None, blocks: int = 5, Q_Nose: float | None = None,
+                 shaped: bool = True, energy: bool = True) -> dict:
     """Run every temperature twice in one batch -- from the reference's default
     start (init_dipo_avg 0.1, dev 0.02, p = 0) and from the warm start -- record
     the potential energy per cell every `record` steps, and return per
     temperature the equilibration step of both runs and the sorted magnitudes
     of <u> over the second half of each run, as the mean and standard error of
-    `blocks` block averages."""
+    `blocks` block averages.  The equilibration step "tau" is the MSER-5
+    truncation point of the potential-energy series (mser_index; at most half
+    the run, so a run that has not settled counts as n_steps / 2); "tau_band"
+    is the last exit of the smoothed series from a 4-sigma band around its
+    plateau (equilibration_index), which single late excursions can set."""
     from .params import batio3_zvr
     from .system import Batch, System
     rng = np.random.default_rng(seed)
@@ -294,7 +433,8 @@ def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, recor
             if kind == "cold":
                 g.put("dipoR", 0.1 + 0.02 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
             else:
-                ws.initialise(g, float(T), float(Ez), float(GPa), seed=int(rng.integers(1 << 31)))
+                ws.initialise(g, float(T), float(Ez), float(GPa), seed=int(rng.integers(1 << 31)), shaped=shaped,
+                              energy=energy)
             g.md_prologue()
             systems.append(g)
             kinds.append(kind)
@@ -323,10 +463,12 @@ def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, recor
                         block_u[i].append(np.sort(np.abs((u - usum_prev[i]) / (dn * g.N))))
                     usum_prev[i], nadd_prev[i] = u, a["n_add"]
     res = {"temps": list(map(float, temps)), "record": record, "n_steps": n_steps, "L": L,
-           "cold": {"tau": [], "u": [], "u_err": []}, "warm": {"tau": [], "u": [], "u_err": []}}
+           "cold": {"tau": [], "tau_band": [], "u": [], "u_err": []},
+           "warm": {"tau": [], "tau_band": [], "u": [], "u_err": []}}
     for i, g in enumerate(systems):
         b = np.array(block_u[i])
-        res[kinds[i]]["tau"].append(equilibration_index(series[i]) * record)
+        res[kinds[i]]["tau"].append(mser_index(series[i]) * record)
+        res[kinds[i]]["tau_band"].append(equilibration_index(series[i]) * record)
         res[kinds[i]]["u"].append(b.mean(axis=0).tolist())
         res[kinds[i]]["u_err"].append((b.std(axis=0, ddof=1) / np.sqrt(len(b))).tolist() if len(b) > 1
                                       else [float("nan")] * 3)

@@ -295,15 +295,16 @@ int fhip_batch_classes(const fhip_batch *batch);
 
 /* Learned warm start (no counterpart in the reference, which starts every run
  * from init_dipo_avg / init_dipo_dev or a .restart file, initialize-dipoR.F:22-58).
- * A 3-128-128-12 perceptron with GELU hidden layers, BF16 MFMA arithmetic with
+ * A 3-128-128-13 perceptron with GELU hidden layers, BF16 MFMA arithmetic with
  * FP32 accumulation and FP32 master weights, trained with Adam on the device.
  * Inputs and targets are normalised by the caller; the meaning of the 3 inputs
- * and 12 outputs is the caller's (feram_amd.warmstart: temperature, E_z and
- * pressure -> sorted |<u>|, per-component variance of u and homogeneous strain).
+ * and 13 outputs is the caller's (feram_amd.warmstart: temperature, E_z and
+ * pressure -> sorted |<u>|, per-component variance of u, homogeneous strain and
+ * the potential energy per cell).
  * Parameters (fhip_mlp_nparams() floats): W1[128][3], b1[128], W2[128][128],
- * b2[128], W3[12][128], b3[12], row-major W[out][in]. */
+ * b2[128], W3[13][128], b3[13], row-major W[out][in]. */
 typedef struct fhip_mlp fhip_mlp;
-enum { FHIP_MLP_IN = 3, FHIP_MLP_HIDDEN = 128, FHIP_MLP_OUT = 12, FHIP_MLP_MAX_SAMPLES = 65536 };
+enum { FHIP_MLP_IN = 3, FHIP_MLP_HIDDEN = 128, FHIP_MLP_OUT = 13, FHIP_MLP_MAX_SAMPLES = 65536 };
 int fhip_mlp_nparams(void);
 /* Glorot-uniform weights and zero biases from seed. */
 int fhip_mlp_create(uint64_t seed, fhip_mlp **mlp);
@@ -311,7 +312,7 @@ void fhip_mlp_destroy(fhip_mlp *mlp);
 const char *fhip_mlp_last_error(const fhip_mlp *mlp);
 int fhip_mlp_set_params(fhip_mlp *mlp, const float *params);   /* also resets Adam's moments */
 int fhip_mlp_get_params(fhip_mlp *mlp, float *params);
-/* y[n][12] = network(x[n][3]) (host arrays). */
+/* y[n][13] = network(x[n][3]) (host arrays). */
 int fhip_mlp_forward(fhip_mlp *mlp, int n, const float *x, float *y);
 /* loss = sum_i sum_o w_o (net(x_i)_o - y_io)^2 / n and its gradient with
  * respect to every parameter (w may be NULL: all 1). */

@@ -2,9 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Only tests/ may import this module; the product
 package never does.  The network has no counterpart in the reference (it is an
-extension of the step path): a 3-128-128-12 perceptron with exact-erf GELU
+extension of the step path): a 3-128-128-13 perceptron with exact-erf GELU
 hidden layers, the FP32 parameter layout of fhip_mlp_set_params (W1[128][3],
-b1, W2[128][128], b2, W3[12][128], b3, row-major W[out][in]) and the loss
+b1, W2[128][128], b2, W3[13][128], b3, row-major W[out][in]) and the loss
 sum_o w_o (net(x)_o - y_o)^2 averaged over the samples.
 
 ``emulate=True`` computes in FP64 but rounds to BF16 (round to nearest even)
@@ -21,7 +21,7 @@ import math
 
 import numpy as np
 
-NIN, NH, NOUT = 3, 128, 12
+NIN, NH, NOUT = 3, 128, 13
 OW1 = 0
 OB1 = OW1 + NH * NIN
 OW2 = OB1 + NH

@@ -1,8 +1,9 @@
 """Multi-step calls replayed from a captured hipGraph (fhip_use_graph) against
 the same calls with every kernel launched directly.
 
-The graph holds one middle step (the same kernels with the same arguments),
-so both runs must agree bit for bit after every call: several calls in a row,
+The graphs hold 16, 4 and 1 middle steps (the same kernels with the same
+arguments, replayed largest first: a 22-step call replays 16 + 4 + 1), so
+both runs must agree bit for bit after every call: several calls in a row,
 with host work between them (a temperature change, which changes the
 captured arguments and must lead to a new capture; reading fields back).
 """
@@ -42,7 +43,7 @@ def test_graph_replay_equals_direct_launches(method, geometry, delta_e):
     print(f"[gpu] step graph {method} {geometry} delta_e={delta_e}", flush=True)
     a = _system(method, geometry, delta_e, True)
     b = _system(method, geometry, delta_e, False)
-    for nsteps, kelvin in ((7, None), (5, 320.0), (1, None), (6, None)):
+    for nsteps, kelvin in ((7, None), (5, 320.0), (1, None), (6, None), (22, None)):
         if kelvin is not None:
             a.set_kelvin(kelvin)
             b.set_kelvin(kelvin)

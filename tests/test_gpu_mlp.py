@@ -78,7 +78,9 @@ def test_training_fits_and_is_deterministic():
     rng = np.random.default_rng(4)
     x = rng.uniform(-1, 1, (600, 3))
     t = np.stack([np.sin(2 * x[:, 0]) * x[:, 1], np.cos(x[:, 2]), x[:, 0] * x[:, 1] * x[:, 2]], axis=1)
-    y = np.concatenate([t, t ** 2, np.tanh(3 * t), t[:, :1] - t[:, 1:2], t[:, 2:] + 0.5, t[:, :1] * 0.0 + 1.0], axis=1)
+    y = np.concatenate([t, t ** 2, np.tanh(3 * t), t[:, :1] - t[:, 1:2], t[:, 2:] + 0.5, t[:, :1] * 0.0 + 1.0,
+                        t[:, 1:2] * t[:, 2:]], axis=1)
+    assert y.shape[1] == omlp.NOUT
     a, b = Mlp(seed=9), Mlp(seed=9)
     ha = a.train(x, y, iters=1500, lr=3e-3)
     hb = b.train(x, y, iters=1500, lr=3e-3)
