@@ -21,8 +21,8 @@ the same prologue and two steps from the same doubles in the CPU-baseline leg.
 
 ``roofline`` is the dominant kernel's algorithmic HBM bytes per launch over
 its average launch time, taken with HIP events on the library's stream in a
-separate pass of K steps (direct launches) between the warm-up and the timed
-steps; ``roofline.copy_GBs`` is a 1 GiB device copy timed in the same run (the
+separate pass of max(K, 100) steps (direct launches) between the warm-up and
+the timed steps; ``roofline.copy_GBs`` is a 1 GiB device copy timed in the same run (the
 attainable HBM rate on this box).  ``cpu_baseline`` times the CPU FP64 oracle (OpenMP over
 the host threads) on a bounded sample of the same workload.
 """
@@ -145,10 +145,11 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
 
     if warmup > 0:
         g.nose_poincare(warmup)
-    # per-kernel times: K steps with HIP events around every launch (direct
-    # launches), between the warm-up and the timed steps
+    # per-kernel times: max(K, 100) steps with HIP events around every launch
+    # (direct launches), between the warm-up and the timed steps
+    prof_steps = max(steps, 100)
     g.profile(True)
-    g.nose_poincare(steps)
+    g.nose_poincare(prof_steps)
     sync()
     stats = g.kernel_stats()
     g.profile(False)
@@ -165,7 +166,7 @@ def measure(p, u0, precision: int, steps: int, warmup: int, sync_extra):
     per = {}
     for name, (ms, cnt) in stats.items():
         avg_ms = ms / cnt
-        b = kernel_bytes(name, N, Kh, 9, ws, 1.0 / steps)
+        b = kernel_bytes(name, N, Kh, 9, ws, 1.0 / prof_steps)
         per[name] = {"avg_us": avg_ms * 1e3, "launches": cnt,
                      "GBs": (b / (avg_ms * 1e-3) / 1e9) if b else None, "bytes": b}
     dom = max((k for k in per if per[k]["bytes"]), key=lambda k: per[k]["avg_us"] * per[k]["launches"])
@@ -438,7 +439,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--precision", choices=("both", "fp32", "fp64"), default="both")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
     ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16", "epit_hl"),

@@ -764,47 +764,28 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
             if (k0) {
                 w[0] = w[1] = w[2] = cpx<KT>{KT(0), KT(0)};
             } else {
-                // Cholesky K = G G^T (elastic.F:131-141), then the two
-                // triangular solves of optimize-inho-strain.F:44-57
-                KT G[3][3], Gi[3];
+                // w = -K^-1 (c_mod + C uu) (optimize-inho-strain.F:36-57,
+                // which solves it with the Cholesky factor of elastic.F:131-141):
+                // here through the adjugate of the symmetric 3 x 3 K(k), one
+                // reciprocal of its determinant instead of three square roots
+                // and three divisions on the critical path.  K(k) is positive
+                // definite for k != 0 and well-conditioned (eigenvalues between
+                // about B44 |k|^2 and B11 |k|^2 for a cubic elastic tensor), so
+                // both solves agree to rounding.
+                const KT a00 = K[1][1] * K[2][2] - K[1][2] * K[1][2];
+                const KT a01 = K[0][2] * K[1][2] - K[0][1] * K[2][2];
+                const KT a02 = K[0][1] * K[1][2] - K[0][2] * K[1][1];
+                const KT a11 = K[0][0] * K[2][2] - K[0][2] * K[0][2];
+                const KT a12 = K[0][1] * K[0][2] - K[0][0] * K[1][2];
+                const KT a22 = K[0][0] * K[1][1] - K[0][1] * K[0][1];
+                const KT det = K[0][0] * a00 + K[0][1] * a01 + K[0][2] * a02;
+                const KT id = KT(-1) / det;
+                cpx<KT> r[3];
 #pragma unroll
-                for (int bb = 0; bb < 3; ++bb) {
-                    KT s2 = KT(0);
-#pragma unroll
-                    for (int l2 = 0; l2 < bb; ++l2) s2 += G[bb][l2] * G[bb][l2];
-                    if constexpr (sizeof(KT) == 4) {
-                        // FP32 mode: the reciprocal square root in one
-                        // instruction (1 ulp) instead of a correctly rounded
-                        // sqrt and divide; G's diagonal is not used again
-                        Gi[bb] = __builtin_amdgcn_rsqf(K[bb][bb] - s2);
-                    } else {
-                        G[bb][bb] = sqrt(K[bb][bb] - s2);
-                        Gi[bb] = KT(1) / G[bb][bb];
-                    }
-#pragma unroll
-                    for (int aa = bb + 1; aa < 3; ++aa) {
-                        KT s3 = KT(0);
-#pragma unroll
-                        for (int l2 = 0; l2 < bb; ++l2) s3 += G[aa][l2] * G[bb][l2];
-                        G[aa][bb] = (K[aa][bb] - s3) * Gi[bb];
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < 3; ++q) w[q] = {-cm[q].x - Cu[q].x, -cm[q].y - Cu[q].y};
-#pragma unroll
-                for (int aa = 0; aa < 3; ++aa) {
-                    KT tr = KT(0), ti = KT(0);
-#pragma unroll
-                    for (int bb = 0; bb < aa; ++bb) { tr += G[aa][bb] * w[bb].x; ti += G[aa][bb] * w[bb].y; }
-                    w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
-                }
-#pragma unroll
-                for (int aa = 2; aa >= 0; --aa) {
-                    KT tr = KT(0), ti = KT(0);
-#pragma unroll
-                    for (int bb = aa + 1; bb < 3; ++bb) { tr += w[bb].x * G[bb][aa]; ti += w[bb].y * G[bb][aa]; }
-                    w[aa] = {(w[aa].x - tr) * Gi[aa], (w[aa].y - ti) * Gi[aa]};
-                }
+                for (int q = 0; q < 3; ++q) r[q] = {cm[q].x + Cu[q].x, cm[q].y + Cu[q].y};
+                w[0] = {(a00 * r[0].x + a01 * r[1].x + a02 * r[2].x) * id, (a00 * r[0].y + a01 * r[1].y + a02 * r[2].y) * id};
+                w[1] = {(a01 * r[0].x + a11 * r[1].x + a12 * r[2].x) * id, (a01 * r[0].y + a11 * r[1].y + a12 * r[2].y) * id};
+                w[2] = {(a02 * r[0].x + a12 * r[1].x + a22 * r[2].x) * id, (a02 * r[0].y + a12 * r[1].y + a22 * r[2].y) * id};
             }
         } else {
             // acoustic MD: w(k) = i FFT(w) (acouFFT.F:19-24);

@@ -12,6 +12,7 @@
 #include "feram_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -52,163 +53,245 @@ struct fo_ctx {
     double delta_e, delta_e_acou;
     double *e_layer_dipo, *e_layer_acou;   /* Lz */
     int *n_sum_layer;                      /* Lz: counted cells per z layer */
+    /* per z-plane partial sums of the reductions (3 x Lz) */
+    double *planes;
 };
+
+/* Reductions over cells or k points run in parallel as one partial sum per
+ * z plane (each in the reference's loop order), and the planes are added in
+ * order: the result does not depend on the number of threads. */
+static double *plane_buf(fo_ctx *o, int which) { return o->planes + (size_t)which * o->p.Lz; }
+static double plane_sum(const double *part, int Lz)
+{
+    double t = 0.0;
+    for (int iz = 0; iz < Lz; ++iz) t = t + part[iz];
+    return t;
+}
 
 #define H6(a, i, j) ((a)[(i) + 6 * (j)])   /* 0-based (i,j) */
 
 /* ------------------------------------------------------------------------- */
-/* 1-D mixed-radix complex DFT (decimation in time, recursive).              */
+/* 1-D mixed-radix complex DFT: Stockham autosort passes (radix 4, 2, then   */
+/* 3, 5 and any other prime as a plain DFT of that length), applied to B     */
+/* interleaved lines at once (element t of line b at index t*B + b), with    */
+/* per-pass twiddle tables.  FFTW's conventions: sign -1 forward, +1 inverse, */
+/* neither normalised.                                                       */
 /* ------------------------------------------------------------------------- */
-static void fft_rec(const double *in, double *out, int n, int stride,
-                    const double *tw, int twn, int twstep, double *scratch)
+#define FFT_MAXPASS 40
+#define FFT_MAXRADIX 256     /* largest prime factor of a transform length */
+typedef struct {
+    int n, sign, npass;
+    int r[FFT_MAXPASS], ns[FFT_MAXPASS];
+    double *tw[FFT_MAXPASS];     /* pass s: [k < ns][q = 1 .. r-1] of e^{sign 2 pi i q k / (ns r)} */
+    double *root[FFT_MAXPASS];   /* pass s: [j < r] of e^{sign 2 pi i j / r} (plain DFT passes) */
+} fft_plan;
+
+static void plan_make(fft_plan *pl, int n, int sign)
 {
-    /* in: elements in[2*q*stride]; out: contiguous n elements.
-     * tw: table of e^{sign 2 pi i j / twn}, twstep = twn/n. */
-    if (n == 1) {
-        out[0] = in[0];
-        out[1] = in[1];
-        return;
-    }
-    int r = 0;
-    if (n % 4 == 0) r = 4;
-    else if (n % 2 == 0) r = 2;
-    else if (n % 3 == 0) r = 3;
-    else if (n % 5 == 0) r = 5;
-    else {
-        for (int f = 7; f * f <= n; f += 2)
-            if (n % f == 0) { r = f; break; }
-        if (r == 0) r = n;
-    }
-    int m = n / r;
-    if (m == 1) {
-        /* direct DFT of length r */
-        for (int s = 0; s < r; ++s) {
-            double re = 0.0, im = 0.0;
-            for (int q = 0; q < r; ++q) {
-                int idx = (int)(((long)q * s % r) * twstep % twn);
-                double wr = tw[2 * idx], wi = tw[2 * idx + 1];
-                double xr = in[2 * q * stride], xi = in[2 * q * stride + 1];
-                re += xr * wr - xi * wi;
-                im += xr * wi + xi * wr;
-            }
-            out[2 * s] = re;
-            out[2 * s + 1] = im;
+    memset(pl, 0, sizeof(*pl));
+    pl->n = n;
+    pl->sign = sign;
+    int rem = n, ns = 1;
+    while (rem > 1) {
+        int r;
+        if (rem % 4 == 0) r = 4;
+        else if (rem % 2 == 0) r = 2;
+        else {
+            r = rem;
+            for (int f = 3; f * f <= rem; f += 2)
+                if (rem % f == 0) { r = f; break; }
         }
-        return;
-    }
-    for (int q = 0; q < r; ++q)
-        fft_rec(in + 2 * q * stride, out + 2 * q * m, m, stride * r, tw, twn,
-                twstep * r, scratch + 2 * n);
-    /* combine: X[k + s m] = sum_q w_n^{q (k + s m)} Y_q[k] */
-    for (int k = 0; k < m; ++k) {
-        for (int q = 0; q < r; ++q) {
-            scratch[2 * q] = out[2 * (q * m + k)];
-            scratch[2 * q + 1] = out[2 * (q * m + k) + 1];
+        if (r > FFT_MAXRADIX) {
+            fprintf(stderr, "feram_oracle: FFT length %d has a prime factor above %d\n", n, FFT_MAXRADIX);
+            abort();
         }
-        for (int s = 0; s < r; ++s) {
-            double re = 0.0, im = 0.0;
-            int kk = k + s * m;
-            for (int q = 0; q < r; ++q) {
-                int idx = (int)(((long)q * kk % n) * twstep % twn);
-                double wr = tw[2 * idx], wi = tw[2 * idx + 1];
-                double xr = scratch[2 * q], xi = scratch[2 * q + 1];
-                re += xr * wr - xi * wi;
-                im += xr * wi + xi * wr;
+        const int s = pl->npass++;
+        pl->r[s] = r;
+        pl->ns[s] = ns;
+        pl->tw[s] = (double *)malloc(sizeof(double) * 2 * (size_t)ns * (r > 1 ? r - 1 : 1));
+        for (int k = 0; k < ns; ++k)
+            for (int q = 1; q < r; ++q) {
+                const double a = M_2PI_D * (double)(q * k) / (double)(ns * r);
+                pl->tw[s][2 * (k * (r - 1) + q - 1)] = cos(a);
+                pl->tw[s][2 * (k * (r - 1) + q - 1) + 1] = sign * sin(a);
             }
-            out[2 * kk] = re;
-            out[2 * kk + 1] = im;
+        pl->root[s] = (double *)malloc(sizeof(double) * 2 * r);
+        for (int j = 0; j < r; ++j) {
+            const double a = M_2PI_D * (double)j / (double)r;
+            pl->root[s][2 * j] = cos(a);
+            pl->root[s][2 * j + 1] = sign * sin(a);
+        }
+        ns *= r;
+        rem /= r;
+    }
+}
+
+static void plan_free(fft_plan *pl)
+{
+    for (int s = 0; s < pl->npass; ++s) { free(pl->tw[s]); free(pl->root[s]); }
+    pl->npass = 0;
+}
+
+/* one Stockham pass of radix r at stride ns on B lines: in -> out */
+static void fft_pass(const fft_plan *pl, int s, const double *in, double *out, int B)
+{
+    const int n = pl->n, r = pl->r[s], ns = pl->ns[s], m = n / r, sg = pl->sign;
+    const double *tw = pl->tw[s], *root = pl->root[s];
+    double vr[FFT_MAXRADIX], vi[FFT_MAXRADIX];
+    for (int j = 0; j < m; ++j) {
+        const int k = j % ns;
+        const int od = (j / ns) * ns * r + k;
+        const double *w = tw + 2 * k * (r - 1);
+        for (int b = 0; b < B; ++b) {
+            for (int q = 0; q < r; ++q) {
+                const size_t ii = 2 * ((size_t)(j + q * m) * B + b);
+                double xr = in[ii], xi = in[ii + 1];
+                if (q > 0 && ns > 1) {
+                    const double wr = w[2 * (q - 1)], wi = w[2 * (q - 1) + 1];
+                    const double tr = xr * wr - xi * wi;
+                    xi = xr * wi + xi * wr;
+                    xr = tr;
+                }
+                vr[q] = xr;
+                vi[q] = xi;
+            }
+            if (r == 2) {
+                const double ar = vr[0], ai = vi[0];
+                vr[0] = ar + vr[1]; vi[0] = ai + vi[1];
+                vr[1] = ar - vr[1]; vi[1] = ai - vi[1];
+            } else if (r == 4) {
+                /* X1 = (v0 - v2) + w (v1 - v3), w = sign i */
+                const double s02r = vr[0] + vr[2], s02i = vi[0] + vi[2];
+                const double d02r = vr[0] - vr[2], d02i = vi[0] - vi[2];
+                const double s13r = vr[1] + vr[3], s13i = vi[1] + vi[3];
+                const double d13r = vr[1] - vr[3], d13i = vi[1] - vi[3];
+                const double er = -sg * d13i, ei = sg * d13r;     /* (sign i) d13 */
+                vr[0] = s02r + s13r; vi[0] = s02i + s13i;
+                vr[2] = s02r - s13r; vi[2] = s02i - s13i;
+                vr[1] = d02r + er;   vi[1] = d02i + ei;
+                vr[3] = d02r - er;   vi[3] = d02i - ei;
+            } else {
+                double xr[FFT_MAXRADIX], xi[FFT_MAXRADIX];
+                for (int q = 0; q < r; ++q) { xr[q] = vr[q]; xi[q] = vi[q]; }
+                for (int t = 0; t < r; ++t) {
+                    double ar = 0.0, ai = 0.0;
+                    for (int q = 0; q < r; ++q) {
+                        const int e = (q * t) % r;
+                        const double wr = root[2 * e], wi = root[2 * e + 1];
+                        ar += xr[q] * wr - xi[q] * wi;
+                        ai += xr[q] * wi + xi[q] * wr;
+                    }
+                    vr[t] = ar;
+                    vi[t] = ai;
+                }
+            }
+            for (int q = 0; q < r; ++q) {
+                const size_t oo = 2 * ((size_t)(od + q * ns) * B + b);
+                out[oo] = vr[q];
+                out[oo + 1] = vi[q];
+            }
         }
     }
 }
 
-static double *make_twiddles(int n, int sign)
+/* transform B interleaved lines held in buf (work: same size); result in buf */
+static void fft_lines(const fft_plan *pl, double *buf, double *work, int B)
 {
-    double *tw = (double *)malloc(sizeof(double) * 2 * n);
-    for (int j = 0; j < n; ++j) {
-        /* exact octant-reduced angle keeps the table symmetric */
-        double a = M_2PI_D * (double)j / (double)n;
-        tw[2 * j] = cos(a);
-        tw[2 * j + 1] = sign * sin(a);
+    double *a = buf, *b = work;
+    for (int s = 0; s < pl->npass; ++s) {
+        fft_pass(pl, s, a, b, B);
+        double *t = a; a = b; b = t;
     }
-    return tw;
+    if (a != buf) memcpy(buf, a, sizeof(double) * 2 * (size_t)pl->n * B);
 }
 
 void fo_fft1d(double *x, int n, int sign)
 {
-    double *tw = make_twiddles(n, sign);
-    double *out = (double *)malloc(sizeof(double) * 2 * n);
-    double *scr = (double *)malloc(sizeof(double) * 2 * n * 4 + 64);
-    fft_rec(x, out, n, 1, tw, n, 1, scr);
-    memcpy(x, out, sizeof(double) * 2 * n);
-    free(tw); free(out); free(scr);
+    fft_plan pl;
+    plan_make(&pl, n, sign);
+    double *work = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    fft_lines(&pl, x, work, 1);
+    free(work);
+    plan_free(&pl);
 }
 
-/* line transform with a prebuilt table; buf holds n complex, work >= 10 n */
-static void fft_line(double *buf, int n, const double *tw, double *work)
-{
-    fft_rec(buf, work, n, 1, tw, n, 1, work + 2 * n);
-    memcpy(buf, work, sizeof(double) * 2 * n);
-}
+/* lines transformed together in the 3-D passes (cache lines of the strided
+ * y and z gathers are used whole) */
+#define FFT_B 8
 
 /* Rank-3 r2c with FFTW's n=(Lz,Ly,Lx) row-major semantics: x is the real
  * (halved) dimension (coord_module.F:151-157, dipoFFT.F:19). */
 void fo_r2c(fo_ctx *o, const double *in, int ncomp, int comp, double *out)
 {
     const int Lx = o->p.Lx, Ly = o->p.Ly, Lz = o->p.Lz, Kx = o->Kx;
-    double *twx = make_twiddles(Lx, -1), *twy = make_twiddles(Ly, -1),
-           *twz = make_twiddles(Lz, -1);
+    fft_plan px, py, pz;
+    plan_make(&px, Lx, -1);
+    plan_make(&py, Ly, -1);
+    plan_make(&pz, Lz, -1);
     int nmax = Lx > Ly ? Lx : Ly; if (Lz > nmax) nmax = Lz;
+    const int nyb = (Ly + FFT_B - 1) / FFT_B, nkb = (Kx + FFT_B - 1) / FFT_B;
+#define SPEC(kx, y, z) (out + 2 * ((size_t)ncomp * ((kx) + (size_t)Kx * ((y) + (size_t)Ly * (z))) + comp))
 #pragma omp parallel
     {
-        double *buf = (double *)malloc(sizeof(double) * 2 * nmax);
-        double *work = (double *)malloc(sizeof(double) * 2 * nmax * 12 + 64);
+        double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)nmax * FFT_B);
+        double *work = (double *)malloc(sizeof(double) * 2 * (size_t)nmax * FFT_B);
 #pragma omp for collapse(2) schedule(static)
         for (int z = 0; z < Lz; ++z)
-            for (int y = 0; y < Ly; ++y) {
-                for (int x = 0; x < Lx; ++x) {
-                    buf[2 * x] = in[(size_t)ncomp * (x + Lx * (y + (size_t)Ly * z)) + comp];
-                    buf[2 * x + 1] = 0.0;
-                }
-                fft_line(buf, Lx, twx, work);
-                for (int kx = 0; kx < Kx; ++kx) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    out[2 * (ncomp * k + comp)] = buf[2 * kx];
-                    out[2 * (ncomp * k + comp) + 1] = buf[2 * kx + 1];
-                }
+            for (int yb = 0; yb < nyb; ++yb) {
+                const int y0 = yb * FFT_B, B = (Ly - y0 < FFT_B) ? Ly - y0 : FFT_B;
+                for (int x = 0; x < Lx; ++x)
+                    for (int b = 0; b < B; ++b) {
+                        buf[2 * (x * B + b)] = in[(size_t)ncomp * (x + Lx * (y0 + b + (size_t)Ly * z)) + comp];
+                        buf[2 * (x * B + b) + 1] = 0.0;
+                    }
+                fft_lines(&px, buf, work, B);
+                for (int b = 0; b < B; ++b)
+                    for (int kx = 0; kx < Kx; ++kx) {
+                        double *d = SPEC(kx, y0 + b, z);
+                        d[0] = buf[2 * (kx * B + b)];
+                        d[1] = buf[2 * (kx * B + b) + 1];
+                    }
             }
 #pragma omp for collapse(2) schedule(static)
         for (int z = 0; z < Lz; ++z)
-            for (int kx = 0; kx < Kx; ++kx) {
-                for (int y = 0; y < Ly; ++y) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    buf[2 * y] = out[2 * (ncomp * k + comp)];
-                    buf[2 * y + 1] = out[2 * (ncomp * k + comp) + 1];
-                }
-                fft_line(buf, Ly, twy, work);
-                for (int y = 0; y < Ly; ++y) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    out[2 * (ncomp * k + comp)] = buf[2 * y];
-                    out[2 * (ncomp * k + comp) + 1] = buf[2 * y + 1];
-                }
+            for (int kb = 0; kb < nkb; ++kb) {
+                const int k0 = kb * FFT_B, B = (Kx - k0 < FFT_B) ? Kx - k0 : FFT_B;
+                for (int y = 0; y < Ly; ++y)
+                    for (int b = 0; b < B; ++b) {
+                        const double *d = SPEC(k0 + b, y, z);
+                        buf[2 * (y * B + b)] = d[0];
+                        buf[2 * (y * B + b) + 1] = d[1];
+                    }
+                fft_lines(&py, buf, work, B);
+                for (int y = 0; y < Ly; ++y)
+                    for (int b = 0; b < B; ++b) {
+                        double *d = SPEC(k0 + b, y, z);
+                        d[0] = buf[2 * (y * B + b)];
+                        d[1] = buf[2 * (y * B + b) + 1];
+                    }
             }
 #pragma omp for collapse(2) schedule(static)
         for (int y = 0; y < Ly; ++y)
-            for (int kx = 0; kx < Kx; ++kx) {
-                for (int z = 0; z < Lz; ++z) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    buf[2 * z] = out[2 * (ncomp * k + comp)];
-                    buf[2 * z + 1] = out[2 * (ncomp * k + comp) + 1];
-                }
-                fft_line(buf, Lz, twz, work);
-                for (int z = 0; z < Lz; ++z) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    out[2 * (ncomp * k + comp)] = buf[2 * z];
-                    out[2 * (ncomp * k + comp) + 1] = buf[2 * z + 1];
-                }
+            for (int kb = 0; kb < nkb; ++kb) {
+                const int k0 = kb * FFT_B, B = (Kx - k0 < FFT_B) ? Kx - k0 : FFT_B;
+                for (int z = 0; z < Lz; ++z)
+                    for (int b = 0; b < B; ++b) {
+                        const double *d = SPEC(k0 + b, y, z);
+                        buf[2 * (z * B + b)] = d[0];
+                        buf[2 * (z * B + b) + 1] = d[1];
+                    }
+                fft_lines(&pz, buf, work, B);
+                for (int z = 0; z < Lz; ++z)
+                    for (int b = 0; b < B; ++b) {
+                        double *d = SPEC(k0 + b, y, z);
+                        d[0] = buf[2 * (z * B + b)];
+                        d[1] = buf[2 * (z * B + b) + 1];
+                    }
             }
         free(buf); free(work);
     }
-    free(twx); free(twy); free(twz);
+#undef SPEC
+    plan_free(&px); plan_free(&py); plan_free(&pz);
 }
 
 /* Rank-3 c2r (unnormalised): complex inverse along z and y, then the
@@ -218,64 +301,84 @@ void fo_c2r(fo_ctx *o, const double *in, int ncomp, int comp, double *out)
     const int Lx = o->p.Lx, Ly = o->p.Ly, Lz = o->p.Lz, Kx = o->Kx;
     const size_t Kh = (size_t)o->Kh;
     double *tmp = (double *)malloc(sizeof(double) * 2 * Kh);
-    double *twx = make_twiddles(Lx, +1), *twy = make_twiddles(Ly, +1),
-           *twz = make_twiddles(Lz, +1);
+    fft_plan px, py, pz;
+    plan_make(&px, Lx, +1);
+    plan_make(&py, Ly, +1);
+    plan_make(&pz, Lz, +1);
     int nmax = Lx > Ly ? Lx : Ly; if (Lz > nmax) nmax = Lz;
+    const int nyb = (Ly + FFT_B - 1) / FFT_B, nkb = (Kx + FFT_B - 1) / FFT_B;
     for (size_t k = 0; k < Kh; ++k) {
         tmp[2 * k] = in[2 * (ncomp * k + comp)];
         tmp[2 * k + 1] = in[2 * (ncomp * k + comp) + 1];
     }
+#define TMP(kx, y, z) (tmp + 2 * ((kx) + (size_t)Kx * ((y) + (size_t)Ly * (z))))
 #pragma omp parallel
     {
-        double *buf = (double *)malloc(sizeof(double) * 2 * nmax);
-        double *work = (double *)malloc(sizeof(double) * 2 * nmax * 12 + 64);
+        double *buf = (double *)malloc(sizeof(double) * 2 * (size_t)nmax * FFT_B);
+        double *work = (double *)malloc(sizeof(double) * 2 * (size_t)nmax * FFT_B);
 #pragma omp for collapse(2) schedule(static)
         for (int y = 0; y < Ly; ++y)
-            for (int kx = 0; kx < Kx; ++kx) {
-                for (int z = 0; z < Lz; ++z) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    buf[2 * z] = tmp[2 * k]; buf[2 * z + 1] = tmp[2 * k + 1];
-                }
-                fft_line(buf, Lz, twz, work);
-                for (int z = 0; z < Lz; ++z) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    tmp[2 * k] = buf[2 * z]; tmp[2 * k + 1] = buf[2 * z + 1];
-                }
+            for (int kb = 0; kb < nkb; ++kb) {
+                const int k0 = kb * FFT_B, B = (Kx - k0 < FFT_B) ? Kx - k0 : FFT_B;
+                for (int z = 0; z < Lz; ++z)
+                    for (int b = 0; b < B; ++b) {
+                        const double *d = TMP(k0 + b, y, z);
+                        buf[2 * (z * B + b)] = d[0];
+                        buf[2 * (z * B + b) + 1] = d[1];
+                    }
+                fft_lines(&pz, buf, work, B);
+                for (int z = 0; z < Lz; ++z)
+                    for (int b = 0; b < B; ++b) {
+                        double *d = TMP(k0 + b, y, z);
+                        d[0] = buf[2 * (z * B + b)];
+                        d[1] = buf[2 * (z * B + b) + 1];
+                    }
             }
 #pragma omp for collapse(2) schedule(static)
         for (int z = 0; z < Lz; ++z)
-            for (int kx = 0; kx < Kx; ++kx) {
-                for (int y = 0; y < Ly; ++y) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    buf[2 * y] = tmp[2 * k]; buf[2 * y + 1] = tmp[2 * k + 1];
-                }
-                fft_line(buf, Ly, twy, work);
-                for (int y = 0; y < Ly; ++y) {
-                    size_t k = kx + (size_t)Kx * (y + (size_t)Ly * z);
-                    tmp[2 * k] = buf[2 * y]; tmp[2 * k + 1] = buf[2 * y + 1];
-                }
+            for (int kb = 0; kb < nkb; ++kb) {
+                const int k0 = kb * FFT_B, B = (Kx - k0 < FFT_B) ? Kx - k0 : FFT_B;
+                for (int y = 0; y < Ly; ++y)
+                    for (int b = 0; b < B; ++b) {
+                        const double *d = TMP(k0 + b, y, z);
+                        buf[2 * (y * B + b)] = d[0];
+                        buf[2 * (y * B + b) + 1] = d[1];
+                    }
+                fft_lines(&py, buf, work, B);
+                for (int y = 0; y < Ly; ++y)
+                    for (int b = 0; b < B; ++b) {
+                        double *d = TMP(k0 + b, y, z);
+                        d[0] = buf[2 * (y * B + b)];
+                        d[1] = buf[2 * (y * B + b) + 1];
+                    }
             }
 #pragma omp for collapse(2) schedule(static)
         for (int z = 0; z < Lz; ++z)
-            for (int y = 0; y < Ly; ++y) {
-                size_t k0 = (size_t)Kx * (y + (size_t)Ly * z);
-                for (int kx = 0; kx < Kx; ++kx) {
-                    buf[2 * kx] = tmp[2 * (k0 + kx)];
-                    buf[2 * kx + 1] = tmp[2 * (k0 + kx) + 1];
+            for (int yb = 0; yb < nyb; ++yb) {
+                const int y0 = yb * FFT_B, B = (Ly - y0 < FFT_B) ? Ly - y0 : FFT_B;
+                for (int b = 0; b < B; ++b) {
+                    const double *row = TMP(0, y0 + b, z);
+                    for (int kx = 0; kx < Kx; ++kx) {
+                        buf[2 * (kx * B + b)] = row[2 * kx];
+                        buf[2 * (kx * B + b) + 1] = row[2 * kx + 1];
+                    }
+                    buf[2 * b + 1] = 0.0;
+                    if (Lx % 2 == 0) buf[2 * ((Lx / 2) * B + b) + 1] = 0.0;
+                    for (int kx = Kx; kx < Lx; ++kx) {
+                        buf[2 * (kx * B + b)] = buf[2 * ((Lx - kx) * B + b)];
+                        buf[2 * (kx * B + b) + 1] = -buf[2 * ((Lx - kx) * B + b) + 1];
+                    }
                 }
-                buf[1] = 0.0;
-                if (Lx % 2 == 0) buf[2 * (Lx / 2) + 1] = 0.0;
-                for (int kx = Kx; kx < Lx; ++kx) {
-                    buf[2 * kx] = buf[2 * (Lx - kx)];
-                    buf[2 * kx + 1] = -buf[2 * (Lx - kx) + 1];
-                }
-                fft_line(buf, Lx, twx, work);
-                for (int x = 0; x < Lx; ++x)
-                    out[(size_t)ncomp * (x + Lx * (y + (size_t)Ly * z)) + comp] = buf[2 * x];
+                fft_lines(&px, buf, work, B);
+                for (int b = 0; b < B; ++b)
+                    for (int x = 0; x < Lx; ++x)
+                        out[(size_t)ncomp * (x + Lx * (y0 + b + (size_t)Ly * z)) + comp] = buf[2 * (x * B + b)];
             }
         free(buf); free(work);
     }
-    free(tmp); free(twx); free(twy); free(twz);
+#undef TMP
+    free(tmp);
+    plan_free(&px); plan_free(&py); plan_free(&pz);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -327,6 +430,7 @@ fo_ctx *fo_create(const fo_params *pp)
     o->inhomo_coup2_K = zalloc(18 * Kh);
     o->e_layer_dipo = zalloc(p->Lz); o->e_layer_acou = zalloc(p->Lz);
     o->n_sum_layer = (int *)calloc(p->Lz, sizeof(int));
+    o->planes = zalloc(3 * (size_t)p->Lz);
     /* alloc_Coord (coord_module.F:99-143) and read_defects_u defaults */
     for (size_t c = 0; c < N; ++c) {
         o->mass_dipo_inv[c] = 1.0 / p->mass_dipo[0];
@@ -384,7 +488,7 @@ void fo_destroy(fo_ctx *o)
                       o->c_k_modulation_k, o->long_range, o->short_range, o->inhomo_K,
                       o->inhomo_K_G, o->inhomo_coup2_K};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
-    free(o->e_layer_dipo); free(o->e_layer_acou); free(o->n_sum_layer);
+    free(o->e_layer_dipo); free(o->e_layer_acou); free(o->n_sum_layer); free(o->planes);
     free(o->relaxor); free(o->sum_p); free(o);
 }
 
@@ -717,6 +821,7 @@ static void dipoFFT(fo_ctx *o)
 {
     const size_t N = o->N;
     for (int a = 0; a < 3; ++a) fo_r2c(o, o->dipoR, 3, a, o->dipoK);
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         const double *u = &o->dipoR[3 * c];
         double *d = &o->didiR[6 * c];
@@ -724,6 +829,7 @@ static void dipoFFT(fo_ctx *o)
         d[3] = u[1] * u[2]; d[4] = u[2] * u[0]; d[5] = u[0] * u[1];
     }
     for (int j = 0; j < 6; ++j) fo_r2c(o, o->didiR, 6, j, o->didiK);
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c)
         for (int a = 0; a < 3; ++a) o->Z_u_R[3 * c + a] = o->Z_starR[c] * o->dipoR[3 * c + a];
     for (int a = 0; a < 3; ++a) fo_r2c(o, o->Z_u_R, 3, a, o->Z_u_K);
@@ -744,6 +850,7 @@ static void acouFFT(fo_ctx *o)
 static void dVddi_dipole_dipole(fo_ctx *o, fo_energy *e)
 {
     const size_t N = o->N, Kh = o->Kh;
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < Kh; ++k)
         for (int a = 0; a < 3; ++a) {
             double re = 0.0, im = 0.0;
@@ -755,8 +862,10 @@ static void dVddi_dipole_dipole(fo_ctx *o, fo_energy *e)
             o->dipoT[2 * (3 * k + a)] = re; o->dipoT[2 * (3 * k + a) + 1] = im;
         }
     for (int a = 0; a < 3; ++a) fo_c2r(o, o->dipoT, 3, a, o->dVddi);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < 3 * N; ++i) o->dVddi[i] = o->dVddi[i] * o->Ninv;
 
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < Kh; ++k)
         for (int a = 0; a < 3; ++a) {
             double re = 0.0, im = 0.0;
@@ -768,21 +877,28 @@ static void dVddi_dipole_dipole(fo_ctx *o, fo_energy *e)
             o->dipoT[2 * (3 * k + a)] = re; o->dipoT[2 * (3 * k + a) + 1] = im;
         }
     for (int a = 0; a < 3; ++a) fo_c2r(o, o->dipoT, 3, a, o->dVddi_long);
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c)
         for (int a = 0; a < 3; ++a)
             o->dVddi_long[3 * c + a] = o->Z_starR[c] * o->dVddi_long[3 * c + a] * o->Ninv;
 
-    double esr = 0.0, elr = 0.0;
-    for (size_t c = 0; c < N; ++c) {
-        const double *u = &o->dipoR[3 * c], *f = &o->dVddi[3 * c];
-        esr = esr + (u[0] * f[0] + u[1] * f[1] + u[2] * f[2]);
+    const int Lz = o->p.Lz;
+    const size_t NP = N / Lz;
+    double *esr = plane_buf(o, 0), *elr = plane_buf(o, 1);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < Lz; ++iz) {
+        double ts = 0.0, tl = 0.0;
+        for (size_t c = iz * NP; c < (iz + 1) * NP; ++c) {
+            const double *u = &o->dipoR[3 * c], *f = &o->dVddi[3 * c], *g = &o->dVddi_long[3 * c];
+            ts = ts + (u[0] * f[0] + u[1] * f[1] + u[2] * f[2]);
+            tl = tl + (u[0] * g[0] + u[1] * g[1] + u[2] * g[2]);
+        }
+        esr[iz] = ts;
+        elr[iz] = tl;
     }
-    e->short_range = esr / 2;
-    for (size_t c = 0; c < N; ++c) {
-        const double *u = &o->dipoR[3 * c], *f = &o->dVddi_long[3 * c];
-        elr = elr + (u[0] * f[0] + u[1] * f[1] + u[2] * f[2]);
-    }
-    e->long_range = elr / 2;
+    e->short_range = plane_sum(esr, Lz) / 2;
+    e->long_range = plane_sum(elr, Lz) / 2;
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < 3 * N; ++i) o->dVddi[i] = o->dVddi[i] + o->dVddi_long[i];
 }
 
@@ -799,6 +915,7 @@ static void dVddi_unharmonic(fo_ctx *o, fo_energy *e)
         tk2[r] = 2 * p->P_k2[r];
         fk[r] = 4 * p->P_k2[r] - 2 * p->P_k3[r];
     }
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c) {
         int r = o->relaxor[c] - 1;
         const double *u = &o->dipoR[3 * c], *d = &o->didiR[6 * c];
@@ -812,8 +929,13 @@ static void dVddi_unharmonic(fo_ctx *o, fo_energy *e)
                                   - tk2[r] * (d[a] * d[a]) - fk[r] * d[a1] * d[a2]);
         }
     }
-    double unharm0 = 0.0;
-    for (size_t c = 0; c < N; ++c) {
+    const int Lz = p->Lz;
+    const size_t NP = N / Lz;
+    double *part = plane_buf(o, 0);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < Lz; ++iz) {
+      double unharm0 = 0.0;
+      for (size_t c = iz * NP; c < (iz + 1) * NP; ++c) {
         int r = o->relaxor[c] - 1;
         const double *d = &o->didiR[6 * c];
         double s = d[0] + d[1] + d[2];
@@ -826,8 +948,10 @@ static void dVddi_unharmonic(fo_ctx *o, fo_energy *e)
                             + (d[2] * d[2]) * (d[0] + d[1]))
             + p->P_k3[r] * d[0] * d[1] * d[2]
             + p->P_k4[r] * ((s * s) * (s * s));
+      }
+      part[iz] = unharm0;
     }
-    e->unharmonic = unharm0;
+    e->unharmonic = plane_sum(part, Lz);
 }
 
 /* optimize_homo_strain (optimize-homo-strain.F:8-89) */
@@ -899,6 +1023,7 @@ static void dVddi_homo_coupling(fo_ctx *o)
     for (int j = 0; j < 6; ++j)
         for (int i = 0; i < 6; ++i)
             e[j] = e[j] + o->strain[i] * H6(o->homogeneous_coupling_2, i, j);
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < (size_t)o->N; ++c)
         add_coupling_force(&o->dVddi[3 * c], &o->dipoR[3 * c], e);
 }
@@ -906,6 +1031,7 @@ static void dVddi_homo_coupling(fo_ctx *o)
 /* optimize_inho_strain (optimize-inho-strain.F:10-60) */
 static void optimize_inho_strain(fo_ctx *o)
 {
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < (size_t)o->Kh; ++k) {
         const double *C = &o->inhomo_coup2_K[18 * k], *G = &o->inhomo_K_G[12 * k];
         const double *dd = &o->didiK[12 * k], *cm = &o->c_k_modulation_k[6 * k];
@@ -941,6 +1067,7 @@ static void optimize_inho_strain(fo_ctx *o)
 /* dVdac (dVdac.F:11-63) */
 static void dVdac(fo_ctx *o)
 {
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < (size_t)o->Kh; ++k) {
         const double *C = &o->inhomo_coup2_K[18 * k], *K = &o->inhomo_K[9 * k];
         const double *dd = &o->didiK[12 * k], *cm = &o->c_k_modulation_k[6 * k];
@@ -957,6 +1084,7 @@ static void dVdac(fo_ctx *o)
         }
     }
     for (int a = 0; a < 3; ++a) fo_c2r(o, o->dVdacK, 3, a, o->dVdacR);
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < 3 * (size_t)o->N; ++i) o->dVdacR[i] = o->dVdacR[i] * o->Ninv;
 }
 
@@ -966,13 +1094,16 @@ static void inhomogeneous_energies(fo_ctx *o, fo_energy *e)
 {
     const fo_params *p = &o->p;
     const int Lx = p->Lx, Ly = p->Ly, Lz = p->Lz, Kx = o->Kx;
-    double tS = 0.0;
-    for (int iz = 0; iz < Lz; ++iz)
+    double *pS = plane_buf(o, 0), *pC = plane_buf(o, 1), *pM = plane_buf(o, 2);
+    const int modulation = p->modulation_constant != 0.0;
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < Lz; ++iz) {
+        const int xs[3] = {0, 1, Lx / 2};
+        double tS = 0.0, tC = 0.0, tM = 0.0;
         for (int iy = 0; iy < Ly; ++iy)
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) {
                     double t[3];
-                    int xs[3] = {0, 1, Lx / 2};
                     for (int part = 0; part < 3; ++part) {
                         int x0 = part == 1 ? 1 : xs[part], x1 = part == 1 ? Lx / 2 - 1 : xs[part];
                         double s = 0.0;
@@ -988,15 +1119,10 @@ static void inhomogeneous_energies(fo_ctx *o, fo_energy *e)
                     }
                     tS = tS + t[0] / 2 + t[1] + t[2] / 2;
                 }
-    e->inho_strain = tS * o->Ninv;
-
-    tS = 0.0;
-    for (int iz = 0; iz < Lz; ++iz)
         for (int iy = 0; iy < Ly; ++iy)
             for (int i = 0; i < 6; ++i)
                 for (int a = 0; a < 3; ++a) {
                     double t[3];
-                    int xs[3] = {0, 1, Lx / 2};
                     for (int part = 0; part < 3; ++part) {
                         int x0 = part == 1 ? 1 : xs[part], x1 = part == 1 ? Lx / 2 - 1 : xs[part];
                         double s = 0.0;
@@ -1009,33 +1135,31 @@ static void inhomogeneous_energies(fo_ctx *o, fo_energy *e)
                         }
                         t[part] = s;
                     }
-                    tS = tS + t[0] / 2 + t[1] + t[2] / 2;
+                    tC = tC + t[0] / 2 + t[1] + t[2] / 2;
                 }
-    e->inho_coupling = tS * 2 * o->Ninv;
-
-    if (p->modulation_constant == 0.0) {
-        e->inho_modulation = 0.0;
-        return;
-    }
-    tS = 0.0;
-    for (int iz = 0; iz < Lz; ++iz)
-        for (int iy = 0; iy < Ly; ++iy)
-            for (int a = 0; a < 3; ++a) {
-                double t[3];
-                int xs[3] = {0, 1, Lx / 2};
-                for (int part = 0; part < 3; ++part) {
-                    int x0 = part == 1 ? 1 : xs[part], x1 = part == 1 ? Lx / 2 - 1 : xs[part];
-                    double s = 0.0;
-                    for (int ix = x0; ix <= x1; ++ix) {
-                        size_t k = ix + (size_t)Kx * (iy + (size_t)Ly * iz);
-                        const double *w = &o->acouK[6 * k], *cm = &o->c_k_modulation_k[6 * k];
-                        s = s + (w[2 * a] * cm[2 * a] + w[2 * a + 1] * cm[2 * a + 1]);
+        if (modulation)
+            for (int iy = 0; iy < Ly; ++iy)
+                for (int a = 0; a < 3; ++a) {
+                    double t[3];
+                    for (int part = 0; part < 3; ++part) {
+                        int x0 = part == 1 ? 1 : xs[part], x1 = part == 1 ? Lx / 2 - 1 : xs[part];
+                        double s = 0.0;
+                        for (int ix = x0; ix <= x1; ++ix) {
+                            size_t k = ix + (size_t)Kx * (iy + (size_t)Ly * iz);
+                            const double *w = &o->acouK[6 * k], *cm = &o->c_k_modulation_k[6 * k];
+                            s = s + (w[2 * a] * cm[2 * a] + w[2 * a + 1] * cm[2 * a + 1]);
+                        }
+                        t[part] = s;
                     }
-                    t[part] = s;
+                    tM = tM + t[0] / 2 + t[1] + t[2] / 2;
                 }
-                tS = tS + t[0] / 2 + t[1] + t[2] / 2;
-            }
-    e->inho_modulation = tS * 2 * o->Ninv;
+        pS[iz] = tS;
+        pC[iz] = tC;
+        pM[iz] = tM;
+    }
+    e->inho_strain = plane_sum(pS, Lz) * o->Ninv;
+    e->inho_coupling = plane_sum(pC, Lz) * 2 * o->Ninv;
+    e->inho_modulation = modulation ? plane_sum(pM, Lz) * 2 * o->Ninv : 0.0;
 }
 
 /* dVddi_inho_coupling (dVddi_inho_coupling.F:9-84) */
@@ -1043,6 +1167,7 @@ static void dVddi_inho_coupling(fo_ctx *o)
 {
     const fo_params *p = &o->p;
     const int Lx = p->Lx, Ly = p->Ly, Lz = p->Lz, Kx = o->Kx;
+#pragma omp parallel for schedule(static)
     for (int iz = 0; iz < Lz; ++iz)
         for (int iy = 0; iy < Ly; ++iy)
             for (int ix = 0; ix <= Lx / 2; ++ix) {
@@ -1062,6 +1187,7 @@ static void dVddi_inho_coupling(fo_ctx *o)
             }
     for (int j = 0; j < 6; ++j) fo_c2r(o, o->inhoK, 6, j, o->inhoR);
     const double *C2 = o->homogeneous_coupling_2;
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < (size_t)o->N; ++c) {
         const double *h = &o->inhoR[6 * c];
         double e[6];
@@ -1086,15 +1212,23 @@ void fo_mirror_force_mirror(fo_ctx *o, fo_energy *e)
     if (p->E_field_always_zero) {
         e->dipole_E_field = 0.0;
     } else {
+#pragma omp parallel for schedule(static)
         for (size_t c = 0; c < (size_t)o->N; ++c)
             for (int a = 0; a < 3; ++a)
                 o->dVddi[3 * c + a] = o->dVddi[3 * c + a] - o->Z_starR[c] * o->E_field[3 * c + a];
-        double t = 0.0;
-        for (size_t c = 0; c < (size_t)o->N; ++c) {
-            const double *u = &o->dipoR[3 * c], *E = &o->E_field[3 * c];
-            t = t + o->Z_starR[c] * (u[0] * E[0] + u[1] * E[1] + u[2] * E[2]);
+        const int Lz = p->Lz;
+        const size_t NP = (size_t)o->N / Lz;
+        double *part = plane_buf(o, 0);
+#pragma omp parallel for schedule(static)
+        for (int iz = 0; iz < Lz; ++iz) {
+            double t = 0.0;
+            for (size_t c = iz * NP; c < (iz + 1) * NP; ++c) {
+                const double *u = &o->dipoR[3 * c], *E = &o->E_field[3 * c];
+                t = t + o->Z_starR[c] * (u[0] * E[0] + u[1] * E[1] + u[2] * E[2]);
+            }
+            part[iz] = t;
         }
-        e->dipole_E_field = -t;
+        e->dipole_E_field = -plane_sum(part, Lz);
     }
     optimize_homo_strain(o, e);
     dVddi_homo_coupling(o);
@@ -1110,20 +1244,34 @@ void fo_mirror_force_mirror(fo_ctx *o, fo_energy *e)
 /* ------------------------------------------------------------------------- */
 double fo_kinetic_energy_dipo(fo_ctx *o)
 {
-    double t = 0.0;
-    for (size_t c = 0; c < (size_t)o->N; ++c)
-        for (int a = 0; a < 3; ++a)
-            if (o->sum_p[c]) t = t + o->dipoP[3 * c + a] * o->dipoP[3 * c + a] * o->mass_dipo_inv[c];
-    return t / o->n_sum_p * o->N / 2;
+    const int Lz = o->p.Lz;
+    const size_t NP = (size_t)o->N / Lz;
+    double *part = plane_buf(o, 2);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < Lz; ++iz) {
+        double t = 0.0;
+        for (size_t c = iz * NP; c < (iz + 1) * NP; ++c)
+            for (int a = 0; a < 3; ++a)
+                if (o->sum_p[c]) t = t + o->dipoP[3 * c + a] * o->dipoP[3 * c + a] * o->mass_dipo_inv[c];
+        part[iz] = t;
+    }
+    return plane_sum(part, Lz) / o->n_sum_p * o->N / 2;
 }
 
 double fo_kinetic_energy_acou(fo_ctx *o)
 {
-    double t = 0.0;
-    for (size_t c = 0; c < (size_t)o->N; ++c)
-        for (int a = 0; a < 3; ++a)
-            t = t + o->acouP[3 * c + a] * o->acouP[3 * c + a] * o->mass_acou_inv[c];
-    return t / 2;
+    const int Lz = o->p.Lz;
+    const size_t NP = (size_t)o->N / Lz;
+    double *part = plane_buf(o, 2);
+#pragma omp parallel for schedule(static)
+    for (int iz = 0; iz < Lz; ++iz) {
+        double t = 0.0;
+        for (size_t c = iz * NP; c < (iz + 1) * NP; ++c)
+            for (int a = 0; a < 3; ++a)
+                t = t + o->acouP[3 * c + a] * o->acouP[3 * c + a] * o->mass_acou_inv[c];
+        part[iz] = t;
+    }
+    return plane_sum(part, Lz) / 2;
 }
 
 double fo_sum_potential_energies(const fo_energy *e)
@@ -1147,6 +1295,7 @@ void fo_nose_poincare(fo_ctx *o, fo_energy *e)
     const size_t N = o->N;
     double *s = &o->nose[0], *pi = &o->nose[1];
     double dt2s = o->dt_2 * *s;
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < 3 * N; ++i) o->dipoP[i] = o->dipoP[i] - dt2s * o->dVddi[i];
     e->dipo_kinetic = fo_kinetic_energy_dipo(o);
     double CB = (double)o->n_freedom * o->temperature_eV * (1.0 + log(*s))
@@ -1156,6 +1305,7 @@ void fo_nose_poincare(fo_ctx *o, fo_energy *e)
     double s_old = *s;
     *s = *s * (1.0 + o->dt_2 * *pi / p->Q_Nose) / (1.0 - o->dt_2 * *pi / p->Q_Nose);
     double dt2_s_s = o->dt_2 * (1 / *s + 1 / s_old);
+#pragma omp parallel for schedule(static)
     for (size_t c = 0; c < N; ++c)
         for (int a = 0; a < 3; ++a)
             o->dipoR[3 * c + a] = o->dipoR[3 * c + a] + dt2_s_s * o->dipoP[3 * c + a] * o->mass_dipo_inv[c];
@@ -1165,6 +1315,7 @@ void fo_nose_poincare(fo_ctx *o, fo_energy *e)
     *pi = *pi + o->dt_2 * (2 * e->dipo_kinetic / (*s * *s) - (double)o->n_freedom * o->temperature_eV)
         - o->dt_2 * delta_H;
     dt2s = o->dt_2 * *s;
+#pragma omp parallel for schedule(static)
     for (size_t i = 0; i < 3 * N; ++i) o->dipoP[i] = o->dipoP[i] - dt2s * o->dVddi[i];
     e->dipo_kinetic = fo_kinetic_energy_dipo(o);
     fo_nose_poincare_hamiltonian(o, e);
