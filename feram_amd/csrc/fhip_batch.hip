@@ -51,6 +51,7 @@ struct BClass {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     bool avg = false, hist = false;
+    bool heat = false;                  // a leapfrog member has heat-source layers
 };
 
 }  // namespace
@@ -125,6 +126,13 @@ void fill_rep(Rep<T> &r, fhip_ctx *ctx, int method)
     r.zblocks = ctx->zblocks[tix<T>()];
     r.kblocks = ctx->kblocks;
     r.ravg = ctx->avg_ravg ? 1 : 0;
+    if (method == FHIP_LF && (ctx->delta_e != 0.0 || ctx->delta_e_acou != 0.0)) {
+        r.el = ctx->el;
+        r.el_sum = ctx->el_sum;
+        r.nlayer = ctx->nlayer;
+        r.delta_e = ctx->delta_e;
+        r.delta_e_acou = ctx->delta_e_acou;
+    }
 }
 
 template <typename T>
@@ -238,6 +246,10 @@ int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int ne
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const unsigned n = (unsigned)k.members.size();
+    if (k.heat) {                       // leapfrog.F:38-83, before the drift
+        kb_layer_ke<T><<<dim3(k.Lz, 1, n), NT, 0, s>>>(r);
+        kb_layer_scale<T><<<dim3((k.Lx * k.Ly + NT - 1) / NT, 2, n), NT, 0, s>>>(r);
+    }
     // film/epit members: source rows first, then the mirrored rows from them
     // (bulk members do everything in the first launch and skip the second)
     RC(l_xfwd<T>(b, k, s, k.film ? XP_SOURCE : XP_ALL));
@@ -259,6 +271,7 @@ int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int ne
         if (k.hist) kb_avg_cells<T, 1><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
         else kb_avg_cells<T, 0><<<dim3(k.kblocks, 1, n), NT, 0, s>>>(r);
         kb_avg_fold<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
+        if (k.heat) kb_layer_acc<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bfail(b, FHIP_ERR_HIP, std::string("launch finish: ") + hipGetErrorString(e));
@@ -331,6 +344,7 @@ std::vector<int> graph_key(const fhip_batch *b)
     for (const auto &k : b->classes) {
         key.push_back(k.avg ? 1 : 0);
         key.push_back(k.hist ? 1 : 0);
+        key.push_back(k.heat ? 1 : 0);
         key.push_back(k.xrows);
         key.push_back(k.ytile);
     }
@@ -453,12 +467,11 @@ int tune_class(fhip_batch *b, BClass &k)
 int refresh(fhip_batch *b)
 {
     for (auto &k : b->classes) {
-        bool avg = false, hist = false;
+        bool avg = false, hist = false, heat = false;
         for (size_t i = 0; i < k.members.size(); ++i) {
             fhip_ctx *c = k.members[i];
             if (c->avg_auto && !c->davg) return bfail(b, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
-            if (c->delta_e != 0.0 || c->delta_e_acou != 0.0)
-                return bfail(b, FHIP_ERR_UNSUPPORTED, "heat-source layers are stepped by the per-context calls");
+            if (b->method == FHIP_LF && (c->delta_e != 0.0 || c->delta_e_acou != 0.0)) heat = true;
             if (i == 0) {
                 avg = c->avg_auto;
                 hist = c->avg_auto && c->avg_hist;
@@ -469,6 +482,7 @@ int refresh(fhip_batch *b)
         }
         k.avg = avg;
         k.hist = hist;
+        k.heat = heat;
         RC(k.prec == FHIP_FP32 ? upload_reps<float>(b, k) : upload_reps<double>(b, k));
     }
     return FHIP_OK;
@@ -527,9 +541,7 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
         else if (c->p.method != b->method)
             return bfail(b, FHIP_ERR_UNSUPPORTED, "every replica of a batch uses the same method");
         if (!batch_size_ok(c->Lx) || !batch_size_ok(c->Ly) || !batch_size_ok(c->Lz))
-            return bfail(b, FHIP_ERR_UNSUPPORTED,
-                         "batched supercell edges are 4 6 8 12 16 20 24 32 40 48 60 64 (larger ones fill the device "
-                         "alone)");
+            return bfail(b, FHIP_ERR_UNSUPPORTED, "supercell edge not supported");
         if (c->c.zsp)
             return bfail(b, FHIP_ERR_UNSUPPORTED, "species with different Z* are stepped by the per-context calls");
         b->ctxs.push_back(c);

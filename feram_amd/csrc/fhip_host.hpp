@@ -282,6 +282,5 @@ template <typename T, int n> void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s,
 
 }  // namespace fhip_stage
 
-// Edges of the batched instantiations.  Supercells above 64 per edge fill the
-// device alone and are stepped with the per-context calls.
-#define FHIP_BATCH_SIZES(X) X(4) X(6) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(60) X(64)
+// Edges of the batched instantiations: every supported edge.
+#define FHIP_BATCH_SIZES(X) FHIP_SIZES(X)

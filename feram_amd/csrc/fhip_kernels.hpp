@@ -908,7 +908,7 @@ template <> struct kspace_type<float> { using type = float; };
 // and the side buffer of the kx = 0 pencils, two NF-field values per point,
 // that the host sizes as dynamic LDS.
 constexpr int ZTILE_MAX = 64;
-constexpr int ZSTRAIN_FB = 2;   // strain fields transformed together (DD and acoustic groups: 3)
+constexpr int ZSTRAIN_FB = 3;   // strain fields transformed together (DD and acoustic groups: 3)
 template <typename T, int LZ>
 struct ZTile {
     static constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value < ZTILE_MAX
@@ -2153,10 +2153,10 @@ static __global__ void k_reset_scale(Scalars* sc)
 // layer Lz/4 scaled up and of layer 3Lz/4 down so that their energies change
 // by +-Lx Ly delta_e (and delta_e_acou for the acoustic momenta).
 // ---------------------------------------------------------------------------
-static __global__ __launch_bounds__(NT) void k_layer_ke(Consts c, Fields f, const int* __restrict__ nlayer, double* el)
+__device__ __forceinline__ void layer_ke_body(const Consts& c, const Fields& f, const int* __restrict__ nlayer,
+                                              double* el, int iz)
 {
     __shared__ double red[NT];
-    const int iz = blockIdx.x;
     const int LxLy = c.Lx * c.Ly;
     double ke = 0.0, kea = 0.0;
     for (int i = threadIdx.x; i < LxLy; i += NT) {
@@ -2179,15 +2179,20 @@ static __global__ __launch_bounds__(NT) void k_layer_ke(Consts c, Fields f, cons
     }
 }
 
-static __global__ __launch_bounds__(NT) void k_layer_scale(Consts c, Fields f, const double* __restrict__ el,
-                                                    double delta_e, double delta_e_acou)
+static __global__ __launch_bounds__(NT) void k_layer_ke(Consts c, Fields f, const int* __restrict__ nlayer, double* el)
+{
+    layer_ke_body(c, f, nlayer, el, blockIdx.x);
+}
+
+__device__ __forceinline__ void layer_scale_body(const Consts& c, const Fields& f, const double* __restrict__ el,
+                                                 double delta_e, double delta_e_acou, int bx, int by)
 {
     const int LxLy = c.Lx * c.Ly;
-    const int i = blockIdx.x * NT + threadIdx.x;
+    const int i = bx * NT + threadIdx.x;
     if (i >= LxLy) return;
     const int q1 = c.Lz / 4, q3 = q1 * 3;
-    const int iz = blockIdx.y ? q3 : q1;
-    const double sgn = blockIdx.y ? -1.0 : 1.0;
+    const int iz = by ? q3 : q1;
+    const double sgn = by ? -1.0 : 1.0;
     const size_t cc = (size_t)iz * LxLy + i;
     if (delta_e != 0.0) {                                                            // :75-78
         const double e = el[iz];
@@ -2203,10 +2208,20 @@ static __global__ __launch_bounds__(NT) void k_layer_scale(Consts c, Fields f, c
     }
 }
 
+static __global__ __launch_bounds__(NT) void k_layer_scale(Consts c, Fields f, const double* __restrict__ el,
+                                                    double delta_e, double delta_e_acou)
+{
+    layer_scale_body(c, f, el, delta_e, delta_e_acou, blockIdx.x, blockIdx.y);
+}
+
 // Sums of the layer energies over the averaging steps (average_module.F:82-85).
-static __global__ void k_layer_acc(const double* __restrict__ el, double* sum, int n)
+__device__ __forceinline__ void layer_acc_body(const double* __restrict__ el, double* sum, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) sum[i] = sum[i] + el[i];
+}
+static __global__ void k_layer_acc(const double* __restrict__ el, double* sum, int n)
+{
+    layer_acc_body(el, sum, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -2547,6 +2562,11 @@ struct Rep {
     unsigned long long* dist1;
     int* dist3;
     int maxblocks, xblocks, zblocks, kblocks, ravg;
+    // heat-source layers of a leapfrog replica (delta_e = delta_e_acou = 0: none)
+    double* el;
+    double* el_sum;
+    const int* nlayer;
+    double delta_e, delta_e_acou;
 };
 
 template <typename T, int LX, int RB>
@@ -2659,6 +2679,33 @@ __global__ __launch_bounds__(NT) void kb_avg_fold(const Rep<T>* __restrict__ rep
 {
     const Rep<T>& r = reps[blockIdx.z];
     avg_fold_body(r.sc, r.acc, r.partials, r.maxblocks, r.kblocks, 0);
+}
+
+// Heat-source layers of the leapfrog replicas that have them (grid x: the
+// largest Lz of the class for the layer energies, the x tiles of a layer for
+// the rescale; other replicas return at once).
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_layer_ke(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    if ((r.delta_e == 0.0 && r.delta_e_acou == 0.0) || (int)blockIdx.x >= r.c.Lz) return;
+    layer_ke_body(r.c, r.f, r.nlayer, r.el, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_layer_scale(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    if (r.delta_e == 0.0 && r.delta_e_acou == 0.0) return;
+    layer_scale_body(r.c, r.f, r.el, r.delta_e, r.delta_e_acou, blockIdx.x, blockIdx.y);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void kb_layer_acc(const Rep<T>* __restrict__ reps)
+{
+    const Rep<T>& r = reps[blockIdx.z];
+    if (r.delta_e == 0.0) return;                   // as the per-context averages
+    layer_acc_body(r.el, r.el_sum, 2 * r.c.Lz);
 }
 
 }  // namespace fhip

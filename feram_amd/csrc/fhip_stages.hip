@@ -14,7 +14,7 @@ using namespace fhip_host;
 #error "compile with -DFHIP_STAGE_GROUP=<group>"
 #endif
 
-// groups of edges (the batched ones, up to 64, also get the batch launchers)
+// groups of edges (every edge also gets the batch launchers)
 #if FHIP_STAGE_GROUP == 0
 #define FHIP_GROUP_SIZES(X) X(4) X(6)
 #define FHIP_GROUP_BATCH(X) X(4) X(6)
@@ -35,18 +35,25 @@ using namespace fhip_host;
 #define FHIP_GROUP_BATCH(X) X(60) X(64)
 #elif FHIP_STAGE_GROUP == 6
 #define FHIP_GROUP_SIZES(X) X(80) X(81)
+#define FHIP_GROUP_BATCH(X) X(80) X(81)
 #elif FHIP_STAGE_GROUP == 7
 #define FHIP_GROUP_SIZES(X) X(90) X(96)
+#define FHIP_GROUP_BATCH(X) X(90) X(96)
 #elif FHIP_STAGE_GROUP == 8
 #define FHIP_GROUP_SIZES(X) X(100)
+#define FHIP_GROUP_BATCH(X) X(100)
 #elif FHIP_STAGE_GROUP == 9
 #define FHIP_GROUP_SIZES(X) X(128)
+#define FHIP_GROUP_BATCH(X) X(128)
 #elif FHIP_STAGE_GROUP == 10
 #define FHIP_GROUP_SIZES(X) X(192)
+#define FHIP_GROUP_BATCH(X) X(192)
 #elif FHIP_STAGE_GROUP == 11
 #define FHIP_GROUP_SIZES(X) X(243)
+#define FHIP_GROUP_BATCH(X) X(243)
 #elif FHIP_STAGE_GROUP == 12
 #define FHIP_GROUP_SIZES(X) X(256)
+#define FHIP_GROUP_BATCH(X) X(256)
 #else
 #error "unknown FHIP_STAGE_GROUP"
 #endif

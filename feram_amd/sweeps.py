@@ -138,8 +138,8 @@ def hysteresis_loop(systems, n_thermalize: int, n_average: int, n_hl_freq: int, 
     rows = [[] for _ in systems]
     n_end = n_thermalize + n_average
     outputs = [i for i in range(n_thermalize, n_end + 1, n_hl_freq) if i >= i_start]
-    # batches take edges up to 64; larger supercells step through their own contexts
-    with (_Each(systems) if any(max(g.p.Lx, g.p.Ly, g.p.Lz) > 64 for g in systems) else Batch(systems)) as bt:
+    # one supercell steps through its own context, several as one replica batch
+    with (_Each(systems) if len(systems) == 1 else Batch(systems)) as bt:
         i = i_start
         for out_step in outputs:
             bt.step(out_step - i + 1)

@@ -28,6 +28,8 @@ SWEEP = [  # (edges, precision, kelvin, E_z in V/A)
     ((8, 8, 8), 1, 250.0, 0.0),
     ((16, 8, 12), 0, 200.0, 0.0),
     ((20, 40, 60), 0, 350.0, 0.0005),
+    ((8, 8, 128), 0, 300.0, 0.0005),
+    ((96, 8, 8), 1, 320.0, 0.0),
 ]
 
 
@@ -72,18 +74,18 @@ def compare(single, batched, energies_s, energies_b):
 
 def test_np_batch_matches_individual_steps():
     from feram_amd.system import Batch
-    print("[gpu] batch md: 6 replicas, 5 size classes, 7 steps", flush=True)
+    print("[gpu] batch md: 8 replicas, 7 size classes, 22 steps", flush=True)
     single, batched = pair("md")
     es = []
     for s in single:
         s.md_prologue()
     for b in batched:
         b.md_prologue()
-    nsteps = 7
+    nsteps = 22      # 20 middle steps: the 16- and 4-step graphs
     for s in single:
         es.append(s.nose_poincare(nsteps))
     with Batch(batched) as bt:
-        assert bt.classes == 5
+        assert bt.classes == 7
         eb = bt.step(nsteps)
     compare(single, batched, es, eb)
     # a second call continues from the state the first left
@@ -157,9 +159,6 @@ def test_batch_rejects_what_it_cannot_run():
         Batch([a, b])
     with pytest.raises(FhipError, match="twice"):
         Batch([a, a])
-    big = make((8, 8, 96), 0, 300.0, 0.0, "md", 3)
-    with pytest.raises(FhipError, match="64"):
-        Batch([a, big])
     c = make((8, 8, 8), 0, 300.0, 0.0, "md", 4)
     a.avg_reset()
     a.avg_auto(True)

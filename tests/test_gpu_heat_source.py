@@ -89,3 +89,54 @@ def test_leapfrog_heat_source_fp32(oracle_mod):
         a, b = g.get(name), o.field(name)
         assert np.max(np.abs(a - b)) <= 2e-5 * np.max(np.abs(b)), name
     g.close()
+
+
+def test_heat_source_in_a_batch_matches_individual_steps():
+    """Leapfrog replicas with and without heat-source layers (one with acoustic
+    MD and delta_e_acou, one with delta_e only, one without) stepped as one
+    batch, against twin contexts stepped individually: fields, layer energies
+    and their averaged sums after 12 steps (16-step graphs are not reached; 4
+    and 1 are) and after a second call of 20 steps (16 + 1 + 1 + 1)."""
+    from feram_amd.system import Batch, System
+    print("[gpu] heat-source layers in a batch", flush=True)
+    cases = [((8, 6, 16), True, 0.002, 0.001), ((8, 6, 16), False, 0.003, 0.0), ((12, 12, 12), False, 0.0, 0.0),
+             ((8, 6, 16), False, 0.001, 0.0)]
+
+    def make(i, edges, acoustic, de, dea):
+        p = batio3_zvr(edges, method="lf", kelvin=300.0, acoustic_mass_amu=46.44 if acoustic else None)
+        g = System(p)
+        rng = np.random.default_rng(40 + i)
+        shape = (p.Lz, p.Ly, p.Lx, 3)
+        g.put("dipoR", 0.06 + 0.05 * rng.standard_normal(shape))
+        g.put("dipoP", 0.1 * rng.standard_normal(shape))
+        if acoustic:
+            g.put("acouP", 0.05 * rng.standard_normal(shape))
+        if de or dea:
+            g.set_heat_source(de, dea)
+        g.mirror_force_mirror()
+        g.avg_reset()
+        g.avg_auto(True)
+        return g
+
+    single = [make(i, *c) for i, c in enumerate(cases)]
+    batched = [make(i, *c) for i, c in enumerate(cases)]
+    try:
+        for nsteps in (12, 20):
+            es = [s.leapfrog(1, nsteps) for s in single]
+            with Batch(batched) as bt:
+                eb = bt.step(nsteps)
+            for i, (s, b) in enumerate(zip(single, batched)):
+                for f in ("dipoR", "dipoP") + (("acouR", "acouP") if cases[i][1] else ()):
+                    x, y = b.get(f), s.get(f)
+                    assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(y)), (nsteps, i, f)
+                scale = sum(abs(v) for v in es[i].values())
+                for k, v in es[i].items():
+                    assert abs(eb[i][k] - v) <= 1e-12 * scale, (nsteps, i, k)
+                if cases[i][2] or cases[i][3]:
+                    for avg in (False, True):
+                        xs, ys = b.layer_energies(averaged=avg), s.layer_energies(averaged=avg)
+                        for x, y in zip(xs, ys):
+                            assert np.allclose(x, y, rtol=1e-12, atol=1e-300), (nsteps, i, avg)
+    finally:
+        for g in single + batched:
+            g.close()
