@@ -79,8 +79,8 @@ def temperature_chain(temps, seeds=(1, 2, 3, 4), base: Param | None = None, L: i
                 u[it, j] = np.array(a["u"]) / (a["n_add"] * g.avg_u_norm())
                 strain[it, j] = np.array(a["strain"]) / a["n_add"]
                 pe[it, j] = sum(a["energy"][t] for t in POTENTIAL) / (a["n_add"] * g.N)
-                if variance:
-                    var[it, j] = g.get("dipoR").reshape(-1, 3).var(axis=0)
+                if variance:                   # over the counted cells (films: the lower half)
+                    var[it, j] = g.get("dipoR")[g.counted()].var(axis=0)
             if progress:
                 progress(it, T, u[it])
     for g in systems:

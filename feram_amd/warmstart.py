@@ -9,11 +9,15 @@ plain MD produces (``equilibrium_samples``, run as one replica batch) and
 uses it to start new runs near equilibrium (``WarmStart.initialise``).  The
 MD still decides when a run is equilibrated.
 
-Inputs: temperature [K], field E_z [V/A] (along z) and hydrostatic pressure
-[GPa].  Outputs (13): the sorted magnitudes of the mean dipole <u>
-(ascending, so the largest lies along z, the field axis), the per-component
-variance of u over the cells in the same order, the homogeneous strain
-(xx yy zz yz zx xy) in that frame, and the potential energy per cell.
+Inputs: temperature [K], field E_z [V/A] (along z) and the strain control:
+hydrostatic pressure [GPa] for bulk supercells, the clamped in-plane strain
+epi_strain for epitaxial films ('epit', optimize-homo-strain.F:34-41), one
+network per geometry (``WarmStart(control=...)``).  Outputs (13): the mean
+dipole <u> in a canonical frame (bulk: magnitudes sorted ascending, so the
+largest lies along z, the field axis; films: the two in-plane components
+sorted, z kept as the film normal), the per-component variance of u over the
+cells in the same order, the homogeneous strain (xx yy zz yz zx xy) in that
+frame, and the potential energy per cell.
 
 A warm start draws the dipoles as the predicted mean plus Gaussian
 fluctuations with the spectrum of the quadratic dipole energy
@@ -35,16 +39,20 @@ NIN, NOUT = 3, 13
 # inputs are mapped to about [-1, 1] over the trained ranges
 T_MID, T_HALF = 300.0, 150.0
 E_MID, E_HALF = 5e-4, 5e-4
-P_MID, P_HALF = -5.0, 1.0
+P_MID, P_HALF = -5.0, 1.0                 # hydrostatic pressure [GPa] (bulk)
+S_MID, S_HALF = -0.01, 0.01               # epitaxial strain (films)
 
 
 class MlpError(RuntimeError):
     pass
 
 
-def normalise_inputs(T, Ez, GPa) -> np.ndarray:
-    T, Ez, GPa = np.broadcast_arrays(np.asarray(T, float), np.asarray(Ez, float), np.asarray(GPa, float))
-    return np.stack([(T - T_MID) / T_HALF, (Ez - E_MID) / E_HALF, (GPa - P_MID) / P_HALF], axis=-1)
+def normalise_inputs(T, Ez, s, control: str = "pressure") -> np.ndarray:
+    """The three network inputs; s is the pressure [GPa] (control "pressure")
+    or the epitaxial strain (control "epi_strain")."""
+    T, Ez, s = np.broadcast_arrays(np.asarray(T, float), np.asarray(Ez, float), np.asarray(s, float))
+    mid, half = (P_MID, P_HALF) if control == "pressure" else (S_MID, S_HALF)
+    return np.stack([(T - T_MID) / T_HALF, (Ez - E_MID) / E_HALF, (s - mid) / half], axis=-1)
 
 
 class Mlp:
@@ -210,12 +218,16 @@ def shaped_fluctuations(A: np.ndarray, shape, T: float, var, rng) -> np.ndarray:
 _SHEAR = {(1, 2): 3, (2, 1): 3, (2, 0): 4, (0, 2): 4, (0, 1): 5, (1, 0): 5}
 
 
-def canonical(mean_u, var_u, strain, pe):
+def canonical(mean_u, var_u, strain, pe, film: bool = False):
     """Rotate (permute and reflect the axes of) an equilibrium state so that
-    |<u>| ascends from x to z and every <u>_a >= 0; returns the 13 targets
-    (the potential energy per cell pe is frame-independent)."""
+    |<u>| ascends from x to z (films: from x to y, z stays the film normal)
+    and every <u>_a >= 0; returns the 13 targets (the potential energy per
+    cell pe is frame-independent)."""
     mean_u, var_u, strain = np.asarray(mean_u), np.asarray(var_u), np.asarray(strain)
-    perm = np.argsort(np.abs(mean_u), kind="stable")          # new axis a <- old axis perm[a]
+    if film:
+        perm = np.concatenate([np.argsort(np.abs(mean_u[:2]), kind="stable"), [2]])
+    else:
+        perm = np.argsort(np.abs(mean_u), kind="stable")      # new axis a <- old axis perm[a]
     sign = np.where(mean_u >= 0, 1.0, -1.0)
     m = np.abs(mean_u[perm])
     v = var_u[perm]
@@ -306,25 +318,79 @@ def chain_samples(temps, fields=(0.0, 2.5e-4, 5e-4, 1e-3), pressures=(-6.0, -5.0
     return np.array(X), np.array(Y)
 
 
-class WarmStart:
-    """A trained network plus the target normalisation."""
+def epit_base(L=(32, 32, 8), epi_strain: float = -0.01, GPa: float = -5.0, kelvin: float = 350.0,
+              Ez: float = 0.0) -> Param:
+    """The reference's epitaxial-film example (07example-BaTiO3-epit-32x32-L004-
+    D0-5.0GPa-0.01-350-500K/heating/heating.sh:22-61): an 'epit' film of Lz/2
+    layers (gap_id 0) on the bulk parameter set, -5 GPa, Q_Nose 0.1."""
+    from .params import batio3_zvr
+    p = batio3_zvr(L, method="md", kelvin=float(kelvin))
+    p.bulk_or_film = "epit"
+    p.gap_id = 0
+    p.epi_strain = float(epi_strain)
+    p.GPa = [float(GPa)] * 3
+    p.Q_Nose = 0.1
+    p.external_E_field = [0.0, 0.0, float(Ez)]
+    p.E_field_always_zero = float(Ez) == 0.0
+    return p
 
-    def __init__(self, seed: int = 0):
+
+# the reference's default start of its epit example (heating.sh:57-58)
+EPIT_INIT_AVG = np.array([0.0, 0.0, 0.15])
+EPIT_INIT_DEV = 0.04
+
+
+def chain_samples_film(temps, strains=(-0.015, -0.01, -0.005), fields=(0.0, 5e-4), seeds: int = 2,
+                       L=(32, 32, 8), GPa: float = -5.0, n_thermalize: int = 4000, n_average: int = 2000,
+                       precision: int = 0, progress=None):
+    """Training data of the film network: chained heating scans of epitaxial
+    films (epit_base) per (epitaxial strain, field, seed), all chains in one
+    batch, from the example's default start.  Returns (conditions [n][3] = T,
+    E_z, epi_strain; targets [n][13], film canonical frame)."""
+    from .sweeps import temperature_chain
+    bases, conds = [], []
+    for s in strains:
+        for Ez in fields:
+            for _ in range(seeds):
+                bases.append(epit_base(L, s, GPa, float(temps[0]), Ez))
+                conds.append((float(Ez), float(s)))
+    res = temperature_chain(temps, bases=bases, n_thermalize=n_thermalize, n_average=n_average,
+                            precision=precision, variance=True, progress=progress,
+                            init_avg=EPIT_INIT_AVG, init_dev=EPIT_INIT_DEV)
+    X, Y = [], []
+    for i, T in enumerate(res["temps"]):
+        for j, (Ez, s) in enumerate(conds):
+            X.append((T, Ez, s))
+            Y.append(canonical(res["u"][i, j], res["variance"][i, j], res["strain"][i, j], res["pe"][i, j],
+                               film=True))
+    return np.array(X), np.array(Y)
+
+
+class WarmStart:
+    """A trained network plus the target normalisation.  control: the third
+    input, "pressure" (bulk supercells, GPa) or "epi_strain" (epitaxial
+    films)."""
+
+    def __init__(self, seed: int = 0, control: str = "pressure"):
+        if control not in ("pressure", "epi_strain"):
+            raise ValueError("control is 'pressure' or 'epi_strain'")
+        self.control = control
         self.net = Mlp(seed)
         self.y_mean = np.zeros(NOUT)
         self.y_std = np.ones(NOUT)
         self._kernels = {}
 
     def fit(self, conditions, targets, iters: int = 3000, lr: float = 2e-3) -> np.ndarray:
-        x = normalise_inputs(*np.asarray(conditions, float).T)
+        x = normalise_inputs(*np.asarray(conditions, float).T, control=self.control)
         targets = np.asarray(targets, float)
         self.y_mean = targets.mean(axis=0)
         self.y_std = np.maximum(targets.std(axis=0), 1e-12)
         y = (targets - self.y_mean) / self.y_std
         return self.net.train(x, y, iters=iters, lr=lr)
 
-    def predict(self, T, Ez, GPa) -> np.ndarray:
-        x = normalise_inputs(T, Ez, GPa).reshape(-1, NIN)
+    def predict(self, T, Ez, s) -> np.ndarray:
+        """The 13 outputs at temperature T, field E_z and strain control s."""
+        x = normalise_inputs(T, Ez, s, control=self.control).reshape(-1, NIN)
         return self.net.forward(x) * self.y_std + self.y_mean
 
     def kernel(self, system) -> np.ndarray:
@@ -336,18 +402,21 @@ class WarmStart:
             self._kernels[key] = dipole_kernel(p, system.get_long_range())
         return self._kernels[key]
 
-    def initialise(self, system, T: float, Ez: float, GPa: float, seed: int = 0,
+    def initialise(self, system, T: float, Ez: float, s: float, seed: int = 0,
                    momenta: bool = True, shaped: bool = True, energy: bool = True) -> np.ndarray:
-        """u(R) = <u> + fluctuations in the canonical frame (largest mean along
-        z, all means >= 0): with `shaped`, Gaussian fluctuations with the
+        """u(R) = <u> + fluctuations in the canonical frame (bulk: largest mean
+        along z; films: z the film normal; all means >= 0), s the strain
+        control (pressure or epitaxial strain): with `shaped`, Gaussian fluctuations with the
         spectrum of the quadratic dipole energy (shaped_fluctuations) and the
         predicted per-component variance, else white noise sqrt(var) N(0, 1).
         With momenta, p from the Maxwell-Boltzmann distribution at T
         (p_a ~ N(0, m k T)); with `energy` as well, p is then scaled so that
         the kinetic energy per cell is the predicted equilibrium potential plus
         3/2 kT minus the potential energy of the drawn u (one force
-        evaluation), within [1/4, 7/4] of 3/2 kT.  Returns the prediction."""
-        pred = self.predict(T, Ez, GPa)[0]
+        evaluation), within [1/4, 7/4] of 3/2 kT, over the counted cells (a
+        film's mirrored half is rewritten by the mirror).  Returns the
+        prediction."""
+        pred = self.predict(T, Ez, s)[0]
         m, v = np.maximum(pred[:3], 0.0), np.maximum(pred[3:6], 0.0)
         p = system.p
         rng = np.random.default_rng(seed)
@@ -363,7 +432,8 @@ class WarmStart:
             if energy:
                 e = system.mirror_force_mirror()
                 pe0 = sum(e[t] for t in POTENTIAL) / p.N
-                ke = float(np.sum(mom ** 2 / mass)) / (2 * p.N)
+                cnt = system.counted()
+                ke = float(np.sum(mom[cnt] ** 2 / mass)) / (2 * int(cnt.sum()))
                 target = min(max(pred[12] + 1.5 * kT - pe0, 0.25 * 1.5 * kT), 1.75 * 1.5 * kT)
                 mom *= np.sqrt(target / ke)
             system.put("dipoP", mom)
@@ -406,13 +476,16 @@ def equilibration_index(series, tail: float = 0.5, window: int = 5, band: float 
 def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, record: int = 10,
                  Ez: float = 0.0, GPa: float = -5.0, precision: int = 0, seed: int = 1,
                  base: Param | None = None, blocks: int = 5, Q_Nose: float | None = None,
-                 shaped: bool = True, energy: bool = True) -> dict:
+                 shaped: bool = True, energy: bool = True, cold_avg=0.1, cold_dev=0.02) -> dict:
     """Run every temperature twice in one batch -- from the reference's default
     start (init_dipo_avg 0.1, dev 0.02, p = 0) and from the warm start -- record
     the potential energy per cell every `record` steps, and return per
     temperature the equilibration step of both runs and the sorted magnitudes
     of <u> over the second half of each run, as the mean and standard error of
-    `blocks` block averages.  The equilibration step "tau" is the MSER-5
+    `blocks` block averages.  With `base` (e.g. an epitaxial film, epit_base)
+    the runs take its geometry, and a film network gets the base's
+    epi_strain as its third input; cold_avg / cold_dev give the default start
+    (the epit example's is EPIT_INIT_AVG, EPIT_INIT_DEV).  The equilibration step "tau" is the MSER-5
     truncation point of the potential-energy series (mser_index; at most half
     the run, so a run that has not settled counts as n_steps / 2); "tau_band"
     is the last exit of the smoothed series from a 4-sigma band around its
@@ -431,9 +504,10 @@ def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, recor
                 p.Q_Nose = Q_Nose
             g = System(p, precision=precision)
             if kind == "cold":
-                g.put("dipoR", 0.1 + 0.02 * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
+                g.put("dipoR", cold_avg + cold_dev * rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)))
             else:
-                ws.initialise(g, float(T), float(Ez), float(GPa), seed=int(rng.integers(1 << 31)), shaped=shaped,
+                sc = p.epi_strain if ws.control == "epi_strain" else float(GPa)
+                ws.initialise(g, float(T), float(Ez), float(sc), seed=int(rng.integers(1 << 31)), shaped=shaped,
                               energy=energy)
             g.md_prologue()
             systems.append(g)
@@ -460,7 +534,7 @@ def heating_scan(ws: "WarmStart", temps, L: int = 16, n_steps: int = 4000, recor
                     u = np.array(a["u"])
                     dn = a["n_add"] - nadd_prev[i]
                     if dn > 0:
-                        block_u[i].append(np.sort(np.abs((u - usum_prev[i]) / (dn * g.N))))
+                        block_u[i].append(np.sort(np.abs((u - usum_prev[i]) / (dn * g.avg_u_norm()))))
                     usum_prev[i], nadd_prev[i] = u, a["n_add"]
     res = {"temps": list(map(float, temps)), "record": record, "n_steps": n_steps, "L": L,
            "cold": {"tau": [], "tau_band": [], "u": [], "u_err": []},
