@@ -36,6 +36,7 @@ bool batch_size_ok(int n)
 struct BClass {
     int prec = FHIP_FP64;
     int Lx = 0, Ly = 0, Lz = 0, acoustic = 0, nf = 9;
+    int zsp = 0;                        // species with different Z*: split dipole-dipole z group
     int xblocks = 0, xfblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
     size_t zshm_g = 0;                  // dynamic LDS of the paired z pass
     bool film = false;                  // a member mirrors its upper layers (film/epit)
@@ -108,7 +109,7 @@ void fill_rep(Rep<T> &r, fhip_ctx *ctx, int method)
     r.sp = tset<T>(ctx).sp;
     r.za = zargs<T>(ctx, !ctx->c.acoustic);
     r.xa.mode = xf_mode(method);
-    r.xa.nf = ctx->c.acoustic ? 12 : 9;
+    r.xa.nf = 9 + (ctx->c.acoustic ? 3 : 0) + (ctx->c.zsp ? 3 : 0);
     r.ia.mode = xi_mode(method);
     r.ia.materialize = 0;
     r.ia.partials = ctx->partials;
@@ -157,13 +158,13 @@ void set_plan(BClass &k, int xrows, int ytile)
 }
 
 // Plans found by timing, shared by every batch of the process: (Lx, Ly, Lz,
-// precision, acoustic, replicas) -> (xrows, ytile).
+// precision, acoustic, species Z*, replicas) -> (xrows, ytile).
 std::mutex g_plan_mu;
-std::map<std::array<int, 6>, std::pair<int, int>> g_plans;
+std::map<std::array<int, 7>, std::pair<int, int>> g_plans;
 
-std::array<int, 6> plan_key(const BClass &k)
+std::array<int, 7> plan_key(const BClass &k)
 {
-    return {k.Lx, k.Ly, k.Lz, k.prec, k.acoustic, (int)k.members.size()};
+    return {k.Lx, k.Ly, k.Lz, k.prec, k.acoustic, k.zsp, (int)k.members.size()};
 }
 
 // ---- per-stage launches of one class -----------------------------------------
@@ -198,26 +199,26 @@ int l_ypass(fhip_batch *b, BClass &k, hipStream_t s, int sign)
     return bfail(b, FHIP_ERR_UNSUPPORTED, "Ly not batched");
 }
 
-template <typename T, int G2>
+template <typename T, int G2, int DDG>
 const void *zpair_fn(int Lz)
 {
     switch (Lz) {
 #define FHIP_CASE(n) \
-    case n: return fhip_stage::b_zpair_fn<T, n, G2>();
+    case n: return fhip_stage::b_zpair_fn<T, n, G2, DDG>();
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
     return nullptr;
 }
 
-template <typename T, int G2>
+template <typename T, int G2, int DDG>
 int l_zpair(fhip_batch *b, BClass &k, hipStream_t s, int want_acouK)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
     const dim3 g(k.zblocks, 1, (unsigned)k.members.size());
     switch (k.Lz) {
 #define FHIP_CASE(n) \
-    case n: fhip_stage::b_zpair<T, n, G2>(r, g, s, k.zshm_g, want_acouK); BLAUNCH("zpair");
+    case n: fhip_stage::b_zpair<T, n, G2, DDG>(r, g, s, k.zshm_g, want_acouK); BLAUNCH("zpair");
         FHIP_BATCH_SIZES(FHIP_CASE)
 #undef FHIP_CASE
     }
@@ -256,8 +257,13 @@ int class_step(fhip_batch *b, BClass &k, hipStream_t s, bool materialize, int ne
     if (k.film) RC(l_xfwd<T>(b, k, s, XP_MIRROR));
     RC(l_ypass<T>(b, k, s, -1));
     const int want = (materialize && !k.acoustic) ? 1 : 0;
-    if (k.acoustic) RC((l_zpair<T, ZG_ACOU>(b, k, s, want)));
-    else RC((l_zpair<T, ZG_STRAIN>(b, k, s, want)));
+    if (k.zsp) {
+        if (k.acoustic) RC((l_zpair<T, ZG_ACOU, ZG_DDZ>(b, k, s, want)));
+        else RC((l_zpair<T, ZG_STRAIN, ZG_DDZ>(b, k, s, want)));
+    } else {
+        if (k.acoustic) RC((l_zpair<T, ZG_ACOU, ZG_DD>(b, k, s, want)));
+        else RC((l_zpair<T, ZG_STRAIN, ZG_DD>(b, k, s, want)));
+    }
     kb_homo<T><<<dim3(1, 1, n), NT, 0, s>>>(r);
     {
         hipError_t e = hipGetLastError();
@@ -498,7 +504,9 @@ int prepare_lds(fhip_batch *b, BClass &k)
     if (k.zshm_g > 96 * 1024)
         return bfail(b, FHIP_ERR_UNSUPPORTED, "z pass: too many kx = 0 pencils per tile (Lx too small for Lz)");
     fhip_ctx *ctx = nullptr;
-    int rc = ensure_lds(ctx, k.acoustic ? zpair_fn<T, ZG_ACOU>(k.Lz) : zpair_fn<T, ZG_STRAIN>(k.Lz), k.zshm_g);
+    const void *fn = k.zsp ? (k.acoustic ? zpair_fn<T, ZG_ACOU, ZG_DDZ>(k.Lz) : zpair_fn<T, ZG_STRAIN, ZG_DDZ>(k.Lz))
+                           : (k.acoustic ? zpair_fn<T, ZG_ACOU, ZG_DD>(k.Lz) : zpair_fn<T, ZG_STRAIN, ZG_DD>(k.Lz));
+    int rc = ensure_lds(ctx, fn, k.zshm_g);
     if (rc != FHIP_OK) return bfail(b, rc, "z pass: LDS attribute");
     return FHIP_OK;
 }
@@ -542,13 +550,11 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
             return bfail(b, FHIP_ERR_UNSUPPORTED, "every replica of a batch uses the same method");
         if (!batch_size_ok(c->Lx) || !batch_size_ok(c->Ly) || !batch_size_ok(c->Lz))
             return bfail(b, FHIP_ERR_UNSUPPORTED, "supercell edge not supported");
-        if (c->c.zsp)
-            return bfail(b, FHIP_ERR_UNSUPPORTED, "species with different Z* are stepped by the per-context calls");
         b->ctxs.push_back(c);
         BClass *k = nullptr;
         for (auto &q : b->classes)
             if (q.Lx == c->Lx && q.Ly == c->Ly && q.Lz == c->Lz && q.prec == c->precision &&
-                q.acoustic == c->c.acoustic)
+                q.acoustic == c->c.acoustic && q.zsp == c->c.zsp)
                 k = &q;
         if (!k) {
             b->classes.emplace_back();
@@ -558,7 +564,8 @@ int fhip_batch_create(fhip_ctx *const *ctxs, int n, fhip_batch **out)
             k->Ly = c->Ly;
             k->Lz = c->Lz;
             k->acoustic = c->c.acoustic;
-            k->nf = c->c.acoustic ? 12 : 9;
+            k->zsp = c->c.zsp;
+            k->nf = 9 + (c->c.acoustic ? 3 : 0) + (c->c.zsp ? 3 : 0);
             const int t = c->precision == FHIP_FP32 ? 1 : 0;
             k->xblocks = c->xblocks;
             k->xfblocks = c->xfblocks;

@@ -276,8 +276,8 @@ template <typename T, int n, int G2, int DDG> int zpair(fhip_ctx *ctx, const ZAr
 // replica batches: one launch over every replica of a size class
 template <typename T, int n> void b_xfwd(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int phase, int mode);
 template <typename T, int n> void b_ypass(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int sign);
-template <typename T, int n, int G2> void b_zpair(const Rep<T> *r, dim3 g, hipStream_t s, size_t shm, int want_acouK);
-template <typename T, int n, int G2> const void *b_zpair_fn();
+template <typename T, int n, int G2, int DDG> void b_zpair(const Rep<T> *r, dim3 g, hipStream_t s, size_t shm, int want_acouK);
+template <typename T, int n, int G2, int DDG> const void *b_zpair_fn();
 template <typename T, int n> void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize);
 
 }  // namespace fhip_stage

@@ -2597,14 +2597,14 @@ void kb_zpass(const Rep<T>* __restrict__ reps, int want_acouK)
     zpass_body<T, LZ, GROUP>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x, lds);
 }
 
-template <typename T, int LZ, int G2>
+template <typename T, int LZ, int G2, int DDG = ZG_DD>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(G2 == ZG_ACOU ? 1 : 2)))
 void kb_zpair(const Rep<T>* __restrict__ reps, int want_acouK)
 {
     const Rep<T>& r = reps[blockIdx.z];
     ZArgs<T> a = r.za;
     if (!want_acouK) a.acouK = nullptr;
-    zpair_body<T, LZ, G2>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x);
+    zpair_body<T, LZ, G2, DDG>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x);
 }
 
 template <typename T, int LX, int NF>

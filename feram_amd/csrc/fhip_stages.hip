@@ -153,22 +153,25 @@ void b_ypass(const Rep<T> *r, dim3 g, hipStream_t s, bool half, int sign)
     }
 }
 
-template <typename T, int n, int G2>
+template <typename T, int n, int G2, int DDG>
 void b_zpair(const Rep<T> *r, dim3 g, hipStream_t s, size_t shm, int want_acouK)
 {
-    kb_zpair<T, n, G2><<<g, NT, shm, s>>>(r, want_acouK);
+    kb_zpair<T, n, G2, DDG><<<g, NT, shm, s>>>(r, want_acouK);
 }
 
-template <typename T, int n, int G2>
+template <typename T, int n, int G2, int DDG>
 const void *b_zpair_fn()
 {
-    return (const void *)kb_zpair<T, n, G2>;
+    return (const void *)kb_zpair<T, n, G2, DDG>;
 }
 
 template <typename T, int n>
 void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize)
 {
-    if (nf == 12) kb_xinv<T, n, 12><<<g, NT, 0, s>>>(r, materialize);
+    // 9 forward fields, 3 more with acoustic MD and 3 more for species of
+    // different Z* (Z*(R) u)
+    if (nf == 15) kb_xinv<T, n, 15><<<g, NT, 0, s>>>(r, materialize);
+    else if (nf == 12) kb_xinv<T, n, 12><<<g, NT, 0, s>>>(r, materialize);
     else kb_xinv<T, n, 9><<<g, NT, 0, s>>>(r, materialize);
 }
 
@@ -185,10 +188,14 @@ void b_xinv(const Rep<T> *r, dim3 g, hipStream_t s, int nf, int materialize)
 #define FHIP_INST_B(T, n)                                                                   \
     template void b_xfwd<T, n>(const Rep<T> *, dim3, hipStream_t, bool, int, int);          \
     template void b_ypass<T, n>(const Rep<T> *, dim3, hipStream_t, bool, int);              \
-    template void b_zpair<T, n, ZG_STRAIN>(const Rep<T> *, dim3, hipStream_t, size_t, int); \
-    template void b_zpair<T, n, ZG_ACOU>(const Rep<T> *, dim3, hipStream_t, size_t, int);   \
-    template const void *b_zpair_fn<T, n, ZG_STRAIN>();                                     \
-    template const void *b_zpair_fn<T, n, ZG_ACOU>();                                       \
+    template void b_zpair<T, n, ZG_STRAIN, ZG_DD>(const Rep<T> *, dim3, hipStream_t, size_t, int);  \
+    template void b_zpair<T, n, ZG_ACOU, ZG_DD>(const Rep<T> *, dim3, hipStream_t, size_t, int);    \
+    template void b_zpair<T, n, ZG_STRAIN, ZG_DDZ>(const Rep<T> *, dim3, hipStream_t, size_t, int); \
+    template void b_zpair<T, n, ZG_ACOU, ZG_DDZ>(const Rep<T> *, dim3, hipStream_t, size_t, int);   \
+    template const void *b_zpair_fn<T, n, ZG_STRAIN, ZG_DD>();                                      \
+    template const void *b_zpair_fn<T, n, ZG_ACOU, ZG_DD>();                                        \
+    template const void *b_zpair_fn<T, n, ZG_STRAIN, ZG_DDZ>();                                     \
+    template const void *b_zpair_fn<T, n, ZG_ACOU, ZG_DDZ>();                                       \
     template void b_xinv<T, n>(const Rep<T> *, dim3, hipStream_t, int, int);
 
 #define FHIP_INST(n)         \

@@ -189,3 +189,53 @@ def test_autotuned_plan_agrees_with_untuned():
         print(f"  class {p['L']}: x rows {p['x_rows']} (us {p['x_us'][0]:.1f}/{p['x_us'][1]:.1f}), "
               f"y lines {p['y_lines']} (us {p['y_us'][0]:.1f}/{p['y_us'][1]:.1f})", flush=True)
     compare(a, b, ea, eb)
+
+
+@pytest.mark.parametrize("method,acoustic", [("md", False), ("lf", True)])
+def test_species_zstar_replicas_in_a_batch(method, acoustic):
+    """Replicas whose relaxor species carry different Z* (their own size class:
+    Z*(R) u transformed separately, the split dipole-dipole z group) next to
+    replicas of the same edges with one Z*, in one batch, against twin
+    contexts stepped individually (9-step calls: 4 + 4 middle steps)."""
+    from feram_amd.system import Batch, System
+    print(f"[gpu] batch with species Z*: {method} acoustic={acoustic}", flush=True)
+    specs = [((12, 8, 16), [9.956, 7.5, 12.25], 300.0), ((12, 8, 16), [9.956, 7.5, 12.25], 350.0),
+             ((12, 8, 16), [9.956, 9.956, 9.956], 300.0), ((8, 8, 8), [9.956, 8.0, 11.0], 250.0)]
+
+    def make_z(i, edges, zstar, kelvin):
+        p = batio3_zvr(edges, method=method, kelvin=kelvin, acoustic_mass_amu=46.44 if acoustic else None)
+        p.Z_star = list(zstar)
+        p.mass_dipo = [p.mass_dipo[0], 1.3 * p.mass_dipo[0], 0.7 * p.mass_dipo[0]]
+        p.external_E_field = [0.0008, -0.0005, 0.0015]
+        p.E_field_always_zero = False
+        p.Q_Nose = 0.1 * p.N / 16 ** 3
+        g = System(p)
+        rng = np.random.default_rng(70 + i)
+        shape = (p.Lz, p.Ly, p.Lx)
+        g.set_cells(relaxor=rng.integers(1, 4, size=shape).astype(np.int32))
+        g.put("dipoR", 0.05 + 0.04 * rng.standard_normal(shape + (3,)))
+        g.put("dipoP", 0.02 * rng.standard_normal(shape + (3,)))
+        if acoustic:
+            g.put("acouR", 0.02 * rng.standard_normal(shape + (3,)))
+        if method == "md":
+            g.md_prologue()
+        else:
+            g.mirror_force_mirror()
+        return g
+
+    single = [make_z(i, *sp) for i, sp in enumerate(specs)]
+    batched = [make_z(i, *sp) for i, sp in enumerate(specs)]
+    try:
+        es = [s.step(9) for s in single]
+        with Batch(batched) as bt:
+            assert bt.classes == 3
+            eb = bt.step(9)
+        for i, (s, b) in enumerate(zip(single, batched)):
+            for f in ("dipoR", "dipoP", "dVddi") + (("acouR", "acouP") if acoustic else ()):
+                assert rel(b.get(f), s.get(f)) <= TOL64, (i, f)
+            scale = sum(abs(v) for v in es[i].values())
+            for k, v in es[i].items():
+                assert abs(eb[i][k] - v) <= TOL64 * scale, (i, k)
+    finally:
+        for g in single + batched:
+            g.close()
