@@ -635,17 +635,57 @@ enum { EV_N = 0, EV_C = 1, EV_RP = 2, EV_RQ = 3 };
 // energies are added with weight wm times that of a point off the kx = 0
 // and kx = Lx/2 planes, and w(k) goes to acouK when it is materialised
 // (with -conj w at kkp, the implied partner, when kkp >= 0).
+// Table values the k-space stage reads at one point: the per-axis cos/sin
+// and the L(k) entries of the dipole-dipole groups, the k vector of the
+// strain and acoustic groups.  A z pass that prefetches the next group's
+// tile loads them for all its points before that prefetch is issued, since
+// vector-memory loads complete in issue order and a table load issued after
+// the prefetch would wait for the whole prefetch.
+template <typename KT>
+struct KTab {
+    KT cx, cy, cz, sx, sy, sz;
+    KT L[6];
+    KT k[3];
+    cpx<KT> mk;                   // modulation spectrum at the point (0 without modulation)
+};
+
+// MOD: 0 no modulation spectrum, 1 one is applied, -1 c.modulation decides.
+template <typename KT, int GROUP, typename T, int MOD = -1>
+__device__ __forceinline__ KTab<KT> ktab_load(const Consts& c, const ZArgs<T>& a, int kx, int ky, int kz, int kk)
+{
+    KTab<KT> tb;
+    if constexpr (GROUP == ZG_DD || GROUP == ZG_DDZ) {
+        const cpx<double> ex = a.csx[kx], ey = a.csy[ky], ez = a.csz[kz];
+        tb.cx = KT(ex.x); tb.cy = KT(ey.x); tb.cz = KT(ez.x);
+        tb.sx = KT(ex.y); tb.sy = KT(ey.y); tb.sz = KT(ez.y);
+        const T* lt = a.Ltab + kk * 6;
+        // one Z*: the table is applied as Z*^2 L(k)
+        const KT Z2 = KT(c.Z2);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) tb.L[m] = GROUP == ZG_DD ? KT(once_load(lt + m)) * Z2 : KT(once_load(lt + m));
+    } else {
+        tb.k[0] = KT(a.kvx[kx]);
+        tb.k[1] = KT(a.kvy[ky]);
+        tb.k[2] = KT(a.kvz[kz]);
+        tb.mk = cpx<KT>{KT(0), KT(0)};
+        if (MOD == 1 || (MOD < 0 && c.modulation)) {
+            const cpx<double> m0 = a.modk[kk];
+            tb.mk = cpx<KT>{KT(m0.x), KT(m0.y)};
+        }
+    }
+    return tb;
+}
+
 template <typename KT, int GROUP, typename T, int NF>
 __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalars* sc, int kx, int ky, int kz,
-                                      int kk, int kkp, const cpx<KT> (&X)[NF], cpx<KT> (&O)[NF], double wm,
-                                      double& acc0, double& acc1, double& acc2)
+                                      int kk, int kkp, const KTab<KT>& tb, const cpx<KT> (&X)[NF], cpx<KT> (&O)[NF],
+                                      double wm, double& acc0, double& acc1, double& acc2)
 {
     auto put = [&](int q, cpx<KT> o) { O[q] = o; };
     if constexpr (GROUP == ZG_DD) {
         // S(k): Fourier transform of the short-range couplings
         // (dipole-dipole-short-range.F:35-83) in closed form.
-        const cpx<double> ex = a.csx[kx], ey = a.csy[ky], ez = a.csz[kz];
-        const KT cx = KT(ex.x), cy = KT(ey.x), cz = KT(ez.x), sx = KT(ex.y), sy = KT(ey.y), szz = KT(ez.y);
+        const KT cx = tb.cx, cy = tb.cy, cz = tb.cz, sx = tb.sx, sy = tb.sy, szz = tb.sz;
         const KT j1 = KT(c.j[0]), j2 = KT(c.j[1]), j3 = KT(c.j[2]), j4 = KT(c.j[3]), j5 = KT(c.j[4]),
                  j6 = KT(c.j[5]), j7 = KT(c.j[6]);
         const KT k2t = KT(2.0 * c.kappa2[0]);
@@ -657,11 +697,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         S[3] = -4 * j5 * sy * szz - 8 * j7 * sy * szz * cx;   // yz
         S[4] = -4 * j5 * szz * sx - 8 * j7 * szz * sx * cy;   // zx
         S[5] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz;     // xy
-        KT L[6];
-        const T* lt = a.Ltab + kk * 6;
-        const KT Z2 = KT(c.Z2);
-#pragma unroll
-        for (int m = 0; m < 6; ++m) L[m] = KT(once_load(lt + m)) * Z2;
+        const KT (&L)[6] = tb.L;
         const cpx<KT> ux = X[0], uy = X[1], uz = X[2];
         // F = (S + Z^2 L) u ; energies via Parseval
         cpx<KT> fs[3], fl[3];
@@ -683,8 +719,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
     } else if constexpr (GROUP == ZG_DDZ) {
         // species with different Z*: S(k) u^ and L(k) (Z* u)^ separately
         // (dVddi-dipole-dipole.F:20-30, 40-50), E_sr and E_lr by Parseval
-        const cpx<double> ex = a.csx[kx], ey = a.csy[ky], ez = a.csz[kz];
-        const KT cx = KT(ex.x), cy = KT(ey.x), cz = KT(ez.x), sx = KT(ex.y), sy = KT(ey.y), szz = KT(ez.y);
+        const KT cx = tb.cx, cy = tb.cy, cz = tb.cz, sx = tb.sx, sy = tb.sy, szz = tb.sz;
         const KT j1 = KT(c.j[0]), j2 = KT(c.j[1]), j3 = KT(c.j[2]), j4 = KT(c.j[3]), j5 = KT(c.j[4]),
                  j6 = KT(c.j[5]), j7 = KT(c.j[6]);
         const KT k2t = KT(2.0 * c.kappa2[0]);
@@ -696,10 +731,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         S[3] = -4 * j5 * sy * szz - 8 * j7 * sy * szz * cx;
         S[4] = -4 * j5 * szz * sx - 8 * j7 * szz * sx * cy;
         S[5] = -4 * j5 * sx * sy - 8 * j7 * sx * sy * cz;
-        KT L[6];
-        const T* lt = a.Ltab + kk * 6;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) L[m] = KT(once_load(lt + m));
+        const KT (&L)[6] = tb.L;
         auto sym = [&](const KT (&M)[6], const cpx<KT>* v, cpx<KT>* o) {
             o[0] = {M[0] * v[0].x + M[5] * v[1].x + M[4] * v[2].x, M[0] * v[0].y + M[5] * v[1].y + M[4] * v[2].y};
             o[1] = {M[5] * v[0].x + M[1] * v[1].x + M[3] * v[2].x, M[5] * v[0].y + M[1] * v[1].y + M[3] * v[2].y};
@@ -719,7 +751,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         acc0 += 2.0 * wm * (double)es;
         acc1 += 2.0 * wm * (double)el;
     } else {
-        const KT k[3] = {KT(a.kvx[kx]), KT(a.kvy[ky]), KT(a.kvz[kz])};
+        const KT (&k)[3] = tb.k;
         const bool k0 = (kx == 0 && ky == 0 && kz == 0);
         if (k0) {
 #pragma unroll
@@ -727,11 +759,7 @@ __device__ __forceinline__ void keval(const Consts& c, const ZArgs<T>& a, Scalar
         }
         // coupling rhs: c_mod + C uu  (optimize-inho-strain.F:36-42, dVdac.F:46-52)
         cpx<KT> cm[3], Cu[3];
-        cpx<KT> mk = {KT(0), KT(0)};
-        if (c.modulation) {
-            const cpx<double> m0 = a.modk[kk];
-            mk = {KT(m0.x), KT(m0.y)};
-        }
+        const cpx<KT> mk = tb.mk;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const KT fq = KT(c.mod_c) * k[q];
@@ -980,9 +1008,114 @@ struct ZLds {
     int lkx[ZTile<T, LZ>::TB], lky[ZTile<T, LZ>::TB], lord[ZTile<T, LZ>::TB];
 };
 
-template <typename T, int LZ, int GROUP, typename L>
+// Spectral slot of field fi of a z group (the split dipole-dipole group reads
+// u^ from slots 0..2 and (Z* u)^ from zu_slot on).
+template <int GROUP>
+__device__ __forceinline__ int zslot(const Consts& c, int fi)
+{
+    return (GROUP == ZG_DDZ && fi >= 3) ? c.zu_slot + fi - 3 : ZGroupShape<GROUP>::F0 + fi;
+}
+
+// Prefetch of the second group's first ZPF field tiles in a paired z pass.
+// While the first group evaluates its k-space stage, the LDS pool of the
+// transforms is idle; the tiles of the next group's first field batch are
+// copied into it from HBM by LDS-DMA loads (global_load_lds_dwordx4: each
+// lane moves 16 bytes, a wave's 64 lanes fill 1 KB of LDS from a uniform
+// base), which hold no registers while in flight.  Before the first group's
+// inverse transforms take the pool back the tiles move into registers, in
+// the layout the ordinary loads give, and the next group starts from them.
+// Staging layout: field f, element (t, b) of the tile at f*EL + t*TB + b
+// (HBM order, unpadded).  The 16-byte chunks stay inside one tile row (TB a
+// multiple of CH); a chunk that would reach past the last pencil of the
+// spectrum (the last tile when Mx*Ly is not a multiple of TB) or that does
+// not start on 16 bytes is not copied, and its pencils below that end are
+// read from HBM when the tiles move into registers.
+constexpr int ZPF = 3;
+template <typename T, int LZ>
+struct ZPrefetch {
+    static constexpr int TB = ZTile<T, LZ>::TB, EL = ZTile<T, LZ>::EL, PPT = ZTile<T, LZ>::PPT;
+    static constexpr int CH = 16 / (int)sizeof(cpx<T>);     // tile elements per 16-byte chunk
+    static constexpr int NCH = EL / CH;                      // chunks per field tile
+    static constexpr bool OK = (16 % sizeof(cpx<T>)) == 0 && TB % CH == 0;
+};
+
+// Whether the chunk of pencil b (its first pencil) of tile w is copied: it
+// lies inside the spectrum, and its rows start on 16 bytes (Mx*Ly a multiple
+// of CH, which FP32 spectra with odd Mx and Ly are not).
+template <typename T, int LZ>
+__host__ __device__ __forceinline__ bool zchunk_ok(int w, int b0, int nlines)
+{
+    constexpr int TB = ZPrefetch<T, LZ>::TB, CH = ZPrefetch<T, LZ>::CH;
+    return w * TB + b0 + CH <= nlines && nlines % CH == 0;
+}
+
+template <typename T, int LZ, int GROUP>
+__device__ __forceinline__ void zprefetch_issue(const Consts& c, const Spectra<T>& sp, int w, cpx<T>* stage)
+{
+    using P = ZPrefetch<T, LZ>;
+    constexpr int TB = P::TB, EL = P::EL, CH = P::CH, NCH = P::NCH;
+    static_assert(P::OK, "prefetch needs 16-byte chunks inside a tile row");
+    const int nlines = c.Mx * c.Ly;
+    const size_t zstride = (size_t)nlines;
+    const int tid = threadIdx.x;
+    const int wbase = tid & ~63;                             // first thread of this wave
+#pragma unroll
+    for (int f = 0; f < ZPF; ++f) {
+        const cpx<T>* src = sp.s[zslot<GROUP>(c, f)];
+#pragma unroll
+        for (int it = 0; it < (NCH + NT - 1) / NT; ++it) {
+            const int j = it * NT + tid;
+            if (NCH % NT == 0 || j < NCH) {
+                const int e = j * CH, t = e / TB, b = e - t * TB;
+                // every lane issues its copy (a chunk past the end copies the
+                // field's first chunk instead, and is not used), so the number
+                // of loads in flight after the prefetch is known to the compiler
+                // and later waits need not drain it
+                const size_t in = zchunk_ok<T, LZ>(w, b, nlines) ? 1 : 0;
+                const cpx<T>* g = src + ((size_t)(w * TB + b) + (size_t)t * zstride) * in;
+                cpx<T>* l = stage + f * EL + (it * NT + wbase) * CH;   // wave-uniform LDS base
+                __builtin_amdgcn_global_load_lds((const void*)g, (__attribute__((address_space(3))) void*)l, 16, 0, 0);
+            }
+        }
+    }
+}
+
+// Wait for this wave's LDS-DMA loads (vmcnt 0; expcnt and lgkmcnt left free).
+__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <typename T, int LZ, int GROUP>
+__device__ __forceinline__ void zprefetch_collect(const Consts& c, const Spectra<T>& sp, int w, const cpx<T>* stage,
+                                                  cpx<T> (&pre)[ZPF][ZTile<T, LZ>::PPT])
+{
+    using P = ZPrefetch<T, LZ>;
+    constexpr int TB = P::TB, EL = P::EL, PPT = P::PPT, CH = P::CH;
+    using Map = ZMap<NT, TB, LZ>;
+    const int nlines = c.Mx * c.Ly;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int f = 0; f < ZPF; ++f) {
+#pragma unroll
+        for (int i = 0; i < PPT; ++i) {
+            pre[f][i] = cpx<T>{T(0), T(0)};
+            if (Map::live(tid, i)) {
+                int t, b;
+                Map::hbm(tid, i, t, b);
+                const int l = w * TB + b;
+                if (zchunk_ok<T, LZ>(w, b - b % CH, nlines)) pre[f][i] = stage[f * EL + t * TB + b];
+                else if (l < nlines) pre[f][i] = sp.s[zslot<GROUP>(c, f)][(size_t)l + (size_t)t * nlines];
+            }
+        }
+    }
+}
+
+// PIN: the first PIN fields of the group come in pre (prefetched);
+// NEXT >= 0: the group prefetches the first ZPF tiles of group NEXT into pre
+// (only without a modulation spectrum, whose loads would sit in the k-space
+// stage behind the prefetch).
+template <typename T, int LZ, int GROUP, int PIN, int NEXT, typename L>
 __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp, const ZArgs<T>& a, Scalars* sc,
-                                              const cpx<T>* twz, int bx, L& lds)
+                                              const cpx<T>* twz, int bx, L& lds,
+                                              cpx<T> (&pre)[ZPF][ZTile<T, LZ>::PPT])
 {
     constexpr int TB = ZTile<T, LZ>::TB, ST = ZTile<T, LZ>::ST, EL = ZTile<T, LZ>::EL, PPT = ZTile<T, LZ>::PPT;
     constexpr int NF = ZGroupShape<GROUP>::NF;
@@ -1005,9 +1138,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
     const int nlines = c.Mx * c.Ly;
     const size_t zstride = (size_t)nlines;
     for (int i = tid; i < LZ; i += NT) tws[i] = twz[i];
-    // spectral slot of the group's field fi (the split dipole-dipole group
-    // reads u^ from slots 0..2 and (Z* u)^ from zu_slot on)
-    auto slot_of = [&](int fi) { return (GROUP == ZG_DDZ && fi >= 3) ? c.zu_slot + fi - 3 : F0 + fi; };
+    auto slot_of = [&](int fi) { return zslot<GROUP>(c, fi); };
 
     using Map = ZMap<NT, TB, LZ>;
     // element e of the tile in HBM order: LDS slot and HBM offset
@@ -1071,7 +1202,9 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             for (int i = 0; i < PPT; ++i) {
                 const int e = tid + i * NT;
                 v[fi][i] = cpx<T>{T(0), T(0)};
-                if (EL % NT == 0 || e < EL) {
+                if (fi < PIN) {
+                    v[fi][i] = pre[fi < PIN ? fi : 0][i];
+                } else if (EL % NT == 0 || e < EL) {
                     int lds; size_t off;
                     if (hbm_i(i, lds, off)) v[fi][i] = src[off];
                 }
@@ -1093,6 +1226,27 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             if constexpr (REG) Map::hbm(tid, i, t, b);
             else Map::line(tid, i, t, b);
         };
+        // A group that prefetches the next one loads the table values of its
+        // points here, with its tile: the prefetch is issued after the
+        // forward transforms, once every ordinary load has landed, because
+        // the compiler does not order LDS-DMA loads with ordinary ones and
+        // would otherwise drain the prefetch at the first table value used.
+        // Every point gets its load (kx = 0 and empty pencils at a valid
+        // entry, unused) so that no load depends on a branch.
+        constexpr bool PRETAB = NEXT >= 0;
+        KTab<KT> tab[PRETAB ? PPT : 1];
+        if constexpr (PRETAB) {
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                int t, b;
+                pt_i(i, t, b);
+                t = min(t, LZ - 1);
+                const int l = l0 + b;
+                const int ky = l / c.Mx, kx = l - ky * c.Mx;
+                const bool ok = l < nlines;
+                tab[i] = ktab_load<KT, GROUP, T, 0>(c, a, ok ? kx : 0, ok ? ky : 0, t, ok ? l + t * nlines : 0);
+            }
+        }
         if constexpr (REG) {
 #pragma unroll
             for (int fb = 0; fb < NF; fb += FB) {
@@ -1138,6 +1292,10 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
                 *lo = split_pq(x, m, true);
                 *hi = split_pq(m, x, false);
             }
+            // the side buffer is complete before the k-space stage reads it
+            // (this barrier comes before a prefetch is issued: a barrier after
+            // it would wait for the prefetch's LDS writes)
+            __syncthreads();
         } else {
 #pragma unroll
             for (int fb = 0; fb < NF; fb += FB) {
@@ -1183,6 +1341,10 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
         }
 
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        if constexpr (PRETAB) {
+            wait_vmem_all();
+            zprefetch_issue<T, LZ, NEXT>(c, sp, w, buf);
+        }
         // points off the kx = 0 pencils: one evaluation each, in place
 #pragma unroll
         for (int i = 0; i < PPT; ++i) {
@@ -1197,7 +1359,8 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             cpx<KT> X[NF], O[NF];
 #pragma unroll
             for (int q = 0; q < NF; ++q) X[q] = {KT(v[q][i].x), KT(v[q][i].y)};
-            keval<KT, GROUP>(c, a, sc, kx, ky, t, kk, -1, X, O, 1.0, acc0, acc1, acc2);
+            const KTab<KT> tb = PRETAB ? tab[PRETAB ? i : 0] : ktab_load<KT, GROUP>(c, a, kx, ky, t, kk);
+            keval<KT, GROUP>(c, a, sc, kx, ky, t, kk, -1, tb, X, O, 1.0, acc0, acc1, acc2);
 #pragma unroll
             for (int q = 0; q < NF; ++q) v[q][i] = cpx<T>{T(O[q].x), T(O[q].y)};
         }
@@ -1205,7 +1368,6 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
         // evaluation of every point, then the second where the plan has one,
         // its first share kept in the second half of the side buffer
         cpx<T>* first = side + nsp * NF;
-        __syncthreads();
 #pragma unroll 1
         for (int j = tid; j < nsp; j += NT) {
             const int ord = j / LZ, t = j - ord * LZ;
@@ -1214,8 +1376,9 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
 #pragma unroll
             for (int q = 0; q < NF; ++q) X[q] = ev_in(cpx<KT>{KT(side[j * NF + q].x), KT(side[j * NF + q].y)}, pl.m0);
             const int kkp = pl.yp >= 0 ? kindex32(c, 0, pl.yp, pl.zp) : -1;
-            keval<KT, GROUP>(c, a, sc, pl.x0, pl.y0, pl.z0, kindex32(c, pl.x0, pl.y0, pl.z0), kkp, X, O, pl.wm,
-                             acc0, acc1, acc2);
+            const int kk0 = kindex32(c, pl.x0, pl.y0, pl.z0);
+            keval<KT, GROUP>(c, a, sc, pl.x0, pl.y0, pl.z0, kk0, kkp, ktab_load<KT, GROUP>(c, a, pl.x0, pl.y0, pl.z0, kk0),
+                             X, O, pl.wm, acc0, acc1, acc2);
             const KT half = pl.nev == 2 ? KT(0.5) : KT(1);
             cpx<T>* dst = pl.nev == 2 ? first + j * NF : side + j * NF;
 #pragma unroll
@@ -1232,8 +1395,9 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             cpx<KT> X[NF], O[NF];
 #pragma unroll
             for (int q = 0; q < NF; ++q) X[q] = ev_in(cpx<KT>{KT(side[j * NF + q].x), KT(side[j * NF + q].y)}, pl.m1);
-            keval<KT, GROUP>(c, a, sc, pl.x1, pl.y1, pl.z1, kindex32(c, pl.x1, pl.y1, pl.z1), -1, X, O, pl.wm,
-                             acc0, acc1, acc2);
+            const int kk1 = kindex32(c, pl.x1, pl.y1, pl.z1);
+            keval<KT, GROUP>(c, a, sc, pl.x1, pl.y1, pl.z1, kk1, -1, ktab_load<KT, GROUP>(c, a, pl.x1, pl.y1, pl.z1, kk1),
+                             X, O, pl.wm, acc0, acc1, acc2);
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
                 const cpx<KT> r = ev_out(cpx<KT>{KT(first[j * NF + q].x), KT(first[j * NF + q].y)}, O[q], pl.m1,
@@ -1242,6 +1406,14 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
             }
         }
         __syncthreads();
+        if constexpr (NEXT >= 0) {
+            // every wave's prefetch has landed before any wave reads it, and
+            // every read is done before the inverse transforms reuse the pool
+            wait_vmem_all();
+            __syncthreads();
+            zprefetch_collect<T, LZ, NEXT>(c, sp, w, buf, pre);
+            __syncthreads();
+        }
         // inverse transforms; kx = 0 pencils take their values from the side
         // buffer, R pencils joining P(t) (t < Lz/2) and Q(t) (held at Lz - t) back
         auto inv_in = [&](int fi, int i, int t, int b) -> cpx<T> {
@@ -1349,7 +1521,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(GROUP == ZG_
                                               const cpx<T>* __restrict__ twz)
 {
     __shared__ ZLds<T, LZ, zbuf_size<T, LZ, GROUP>()> lds;
-    zpass_body<T, LZ, GROUP>(c, sp, a, sc, twz, blockIdx.x, lds);
+    cpx<T> pre[ZPF][ZTile<T, LZ>::PPT];
+    zpass_body<T, LZ, GROUP, 0, -1>(c, sp, a, sc, twz, blockIdx.x, lds, pre);
 }
 
 // Both z groups of a step on one tile in one workgroup (the dipole-dipole and
@@ -1368,14 +1541,31 @@ __device__ __forceinline__ void zpair_body(const Consts& c, const Spectra<T>& sp
                                            const cpx<T>* twz, int bx)
 {
     __shared__ ZLds<T, LZ, zpair_buf_size<T, LZ, G2>()> lds;
-    if (bx & 1) {
-        zpass_body<T, LZ, G2>(c, sp, a, sc, twz, bx, lds);
-        __syncthreads();
-        zpass_body<T, LZ, DDG>(c, sp, a, sc, twz, bx, lds);
+    static_assert(sizeof(lds.buf) >= ZPF * ZTile<T, LZ>::EL * sizeof(cpx<T>), "prefetch staging exceeds the LDS pool");
+    // the first group prefetches the second group's first field batch
+    // (one Z*, no modulation spectrum)
+    constexpr bool PF = ZPrefetch<T, LZ>::OK && DDG == ZG_DD;
+    cpx<T> pre[ZPF][ZTile<T, LZ>::PPT];
+    if (PF && !c.modulation) {
+        if (bx & 1) {
+            zpass_body<T, LZ, G2, 0, PF ? DDG : -1>(c, sp, a, sc, twz, bx, lds, pre);
+            __syncthreads();
+            zpass_body<T, LZ, DDG, PF ? ZPF : 0, -1>(c, sp, a, sc, twz, bx, lds, pre);
+        } else {
+            zpass_body<T, LZ, DDG, 0, PF ? G2 : -1>(c, sp, a, sc, twz, bx, lds, pre);
+            __syncthreads();
+            zpass_body<T, LZ, G2, PF ? ZPF : 0, -1>(c, sp, a, sc, twz, bx, lds, pre);
+        }
     } else {
-        zpass_body<T, LZ, DDG>(c, sp, a, sc, twz, bx, lds);
-        __syncthreads();
-        zpass_body<T, LZ, G2>(c, sp, a, sc, twz, bx, lds);
+        if (bx & 1) {
+            zpass_body<T, LZ, G2, 0, -1>(c, sp, a, sc, twz, bx, lds, pre);
+            __syncthreads();
+            zpass_body<T, LZ, DDG, 0, -1>(c, sp, a, sc, twz, bx, lds, pre);
+        } else {
+            zpass_body<T, LZ, DDG, 0, -1>(c, sp, a, sc, twz, bx, lds, pre);
+            __syncthreads();
+            zpass_body<T, LZ, G2, 0, -1>(c, sp, a, sc, twz, bx, lds, pre);
+        }
     }
 }
 
@@ -2594,7 +2784,8 @@ void kb_zpass(const Rep<T>* __restrict__ reps, int want_acouK)
     ZArgs<T> a = r.za;
     if (!want_acouK) a.acouK = nullptr;
     __shared__ ZLds<T, LZ, zbuf_size<T, LZ, GROUP>()> lds;
-    zpass_body<T, LZ, GROUP>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x, lds);
+    cpx<T> pre[ZPF][ZTile<T, LZ>::PPT];
+    zpass_body<T, LZ, GROUP, 0, -1>(r.c, r.sp, a, r.sc, r.twz, blockIdx.x, lds, pre);
 }
 
 template <typename T, int LZ, int G2, int DDG = ZG_DD>

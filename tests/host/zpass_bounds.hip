@@ -18,7 +18,14 @@
 //     compiled Lx (Mx = Lx / 2) and Ly;
 //   * the HBM offset l + t * nlines stays below Mh = Mx Ly Lz, and the k-array
 //     index below Kh;
-//   * the block-sum scratch (NS * NT/64 doubles) fits red[NT].
+//   * the block-sum scratch (NS * NT/64 doubles) fits red[NT];
+//   * the paired kernel's LDS-DMA prefetch of the second group's first ZPF
+//     field tiles: every 16-byte chunk's LDS bytes (wave-uniform base plus
+//     16 bytes per lane) lie in the staging area, which fits the pool, the
+//     chunks cover every staged element of a field exactly once, the HBM
+//     bytes a copied chunk reads lie inside the spectrum, and the reads that
+//     move the staged tiles into registers stay in the staging area (or, for
+//     chunks not copied, inside the spectrum).
 // Run on the CPU; prints one line per configuration group and exits 1 on the
 // first violation.
 //
@@ -216,6 +223,69 @@ static void check_group(const char* what, size_t pool_elems, int side_nf)
     }
 }
 
+// The prefetch of zpair_body (zprefetch_issue / zprefetch_collect) for every
+// compiled Lx, Ly at this Lz.
+template <typename T, int LZ>
+static void check_prefetch(size_t pool_elems)
+{
+    using P = ZPrefetch<T, LZ>;
+    if constexpr (!P::OK) return;
+    constexpr int TB = P::TB, EL = P::EL, CH = P::CH, NCH = P::NCH, PPT = P::PPT;
+    using Map = ZMap<NT, TB, LZ>;
+    CHECK((size_t)ZPF * EL <= pool_elems, "prefetch staging LZ=%d: %d > %zu", LZ, ZPF * EL, pool_elems);
+    CHECK(CH * (int)sizeof(cpx<T>) == 16 && TB % CH == 0, "prefetch chunk LZ=%d", LZ);
+    // LDS side: wave-uniform base (it*NT + wave*64)*CH, lane l at +l*CH
+    std::vector<int> cover((size_t)ZPF * EL, 0);
+    for (int f = 0; f < ZPF; ++f)
+        for (int it = 0; it < (NCH + NT - 1) / NT; ++it)
+            for (int tid = 0; tid < g_threads; ++tid) {
+                const int j = it * NT + tid;
+                if (!(NCH % NT == 0 || j < NCH)) continue;
+                const int wbase = tid & ~63, lane = tid & 63;
+                const size_t lds0 = (size_t)f * EL + (size_t)(it * NT + wbase) * CH + (size_t)lane * CH;
+                CHECK(lds0 + CH <= (size_t)ZPF * EL, "prefetch LDS LZ=%d f=%d it=%d tid=%d", LZ, f, it, tid);
+                const int e = j * CH, t = e / TB, b = e - t * TB;
+                CHECK(t < LZ && b + CH <= TB, "prefetch chunk row LZ=%d t=%d b=%d", LZ, t, b);
+                for (int q = 0; q < CH; ++q) {
+                    CHECK(lds0 + q == (size_t)f * EL + (size_t)t * TB + b + q, "prefetch layout LZ=%d", LZ);
+                    cover[lds0 + q] += 1;
+                }
+            }
+    for (size_t e = 0; e < cover.size(); ++e) CHECK(cover[e] == 1, "prefetch coverage LZ=%d e=%zu", LZ, e);
+    // HBM side and the collect, for every tile of every compiled Lx, Ly
+    for (int Lx : kSizes) {
+        if (Lx % 2) continue;
+        const int Mx = Lx / 2;
+        for (int Ly : kSizes) {
+            const int nlines = Mx * Ly;
+            const int ntiles = (nlines + TB - 1) / TB;
+            const size_t Mh = (size_t)nlines * LZ;
+            for (int w : {0, 1, ntiles / 2, ntiles - 2, ntiles - 1}) {
+                if (w < 0 || w >= ntiles) continue;
+                for (int j = 0; j < NCH; ++j) {
+                    const int e = j * CH, t = e / TB, b = e - t * TB;
+                    if (zchunk_ok<T, LZ>(w, b, nlines)) {
+                        const size_t off = (size_t)(w * TB + b) + (size_t)t * nlines;
+                        CHECK(off + CH <= Mh, "prefetch HBM LZ=%d Lx=%d Ly=%d w=%d j=%d", LZ, Lx, Ly, w, j);
+                        CHECK((off * sizeof(cpx<T>)) % 16 == 0, "prefetch alignment LZ=%d Lx=%d Ly=%d", LZ, Lx, Ly);
+                    }
+                }
+                for (int tid = 0; tid < g_threads; ++tid)
+                    for (int i = 0; i < PPT; ++i) {
+                        if (!Map::live(tid, i)) continue;
+                        int t, b;
+                        Map::hbm(tid, i, t, b);
+                        const int l = w * TB + b;
+                        if (zchunk_ok<T, LZ>(w, b - b % CH, nlines))
+                            CHECK((size_t)(2 * EL + t * TB + b) < (size_t)ZPF * EL, "prefetch collect LZ=%d", LZ);
+                        else if (l < nlines)
+                            CHECK((size_t)l + (size_t)t * nlines < Mh, "prefetch direct read LZ=%d", LZ);
+                    }
+            }
+        }
+    }
+}
+
 template <typename T, int LZ>
 static void check_edge()
 {
@@ -227,6 +297,8 @@ static void check_edge()
     check_group<T, LZ, ZG_STRAIN, 0>("zpair(strain) strain", pool_s, 6);
     check_group<T, LZ, ZG_DD, 0>("zpair(acou) dd", pool_a, 9);
     check_group<T, LZ, ZG_ACOU, 0>("zpair(acou) acou", pool_a, 9);
+    check_prefetch<T, LZ>(pool_s);
+    check_prefetch<T, LZ>(pool_a);
     // static LDS of the paired kernel fits the 64 KB static limit
     CHECK(sizeof(ZLds<T, LZ, zpair_buf_size<T, LZ, ZG_STRAIN>()>) <= 64 * 1024, "static LDS strain LZ=%d", LZ);
     CHECK(sizeof(ZLds<T, LZ, zpair_buf_size<T, LZ, ZG_ACOU>()>) <= 64 * 1024, "static LDS acou LZ=%d", LZ);
