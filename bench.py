@@ -48,13 +48,13 @@ def kernel_bytes(name: str, N: int, Kh: int, nf: int, ws: int, materialize_frac:
     """Algorithmic HBM bytes of one launch (compulsory reads + writes).
 
     A packed half spectrum holds N/2 complex points (kx = 0 and kx = Lx/2
-    share column 0); ws: bytes of one spectral word (8 FP64, 4 FP32), a
-    complex point is 2 ws; state fields are FP64 in both modes; the L(k)
-    table has Kh = (Lx/2+1) Ly Lz points of 6 words."""
+    share column 0); ws: bytes of one word (8 FP64, 4 FP32) of the spectra
+    and of the state fields, which follow the mode; a complex point is 2 ws;
+    the L(k) table has Kh = (Lx/2+1) Ly Lz points of 6 words."""
     cw = 2 * ws
     M = N // 2
     if name == "xfwd_drift_r2c":      # read u, pk, code; write u, nf spectra
-        return 3 * 8 * N * 3 + N + nf * cw * M
+        return 3 * ws * N * 3 + N + nf * cw * M
     if name in ("y_forward", "y_inverse"):
         return 2 * nf * cw * M
     if name == "z_dipole_dipole":     # 3 spectra in/out + 6-word L(k) table
@@ -64,13 +64,14 @@ def kernel_bytes(name: str, N: int, Kh: int, nf: int, ws: int, materialize_frac:
     if name == "z_pair":              # both z groups in one launch
         return 2 * 3 * cw * M + 6 * ws * Kh + 2 * (nf - 3) * cw * M
     if name == "xinv_c2r_local_kick":  # read nf spectra, u, pk, code; write pk (+p, F when materialised)
-        return nf * cw * M + 3 * 8 * N * 3 + N + materialize_frac * 2 * 3 * 8 * N
+        return nf * cw * M + 3 * ws * N * 3 + N + materialize_frac * 2 * 3 * ws * N
     return 0.0
 
 
 def step_bytes(N: int, Kh: int, ws: int) -> float:
-    """Compulsory bytes of a 3-sweep fused md step: 18 N w_s + 78 Kh w_f (w_s = 8)."""
-    return 18 * N * 8 + 78 * Kh * ws
+    """Compulsory bytes of a 3-sweep fused md step: 18 N w_s + 78 Kh w_f, the
+    state word w_s and the spectral word w_f both ws (FP64 or FP32 mode)."""
+    return 18 * N * ws + 78 * Kh * ws
 
 
 def cpu_baseline(p, u0, steps: int):

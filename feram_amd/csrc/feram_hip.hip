@@ -127,11 +127,11 @@ int run_zpair(fhip_ctx *ctx, const ZArgs<T> &a)
 
 
 // ---- averages (add_to_Average) ----------------------------------------------
-int avg_add(fhip_ctx *ctx)
+template <typename S>
+int avg_cells(fhip_ctx *ctx)
 {
-    if (!ctx->davg) return fail(ctx, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
     Consts c = ctx->c;
-    Fields f = ctx->f;
+    FieldsT<S> f = fields<S>(ctx);
     double *part = ctx->partials;
     int mb = ctx->maxblocks, kb = ctx->kblocks;
     unsigned long long *d1 = ctx->dist1;
@@ -139,9 +139,18 @@ int avg_add(fhip_ctx *ctx)
     const int ravg = ctx->avg_ravg ? 1 : 0;
     hipStream_t s = ctx->stream;
     if (ctx->avg_hist)
-        RC(launch(ctx, K_AVG, [&] { k_avg_cells<1><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); }));
-    else
-        RC(launch(ctx, K_AVG, [&] { k_avg_cells<0><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); }));
+        return launch(ctx, K_AVG, [&] { k_avg_cells<1, S><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); });
+    return launch(ctx, K_AVG, [&] { k_avg_cells<0, S><<<kb, NT, 0, s>>>(c, f, part, mb, d1, d3, ravg); });
+}
+
+int avg_add(fhip_ctx *ctx)
+{
+    if (!ctx->davg) return fail(ctx, FHIP_ERR_STATE, "averages not reset (fhip_avg_reset)");
+    if (ctx->precision == FHIP_FP32) RC(avg_cells<float>(ctx));
+    else RC(avg_cells<double>(ctx));
+    double *part = ctx->partials;
+    int mb = ctx->maxblocks, kb = ctx->kblocks;
+    hipStream_t s = ctx->stream;
     Scalars *sc = ctx->dsc;
     AvgAcc *acc = ctx->davg;
     RC(launch(ctx, K_AVG, [&] { k_avg_fold<<<1, NT, 0, s>>>(sc, acc, part, mb, kb); }));
@@ -160,17 +169,23 @@ int nforward(const fhip_ctx *ctx) { return 9 + (ctx->c.acoustic ? 3 : 0) + (ctx-
 enum StepKind { SK_FORCE = 0, SK_NP = 1, SK_LF = 2, SK_VS = 3 };
 
 // Heat-source and sink layers at the start of a leapfrog step (leapfrog.F:38-83).
-int heat_source_layers(fhip_ctx *ctx)
+template <typename S>
+int heat_source_layers_t(fhip_ctx *ctx)
 {
     Consts c = ctx->c;
-    Fields f = ctx->f;
+    FieldsT<S> f = fields<S>(ctx);
     const int *nl = ctx->nlayer;
     double *el = ctx->el;
     const double de = ctx->delta_e, dea = ctx->delta_e_acou;
     hipStream_t s = ctx->stream;
-    RC(launch(ctx, K_SCALE, [&] { k_layer_ke<<<ctx->Lz, NT, 0, s>>>(c, f, nl, el); }));
+    RC(launch(ctx, K_SCALE, [&] { k_layer_ke<S><<<ctx->Lz, NT, 0, s>>>(c, f, nl, el); }));
     const dim3 g((ctx->Lx * ctx->Ly + NT - 1) / NT, 2);
-    return launch(ctx, K_SCALE, [&] { k_layer_scale<<<g, NT, 0, s>>>(c, f, el, de, dea); });
+    return launch(ctx, K_SCALE, [&] { k_layer_scale<S><<<g, NT, 0, s>>>(c, f, el, de, dea); });
+}
+
+int heat_source_layers(fhip_ctx *ctx)
+{
+    return ctx->precision == FHIP_FP32 ? heat_source_layers_t<float>(ctx) : heat_source_layers_t<double>(ctx);
 }
 
 template <typename T>
@@ -210,12 +225,12 @@ int force_pipeline_t(fhip_ctx *ctx, int kind, bool materialize, int next_step)
     RC(run_xinv<T>(ctx, ia, nf));
     if (film) {                                   // mirror_force_mirror.F:80-84
         Consts c = ctx->c;
-        Fields f = ctx->f;
+        FieldsT<T> f = fields<T>(ctx);
         const Scalars *sc = ctx->dsc;
         const int mode = ia.mode, mat = ia.materialize;
         const int nb = (int)(((size_t)(ctx->Lz - c.mir_lo) * ctx->Lx * ctx->Ly + NT - 1) / NT);
         hipStream_t s = ctx->stream;
-        RC(launch(ctx, K_XINV, [&] { k_mirror_kick<<<nb, NT, 0, s>>>(c, f, sc, mode, mat); }));
+        RC(launch(ctx, K_XINV, [&] { k_mirror_kick<T><<<nb, NT, 0, s>>>(c, f, sc, mode, mat); }));
     }
     {
         ScalarConsts k = ctx->sk;
@@ -267,6 +282,7 @@ std::vector<unsigned char> step_key(const fhip_ctx *ctx, int kind, int next_step
     put(&ctx->c, sizeof(Consts));
     put(&ctx->sk, sizeof(ScalarConsts));
     put(&ctx->f, sizeof(Fields));
+    put(&ctx->f32, sizeof(ctx->f32));
     const double de[2] = {ctx->delta_e, ctx->delta_e_acou};
     put(de, sizeof(de));
     const void *ptrs[7] = {ctx->davg, ctx->dist1, ctx->dist3, ctx->el, ctx->el_sum, ctx->nlayer, ctx->modk};
@@ -339,16 +355,22 @@ int push_scalars(fhip_ctx *ctx)
 }
 
 
-int launch_kick(fhip_ctx *ctx, bool kick)
+template <typename S>
+int launch_kick_t(fhip_ctx *ctx, bool kick)
 {
     Consts c = ctx->c;
-    Fields f = ctx->f;
+    FieldsT<S> f = fields<S>(ctx);
     const Scalars *sc = ctx->dsc;
     double *part = ctx->partials;
     int mb = ctx->maxblocks, kb = ctx->kblocks;
     hipStream_t s = ctx->stream;
-    if (kick) return launch(ctx, K_KICK, [&] { k_kick<1><<<kb, NT, 0, s>>>(c, f, sc, part, mb); });
-    return launch(ctx, K_KE, [&] { k_kick<0><<<kb, NT, 0, s>>>(c, f, sc, part, mb); });
+    if (kick) return launch(ctx, K_KICK, [&] { k_kick<1, S><<<kb, NT, 0, s>>>(c, f, sc, part, mb); });
+    return launch(ctx, K_KE, [&] { k_kick<0, S><<<kb, NT, 0, s>>>(c, f, sc, part, mb); });
+}
+
+int launch_kick(fhip_ctx *ctx, bool kick)
+{
+    return ctx->precision == FHIP_FP32 ? launch_kick_t<float>(ctx, kick) : launch_kick_t<double>(ctx, kick);
 }
 
 // ---- host-side layout conversion -------------------------------------------
@@ -364,6 +386,27 @@ int alloc_uavg(fhip_ctx *ctx)
     RC(dalloc(ctx, &blk, 3 * (size_t)ctx->N));
     for (int q = 0; q < 3; ++q) ctx->f.uavg[q] = blk + q * (size_t)ctx->N;
     return FHIP_OK;
+}
+
+// State fields are stored in FP32 in the FP32 mode (the host side always
+// sees FP64): their device pointers of that type.
+bool state_field(int field)
+{
+    return field == FHIP_DIPO_R || field == FHIP_DIPO_P || field == FHIP_DVDDI || field == FHIP_ACOU_R ||
+           field == FHIP_ACOU_P || field == FHIP_DVDAC;
+}
+
+float **field_ptrs32(fhip_ctx *ctx, int field)
+{
+    switch (field) {
+    case FHIP_DIPO_R: return ctx->f32.u;
+    case FHIP_DIPO_P: return ctx->f32.p;
+    case FHIP_DVDDI: return ctx->f32.F;
+    case FHIP_ACOU_R: return ctx->f32.w;
+    case FHIP_ACOU_P: return ctx->f32.pw;
+    case FHIP_DVDAC: return ctx->f32.dVdw;
+    default: return nullptr;
+    }
 }
 
 double **field_ptrs(fhip_ctx *ctx, int field)
@@ -839,13 +882,16 @@ std::vector<CkptSection> ckpt_sections(fhip_ctx *ctx, int flags)
     std::vector<CkptSection> v;
     const size_t N = ctx->N;
     v.push_back({ctx->dsc, sizeof(Scalars)});
-    v.push_back({ctx->f.u[0], 3 * N * sizeof(double)});
-    v.push_back({ctx->f.p[0], 3 * N * sizeof(double)});
-    v.push_back({ctx->f.F[0], 3 * N * sizeof(double)});
+    // the state in the context's precision
+    const bool f32 = ctx->precision == FHIP_FP32;
+    const size_t sb = 3 * N * (f32 ? sizeof(float) : sizeof(double));
+    v.push_back({f32 ? (void *)ctx->f32.u[0] : (void *)ctx->f.u[0], sb});
+    v.push_back({f32 ? (void *)ctx->f32.p[0] : (void *)ctx->f.p[0], sb});
+    v.push_back({f32 ? (void *)ctx->f32.F[0] : (void *)ctx->f.F[0], sb});
     if (ctx->c.acoustic) {
-        v.push_back({ctx->f.w[0], 3 * N * sizeof(double)});
-        v.push_back({ctx->f.pw[0], 3 * N * sizeof(double)});
-        v.push_back({ctx->f.dVdw[0], 3 * N * sizeof(double)});
+        v.push_back({f32 ? (void *)ctx->f32.w[0] : (void *)ctx->f.w[0], sb});
+        v.push_back({f32 ? (void *)ctx->f32.pw[0] : (void *)ctx->f.pw[0], sb});
+        v.push_back({f32 ? (void *)ctx->f32.dVdw[0] : (void *)ctx->f.dVdw[0], sb});
     }
     if (flags & CK_ACOUK) v.push_back({ctx->acouK, 3 * (size_t)ctx->Kh * sizeof(cpx<double>)});
     if (flags & CK_AVG) v.push_back({ctx->davg, sizeof(AvgAcc)});
@@ -942,20 +988,30 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     ctx->maxblocks = std::max(std::max(ctx->xblocks, std::max(ctx->zblocks[0], ctx->zblocks[1])), ctx->kblocks);
 
     const size_t N = ctx->N, Kh = ctx->Kh;
-    double *blk = nullptr;
-    // u, p, pk, F (3 each); w, p_w, dV/dw only with acoustic MD
+    // u, p, pk, F (3 each); w, p_w, dV/dw only with acoustic MD; FP32 in
+    // the FP32 mode
     const int nreal = ctx->c.acoustic ? 21 : 12;
-    RC(dalloc(ctx, &blk, nreal * N));
-    for (int q = 0; q < 3; ++q) {
-        ctx->f.u[q] = blk + (0 + q) * N;
-        ctx->f.p[q] = blk + (3 + q) * N;
-        ctx->f.pk[q] = blk + (6 + q) * N;
-        ctx->f.F[q] = blk + (9 + q) * N;
-        if (ctx->c.acoustic) {
-            ctx->f.w[q] = blk + (12 + q) * N;
-            ctx->f.pw[q] = blk + (15 + q) * N;
-            ctx->f.dVdw[q] = blk + (18 + q) * N;
+    auto place = [&](auto &fs, auto *blk) {
+        for (int q = 0; q < 3; ++q) {
+            fs.u[q] = blk + (0 + q) * N;
+            fs.p[q] = blk + (3 + q) * N;
+            fs.pk[q] = blk + (6 + q) * N;
+            fs.F[q] = blk + (9 + q) * N;
+            if (ctx->c.acoustic) {
+                fs.w[q] = blk + (12 + q) * N;
+                fs.pw[q] = blk + (15 + q) * N;
+                fs.dVdw[q] = blk + (18 + q) * N;
+            }
         }
+    };
+    if (p->precision == FHIP_FP32) {
+        float *blk = nullptr;
+        RC(dalloc(ctx, &blk, nreal * N));
+        place(ctx->f32, blk);
+    } else {
+        double *blk = nullptr;
+        RC(dalloc(ctx, &blk, nreal * N));
+        place(ctx->f, blk);
     }
     RC(dalloc(ctx, &ctx->code, N));
     RC(set_cells_impl(ctx, nullptr, nullptr, nullptr));
@@ -1071,7 +1127,9 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
         for (int y = 0; y < ctx->Ly; ++y)
             for (int x = 0; x < ctx->Lx; ++x)
                 h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] = mod[padded_index(ctx, 0, 1, x, y, z)];
-    double *d = ctx->f.F[0];      // scratch: overwritten by the next force evaluation
+    // scratch: overwritten by the next force evaluation (the dV/du block holds
+    // 3N words of the state type, at least N doubles)
+    double *d = ctx->precision == FHIP_FP32 ? reinterpret_cast<double *>(ctx->f32.F[0]) : ctx->f.F[0];
     HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
     const double *flds[1] = {d};
     {
@@ -1109,8 +1167,10 @@ static int put_impl(fhip_ctx *ctx, int field, const double *host, bool soa)
         RC(dalloc(ctx, &blk, 3 * (size_t)ctx->N));
         for (int q = 0; q < 3; ++q) ctx->f.eloc[q] = blk + q * (size_t)ctx->N;
     }
+    const bool f32 = ctx->precision == FHIP_FP32 && state_field(field);
     double **d = field_ptrs(ctx, field);
-    if (!d || !d[0]) return fail(ctx, FHIP_ERR_ARG, "field not present in this configuration");
+    float **d32 = f32 ? field_ptrs32(ctx, field) : nullptr;
+    if (f32 ? (!d32 || !d32[0]) : (!d || !d[0])) return fail(ctx, FHIP_ERR_ARG, "field not present in this configuration");
     const size_t N = ctx->N;
     std::vector<double> h(3 * N);
     if (soa) {
@@ -1122,9 +1182,16 @@ static int put_impl(fhip_ctx *ctx, int field, const double *host, bool soa)
                     for (int q = 0; q < 3; ++q)
                         h[q * N + x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] = host[padded_index(ctx, q, 3, x, y, z)];
     }
-    for (int q = 0; q < 3; ++q)
-        HIPCHK(hipMemcpyAsync(d[q], h.data() + q * N, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (f32) {
+        const std::vector<float> hf(h.begin(), h.end());          // rounded to nearest
+        for (int q = 0; q < 3; ++q)
+            HIPCHK(hipMemcpyAsync(d32[q], hf.data() + q * N, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    } else {
+        for (int q = 0; q < 3; ++q)
+            HIPCHK(hipMemcpyAsync(d[q], h.data() + q * N, sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     if (field == FHIP_E_LOCAL) ctx->c.has_eloc = 1;
     return FHIP_OK;
 }
@@ -1138,6 +1205,14 @@ static int get_impl(fhip_ctx *ctx, int field, double *host, bool soa)
     std::vector<double> h(3 * N);
     if (field == FHIP_ACOU_R_OPT) {
         RC(reconstruct_w(ctx, h));
+    } else if (ctx->precision == FHIP_FP32 && state_field(field)) {
+        float **d = field_ptrs32(ctx, field);
+        if (!d || !d[0]) return fail(ctx, FHIP_ERR_ARG, "field not present in this configuration");
+        std::vector<float> hf(3 * N);
+        for (int q = 0; q < 3; ++q)
+            HIPCHK(hipMemcpyAsync(hf.data() + q * N, d[q], sizeof(float) * N, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < 3 * N; ++i) h[i] = hf[i];
     } else {
         double **d = field_ptrs(ctx, field);
         if (!d || !d[0]) return fail(ctx, FHIP_ERR_ARG, "field not present in this configuration");
@@ -1580,11 +1655,16 @@ int fhip_velocity_scaling(fhip_ctx *ctx, int nsteps, fhip_energy *e)
     RC(one_step(ctx, SK_VS, true, 0));
     {
         Consts c = ctx->c;
-        Fields f = ctx->f;
         Scalars *sc = ctx->dsc;
         int kb = ctx->kblocks;
         hipStream_t s = ctx->stream;
-        RC(launch(ctx, K_SCALE, [&] { k_apply_scale<<<kb, NT, 0, s>>>(c, f, sc); }));
+        if (ctx->precision == FHIP_FP32) {
+            FieldsT<float> f = fields<float>(ctx);
+            RC(launch(ctx, K_SCALE, [&] { k_apply_scale<float><<<kb, NT, 0, s>>>(c, f, sc); }));
+        } else {
+            Fields f = ctx->f;
+            RC(launch(ctx, K_SCALE, [&] { k_apply_scale<double><<<kb, NT, 0, s>>>(c, f, sc); }));
+        }
         RC(launch(ctx, K_SCALE, [&] { k_reset_scale<<<1, 1, 0, s>>>(sc); }));
     }
     RC(pull_scalars(ctx));

@@ -105,7 +105,7 @@ void fill_rep(Rep<T> &r, fhip_ctx *ctx, int method)
     std::memset(&r, 0, sizeof(r));
     r.c = ctx->c;
     r.k = ctx->sk;
-    r.f = ctx->f;
+    r.f = fields<T>(ctx);
     r.sp = tset<T>(ctx).sp;
     r.za = zargs<T>(ctx, !ctx->c.acoustic);
     r.xa.mode = xf_mode(method);

@@ -71,7 +71,8 @@ struct fhip_ctx {
     std::string err;
     fhip::Consts c{};
     fhip::ScalarConsts sk{};
-    fhip::Fields f{};
+    fhip::Fields f{};                 // FP64 state (plain mode), local field, dipoRavg, cell codes
+    fhip::FieldsT<float> f32{};       // FP32 state (FP32 mode); its other pointers are taken from f
     int precision = FHIP_FP64;
     TypedSet<double> dset;            // FP64 spectra (plain mode) and the FP64 set-up
     TypedSet<float> fset;             // FP32 spectra (FP32 mode)
@@ -119,6 +120,22 @@ template <typename T> constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
 template <typename T> TypedSet<T> &tset(fhip_ctx *ctx);
 template <> inline TypedSet<double> &tset<double>(fhip_ctx *ctx) { return ctx->dset; }
 template <> inline TypedSet<float> &tset<float>(fhip_ctx *ctx) { return ctx->fset; }
+
+// Per-cell fields with state of type S: the state pointers are those of the
+// context's precision (null in the other type), the local field, dipoRavg
+// and cell-code pointers are shared.
+template <typename S> fhip::FieldsT<S> fields(const fhip_ctx *ctx);
+template <> inline fhip::Fields fields<double>(const fhip_ctx *ctx) { return ctx->f; }
+template <> inline fhip::FieldsT<float> fields<float>(const fhip_ctx *ctx)
+{
+    fhip::FieldsT<float> g = ctx->f32;
+    for (int q = 0; q < 3; ++q) {
+        g.eloc[q] = ctx->f.eloc[q];
+        g.uavg[q] = ctx->f.uavg[q];
+    }
+    g.code = ctx->f.code;
+    return g;
+}
 
 inline int fail(fhip_ctx *ctx, int code, const std::string &msg)
 {

@@ -137,18 +137,24 @@ enum EnergyIdx {
     E_ACOU_KIN = 0, E_DIPO_KIN, E_SR, E_LR, E_DE, E_UNH, E_HS, E_HC, E_IS, E_IC, E_IM, E_TOTAL, E_HNP
 };
 
-struct Fields {
-    double* u[3];
-    double* p[3];
-    double* pk[3];       // Nose-Poincare: momentum after the first half kick of the next step
-    double* F[3];        // materialised dV/du
-    double* w[3];
-    double* pw[3];
-    double* dVdw[3];     // materialised dV/dw
+// Per-cell fields.  The state (u, p, the pre-kicked p, dV/du and the
+// acoustic w, p_w, dV/dw) is stored in S: FP64 in the plain mode, FP32 in the
+// FP32 mode, where every update is computed in FP64 registers and rounded
+// once on the store.  The local field and the dipoRavg sums stay FP64.
+template <typename S>
+struct FieldsT {
+    S* u[3];
+    S* p[3];
+    S* pk[3];            // Nose-Poincare: momentum after the first half kick of the next step
+    S* F[3];             // materialised dV/du
+    S* w[3];
+    S* pw[3];
+    S* dVdw[3];          // materialised dV/dw
     double* eloc[3];
     double* uavg[3];     // c%dipoRavg: sum of u over the averaging steps
     const unsigned char* code;   // bits 0-1 species, bit 2 frozen (1/m = 0), bit 3 counted (sum_p)
 };
+using Fields = FieldsT<double>;
 
 template <typename T>
 struct Spectra {
@@ -247,7 +253,7 @@ struct xfwd_rows {
 #endif
 
 template <typename T, int LX, int RB = xfwd_rows<LX>::value>
-__device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XfwdArgs& a,
+__device__ __forceinline__ void xfwd_body(const Consts& c, const FieldsT<T>& f, const Spectra<T>& sp, const XfwdArgs& a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* twx, int bx)
 {
@@ -426,7 +432,7 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const Fields& f, cons
 }
 
 template <typename T, int LX>
-__global__ __launch_bounds__(NT) void k_xfwd(Consts c, Fields f, Spectra<T> sp, XfwdArgs a,
+__global__ __launch_bounds__(NT) void k_xfwd(Consts c, FieldsT<T> f, Spectra<T> sp, XfwdArgs a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* __restrict__ twx)
 {
@@ -1594,7 +1600,7 @@ struct XinvArgs {
 // c2r along x of NF spectral slots into real device fields (XI_GENERIC,
 // set-up and output work), unnormalised: two fields per complex transform.
 template <typename T, int LX, int NF>
-__device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, const Spectra<T>& sp, const XinvArgs& a,
+__device__ __forceinline__ void xinv_body(const Consts& c, const FieldsT<T>& f, const Spectra<T>& sp, const XinvArgs& a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* twx, int bx)
 {
@@ -1654,7 +1660,7 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const Fields& f, cons
 // dVddi_inho_coupling.F, dVddi-unharmonic.F, mirror_force_mirror.F:39-65 and
 // dVddi-homo-coupling.F, the inhomogeneous coupling added field by field.
 template <typename T, int LX, int NF>
-__device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f, const Spectra<T>& sp,
+__device__ __forceinline__ void xinv_step_body(const Consts& c, const FieldsT<T>& f, const Spectra<T>& sp,
                                                const XinvArgs& a, const Scalars* __restrict__ sc,
                                                const cpx<T>* twx, int bx)
 {
@@ -1916,7 +1922,7 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const Fields& f,
 }
 
 template <typename T, int LX, int NF>
-__global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, XinvArgs a,
+__global__ __launch_bounds__(NT) void k_xinv(Consts c, FieldsT<T> f, Spectra<T> sp, XinvArgs a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* __restrict__ twx)
 {
@@ -1930,7 +1936,8 @@ __global__ __launch_bounds__(NT) void k_xinv(Consts c, Fields f, Spectra<T> sp, 
 // kicked with it as xinv kicks the other cells.  Mirrored layers are never
 // counted (read_defects_u.F:52-54), so they add nothing to the kinetic sums.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mirror_kick_body(const Consts& c, const Fields& f, const Scalars* sc, int mode,
+template <typename S>
+__device__ __forceinline__ void mirror_kick_body(const Consts& c, const FieldsT<S>& f, const Scalars* sc, int mode,
                                                     int materialize, int bx)
 {
     const size_t plane = (size_t)c.Lx * c.Ly;
@@ -1965,7 +1972,8 @@ __device__ __forceinline__ void mirror_kick_body(const Consts& c, const Fields& 
     }
 }
 
-static __global__ __launch_bounds__(NT) void k_mirror_kick(Consts c, Fields f, const Scalars* __restrict__ sc,
+template <typename S>
+__global__ __launch_bounds__(NT) void k_mirror_kick(Consts c, FieldsT<S> f, const Scalars* __restrict__ sc,
                                                            int mode, int materialize)
 {
     mirror_kick_body(c, f, sc, mode, materialize, blockIdx.x);
@@ -1974,8 +1982,8 @@ static __global__ __launch_bounds__(NT) void k_mirror_kick(Consts c, Fields f, c
 // ---------------------------------------------------------------------------
 // Kinetic energy of p (optionally after the first Nose-Poincare half kick).
 // ---------------------------------------------------------------------------
-template <int KICK>
-__device__ __forceinline__ void kick_body(const Consts& c, const Fields& f, const Scalars* sc,
+template <int KICK, typename S>
+__device__ __forceinline__ void kick_body(const Consts& c, const FieldsT<S>& f, const Scalars* sc,
                                              double* partials, int maxblocks, int bx)
 {
     __shared__ double red[NT];
@@ -1997,7 +2005,7 @@ __device__ __forceinline__ void kick_body(const Consts& c, const Fields& f, cons
         }
         if (c.acoustic) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) kea += f.pw[q][cc] * f.pw[q][cc] * c.mwinv[s];
+            for (int q = 0; q < 3; ++q) kea += (double)f.pw[q][cc] * f.pw[q][cc] * c.mwinv[s];
         }
     }
     double r[2] = {ke, kea};
@@ -2008,11 +2016,11 @@ __device__ __forceinline__ void kick_body(const Consts& c, const Fields& f, cons
     }
 }
 
-template <int KICK>
-__global__ __launch_bounds__(NT) void k_kick(Consts c, Fields f, const Scalars* __restrict__ sc,
+template <int KICK, typename S>
+__global__ __launch_bounds__(NT) void k_kick(Consts c, FieldsT<S> f, const Scalars* __restrict__ sc,
                                              double* partials, int maxblocks)
 {
-    kick_body<KICK>(c, f, sc, partials, maxblocks, blockIdx.x);
+    kick_body<KICK, S>(c, f, sc, partials, maxblocks, blockIdx.x);
 }
 
 // Deterministic fold of the nb per-block partials of NS slots by one
@@ -2312,7 +2320,8 @@ static __global__ __launch_bounds__(NT) void k_ke_report(ScalarConsts k, Scalars
 }
 
 // Apply the pending velocity-scaling factors so the host sees scaled momenta.
-__device__ __forceinline__ void apply_scale_body(const Consts& c, const Fields& f, const Scalars* sc, int bx)
+template <typename S>
+__device__ __forceinline__ void apply_scale_body(const Consts& c, const FieldsT<S>& f, const Scalars* sc, int bx)
 {
     const size_t cc = (size_t)bx * NT + threadIdx.x;
     if (cc >= (size_t)c.N) return;
@@ -2326,7 +2335,8 @@ __device__ __forceinline__ void apply_scale_body(const Consts& c, const Fields& 
     }
 }
 
-static __global__ __launch_bounds__(NT) void k_apply_scale(Consts c, Fields f, Scalars* sc)
+template <typename S>
+__global__ __launch_bounds__(NT) void k_apply_scale(Consts c, FieldsT<S> f, Scalars* sc)
 {
     apply_scale_body(c, f, sc, blockIdx.x);
 }
@@ -2343,7 +2353,8 @@ static __global__ void k_reset_scale(Scalars* sc)
 // layer Lz/4 scaled up and of layer 3Lz/4 down so that their energies change
 // by +-Lx Ly delta_e (and delta_e_acou for the acoustic momenta).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void layer_ke_body(const Consts& c, const Fields& f, const int* __restrict__ nlayer,
+template <typename S>
+__device__ __forceinline__ void layer_ke_body(const Consts& c, const FieldsT<S>& f, const int* __restrict__ nlayer,
                                               double* el, int iz)
 {
     __shared__ double red[NT];
@@ -2355,10 +2366,10 @@ __device__ __forceinline__ void layer_ke_body(const Consts& c, const Fields& f, 
         const int s = code & 3;
         const double minv = (code & 4) ? 0.0 : c.minv[s];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) ke += f.p[q][cc] * f.p[q][cc] * minv;           // leapfrog.F:47-54
+        for (int q = 0; q < 3; ++q) ke += (double)f.p[q][cc] * f.p[q][cc] * minv;   // leapfrog.F:47-54
         if (c.acoustic) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) kea += f.pw[q][cc] * f.pw[q][cc] * c.mwinv[s];   // :61-68
+            for (int q = 0; q < 3; ++q) kea += (double)f.pw[q][cc] * f.pw[q][cc] * c.mwinv[s];   // :61-68
         }
     }
     double r[2] = {ke, kea};
@@ -2369,12 +2380,14 @@ __device__ __forceinline__ void layer_ke_body(const Consts& c, const Fields& f, 
     }
 }
 
-static __global__ __launch_bounds__(NT) void k_layer_ke(Consts c, Fields f, const int* __restrict__ nlayer, double* el)
+template <typename S>
+__global__ __launch_bounds__(NT) void k_layer_ke(Consts c, FieldsT<S> f, const int* __restrict__ nlayer, double* el)
 {
     layer_ke_body(c, f, nlayer, el, blockIdx.x);
 }
 
-__device__ __forceinline__ void layer_scale_body(const Consts& c, const Fields& f, const double* __restrict__ el,
+template <typename S>
+__device__ __forceinline__ void layer_scale_body(const Consts& c, const FieldsT<S>& f, const double* __restrict__ el,
                                                  double delta_e, double delta_e_acou, int bx, int by)
 {
     const int LxLy = c.Lx * c.Ly;
@@ -2398,7 +2411,8 @@ __device__ __forceinline__ void layer_scale_body(const Consts& c, const Fields& 
     }
 }
 
-static __global__ __launch_bounds__(NT) void k_layer_scale(Consts c, Fields f, const double* __restrict__ el,
+template <typename S>
+__global__ __launch_bounds__(NT) void k_layer_scale(Consts c, FieldsT<S> f, const double* __restrict__ el,
                                                     double delta_e, double delta_e_acou)
 {
     layer_scale_body(c, f, el, delta_e, delta_e_acou, blockIdx.x, blockIdx.y);
@@ -2433,8 +2447,8 @@ struct AvgAcc {
     double n_add;
 };
 
-template <int HIST>
-__device__ __forceinline__ void avg_cells_body(const Consts& c, const Fields& f, double* partials, int maxblocks,
+template <int HIST, typename S>
+__device__ __forceinline__ void avg_cells_body(const Consts& c, const FieldsT<S>& f, double* partials, int maxblocks,
                                                   unsigned long long* dist1, int* dist3, int ravg, int bx)
 {
     __shared__ double red[NT];
@@ -2480,11 +2494,11 @@ __device__ __forceinline__ void avg_cells_body(const Consts& c, const Fields& f,
     }
 }
 
-template <int HIST>
-__global__ __launch_bounds__(NT) void k_avg_cells(Consts c, Fields f, double* partials, int maxblocks,
+template <int HIST, typename S>
+__global__ __launch_bounds__(NT) void k_avg_cells(Consts c, FieldsT<S> f, double* partials, int maxblocks,
                                                   unsigned long long* dist1, int* dist3, int ravg)
 {
-    avg_cells_body<HIST>(c, f, partials, maxblocks, dist1, dist3, ravg, blockIdx.x);
+    avg_cells_body<HIST, S>(c, f, partials, maxblocks, dist1, dist3, ravg, blockIdx.x);
 }
 
 __device__ __forceinline__ void avg_fold_body(const Scalars* sc, AvgAcc* acc,
@@ -2738,7 +2752,7 @@ template <typename T>
 struct Rep {
     Consts c;
     ScalarConsts k;
-    Fields f;
+    FieldsT<T> f;
     Spectra<T> sp;
     ZArgs<T> za;                 // acouK set; a z pass given want_acouK = 0 drops it
     XfwdArgs xa;
@@ -2833,7 +2847,7 @@ template <typename T, int KICK>
 __global__ __launch_bounds__(NT) void kb_kick(const Rep<T>* __restrict__ reps)
 {
     const Rep<T>& r = reps[blockIdx.z];
-    kick_body<KICK>(r.c, r.f, r.sc, r.partials, r.maxblocks, blockIdx.x);
+    kick_body<KICK, T>(r.c, r.f, r.sc, r.partials, r.maxblocks, blockIdx.x);
 }
 
 template <typename T>
@@ -2862,7 +2876,7 @@ template <typename T, int HIST>
 __global__ __launch_bounds__(NT) void kb_avg_cells(const Rep<T>* __restrict__ reps)
 {
     const Rep<T>& r = reps[blockIdx.z];
-    avg_cells_body<HIST>(r.c, r.f, r.partials, r.maxblocks, r.dist1, r.dist3, r.ravg, blockIdx.x);
+    avg_cells_body<HIST, T>(r.c, r.f, r.partials, r.maxblocks, r.dist1, r.dist3, r.ravg, blockIdx.x);
 }
 
 template <typename T>

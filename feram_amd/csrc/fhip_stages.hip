@@ -64,7 +64,7 @@ template <typename T, int n>
 int xfwd(fhip_ctx *ctx, const XfwdArgs &a)
 {
     Consts c = ctx->c;
-    Fields f = ctx->f;
+    FieldsT<T> f = fields<T>(ctx);
     Spectra<T> sp = tset<T>(ctx).sp;
     const Scalars *sc = ctx->dsc;
     const cpx<T> *tw = tset<T>(ctx).twx;
@@ -77,7 +77,7 @@ template <typename T, int n>
 int xinv(fhip_ctx *ctx, const XinvArgs &a, int nf)
 {
     Consts c = ctx->c;
-    Fields f = ctx->f;
+    FieldsT<T> f = fields<T>(ctx);
     Spectra<T> sp = tset<T>(ctx).sp;
     const Scalars *sc = ctx->dsc;
     const cpx<T> *tw = tset<T>(ctx).twx;

@@ -184,8 +184,8 @@ FP32_TOL = 2e-5
 
 @pytest.mark.parametrize("geometry,gap,method", [("epit", 0, "md"), ("film", 2, "lf")])
 def test_film_fp32_trajectory(oracle_mod, geometry, gap, method):
-    """FP32 spectra with FP64 state: the film path against the FP64 oracle
-    to the FP32 trajectory tolerance of the bulk tests."""
+    """FP32 mode (FP32 spectra and stored state): the film path against the
+    FP64 oracle to the FP32 trajectory tolerance of the bulk tests."""
     from feram_amd.system import System
     p = _params((16, 16, 16), geometry, gap, method)
     print(f"[gpu] fp32 {geometry} gap_id={gap} {method}", flush=True)

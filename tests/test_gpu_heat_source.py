@@ -63,7 +63,7 @@ def test_leapfrog_heat_source_matches_oracle(oracle_mod, acoustic):
 
 
 def test_leapfrog_heat_source_fp32(oracle_mod):
-    """FP32 spectra with FP64 state and FP64 layer sums, 10 steps."""
+    """FP32 mode (FP32 spectra and stored state) with FP64 layer sums, 10 steps."""
     from feram_amd.system import System
     print("[gpu] heat-source layers, fp32", flush=True)
     p = batio3_zvr((16, 16, 16), method="lf", kelvin=300.0)

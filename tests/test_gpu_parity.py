@@ -320,8 +320,9 @@ def test_128_cubed_force_is_gradient():
         assert fd == pytest.approx(F[idx], rel=1e-4, abs=1e-6 * np.max(np.abs(F)))
 
 
-# ---- FP32 mode: spectra, FFTs and the k-space stage in FP32; state, local
-# forces, thermostat and energy sums in FP64.  Bound from the FP32-mode target:
+# ---- FP32 mode: spectra, FFTs, the k-space stage and the stored state (u, p)
+# in FP32; local forces, kicks and drifts computed in FP64 registers;
+# thermostat and energy sums in FP64.  Bound from the FP32-mode target:
 # force error <= 1e-5 relative (normwise) against the FP64 oracle.
 FP32_TOL = 1e-5
 
