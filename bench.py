@@ -11,9 +11,9 @@ tables are resident in HBM before the timed region; K steps run back to back
 on the library's stream, bracketed by device synchronisation.
 
 Two modes are timed on the same workload: FP64 plain (everything FP64, the
-reference's arithmetic) and FP32 (spectra, FFTs and the k-space stage in
-FP32; state, local forces, thermostat, energy sums and homogeneous strain in
-FP64).  ``value`` is the FP64 plain-mode rate; the FP32 rate and the ratio
+reference's arithmetic) and FP32 (spectra, FFTs, the k-space stage and the
+stored state u, p in FP32; local forces, drift and kicks computed in FP64
+registers; thermostat, energy sums and homogeneous strain in FP64).  ``value`` is the FP64 plain-mode rate; the FP32 rate and the ratio
 are in ``modes``.  Each mode's force and energy error against the FP64 oracle
 (the C restatement of the reference) after the prologue and after the first
 two steps is reported per run in ``modes.<mode>.vs_oracle``: the oracle runs
@@ -24,7 +24,10 @@ its average launch time, taken with HIP events on the library's stream in a
 separate pass of max(K, 100) steps (direct launches) between the warm-up and
 the timed steps; ``roofline.copy_GBs`` is a 1 GiB device copy timed in the same run (the
 attainable HBM rate on this box).  ``cpu_baseline`` times the CPU FP64 oracle (OpenMP over
-the host threads) on a bounded sample of the same workload.
+the host threads) on a bounded sample of the same workload.  ``ensemble`` is
+the batched 48-replica sweep ({12,16,24,32}^3, T and E_z drawn per replica,
+graph-replayed steps) in each mode: replica-steps/s and the equilibrated
+replicas per hour at 5000 steps per replica.
 """
 from __future__ import annotations
 
@@ -200,6 +203,28 @@ def sweep_replicas(n: int, precision: int, seed: int = 20241203):
     temps = np.array([p.kelvin for p in params])
     fields = np.array([p.external_E_field[2] for p in params])
     return systems, edges, temps, fields
+
+
+def ensemble_rate(n: int, precision: int, steps: int, warmup: int) -> dict:
+    """Replica-steps/s of the n-replica batched sweep with graph-replayed
+    steps, and the equilibrated replicas per hour that gives at 5000 steps
+    per replica (the ensemble workload's own length)."""
+    from feram_amd.system import Batch
+    systems, edges, temps, fields = sweep_replicas(n, precision)
+    cells = int(sum(int(L) ** 3 for L in edges))
+    with Batch(systems) as bt:
+        bt.step(warmup)
+        t0 = time.perf_counter()
+        e = bt.step(steps)
+        dt = time.perf_counter() - t0
+    for g in systems:
+        g.close()
+    if not all(np.isfinite(v) for x in e for v in x.values()):
+        raise RuntimeError("non-finite energies after the timed ensemble steps")
+    rate = n * steps / dt
+    return {"replicas": n, "cells": cells, "steps": steps, "replica_steps_per_s": rate,
+            "ms_per_step": 1e3 * dt / steps, "cell_updates_per_s": cells * steps / dt,
+            "replicas_per_hour_at_5000_steps": 3600.0 * rate / 5000.0}
 
 
 def bench_replicas(args, sync_extra):
@@ -442,6 +467,8 @@ def main():
     ap.add_argument("--precision", choices=("both", "fp32", "fp64"), default="both")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 48-replica ensemble figure of the default workload")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
     ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16", "epit_hl"),
                     default="md128",
@@ -549,6 +576,19 @@ def main():
                 print(f"[bench] {name} vs oracle: force {modes[name]['vs_oracle']['force_rel_err_step2']:.2e}, "
                       f"energy {modes[name]['vs_oracle']['energy_rel_err_step2']:.2e}", file=sys.stderr, flush=True)
 
+    # the batched-ensemble metric (equilibrated replicas per hour) beside the
+    # single-supercell one: 48 replicas of {12,16,24,32}^3, graph-replayed
+    ensemble = None
+    if not args.no_ensemble:
+        ensemble = {}
+        for name, prec in (("fp64", 0), ("fp32", 1)):
+            if args.precision not in ("both", name):
+                continue
+            print(f"[bench] 48-replica ensemble {name}", file=sys.stderr, flush=True)
+            ensemble[name] = ensemble_rate(48, prec, 100, 20)
+            print(f"[bench] ensemble {name}: {ensemble[name]['replica_steps_per_s']:.0f} replica-steps/s",
+                  file=sys.stderr, flush=True)
+
     N = p.N
     out = {
         "metric": f"NP-MD steps/s at {L}^3 ({head_name.upper()} plain mode; the other mode alongside)",
@@ -573,6 +613,7 @@ def main():
                      "copy_GBs": copy_gbs},
         "step_roofline": head["step_roofline"],
         "modes": modes,
+        "ensemble": ensemble,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

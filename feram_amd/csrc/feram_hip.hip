@@ -1127,9 +1127,14 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
         for (int y = 0; y < ctx->Ly; ++y)
             for (int x = 0; x < ctx->Lx; ++x)
                 h[x + (size_t)ctx->Lx * (y + (size_t)ctx->Ly * z)] = mod[padded_index(ctx, 0, 1, x, y, z)];
-    // scratch: overwritten by the next force evaluation (the dV/du block holds
-    // 3N words of the state type, at least N doubles)
-    double *d = ctx->precision == FHIP_FP32 ? reinterpret_cast<double *>(ctx->f32.F[0]) : ctx->f.F[0];
+    // the field goes through a temporary device buffer, so the state (dV/du,
+    // which the next call's first kick reads, included) is left as it is
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * ctx->N));
+    struct FreeOnExit {
+        double *p;
+        ~FreeOnExit() { (void)hipFree(p); }
+    } scratch_guard{d};
     HIPCHK(hipMemcpyAsync(d, h.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
     const double *flds[1] = {d};
     {
@@ -1143,7 +1148,6 @@ int fhip_set_modulation(fhip_ctx *ctx, const double *mod)
         const int kb = (ctx->Kh + NT - 1) / NT;
         RC(launch(ctx, K_SETUP, [&] { k_unpack<double><<<kb, NT, 0, st>>>(c, s0, mk); }));
     }
-    HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * ctx->N, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->c.modulation = 1;
     return collect_events(ctx);

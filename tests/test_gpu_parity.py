@@ -386,6 +386,46 @@ def test_fp32_np_trajectory(oracle_mod):
     assert g.nose[0] == pytest.approx(o.nose[0], rel=FP32_TOL)
 
 
+def test_fp32_state_is_stored_in_fp32():
+    """FP32 mode stores u and p as FP32 words: a put rounds to nearest, a get
+    widens exactly, and the FP64 mode keeps every bit."""
+    p = batio3_zvr((8, 12, 6))
+    rng = np.random.default_rng(31)
+    u = rng.uniform(-0.1, 0.1, size=(p.Lz, p.Ly, p.Lx, 3))
+    pm = rng.standard_normal((p.Lz, p.Ly, p.Lx, 3)) * 1e-3
+    g32, g64 = _system32(p), _system(p)
+    for g in (g32, g64):
+        g.put("dipoR", u)
+        g.put("dipoP", pm)
+    assert np.array_equal(g32.get("dipoR"), u.astype(np.float32).astype(np.float64))
+    assert np.array_equal(g32.get("dipoP"), pm.astype(np.float32).astype(np.float64))
+    assert np.array_equal(g64.get("dipoR"), u) and np.array_equal(g64.get("dipoP"), pm)
+    g32.close()
+    g64.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32"])
+def test_set_modulation_leaves_the_state(precision):
+    """fhip_set_modulation is a set-up call: dV/du (which the next call's first
+    kick reads), u and p stay as they are."""
+    from feram_amd.system import System
+    g_ = load_golden()["modulation_check"]
+    p = param_from_dict(g_["params"])
+    mod = np.zeros((p.Lz, p.Ly, p.Lx))
+    for x, y, z, a in g_["modulation"]:
+        mod[z, y, x] = a
+    p.method = "md"                    # Nose-Poincare: its first kick reads the stored dV/du
+    g = System(p, precision=precision)
+    g.put("dipoR", _random(p, 12))
+    g.md_prologue()
+    g.nose_poincare(2)
+    before = {k: g.get(k) for k in ("dVddi", "dipoR", "dipoP")}
+    g.set_modulation(mod)
+    for k, v in before.items():
+        assert np.array_equal(g.get(k), v), k
+    g.close()
+
+
 def test_fp32_modulation_energies():
     g_ = load_golden()["modulation_check"]
     p = param_from_dict(g_["params"])
