@@ -950,6 +950,8 @@ struct ZTile {
     static constexpr int ST = TB + 1;
     static constexpr int EL = LZ * TB;
     static constexpr int PPT = (EL + NT - 1) / NT;
+    // the first and last pass of each z transform run in registers (see zpass_body)
+    static constexpr bool REG = NT % TB == 0 && EL % NT == 0 && dft_radix(PPT) && LZ % (PPT * PPT) == 0;
 };
 
 // Thread-to-element maps of a z tile (TB pencils of LZ points, NTH threads),
@@ -1225,7 +1227,7 @@ __device__ __forceinline__ void zpass_body(const Consts& c, const Spectra<T>& sp
         // holds the point (t0 + i S, b).  Otherwise the whole transform runs
         // in LDS and the k-space stage reads the tile in line order.
         constexpr int S = NT / TB;
-        constexpr bool REG = NT % TB == 0 && EL % NT == 0 && dft_radix(PPT) && LZ % (PPT * PPT) == 0;
+        constexpr bool REG = ZTile<T, LZ>::REG;
         const int zt0 = tid / TB, zb = tid % TB;
         // the point (t, b) that iteration i of this thread holds after the forward transforms
         auto pt_i = [&](int i, int& t, int& b) {
@@ -1548,9 +1550,10 @@ __device__ __forceinline__ void zpair_body(const Consts& c, const Spectra<T>& sp
 {
     __shared__ ZLds<T, LZ, zpair_buf_size<T, LZ, G2>()> lds;
     static_assert(sizeof(lds.buf) >= ZPF * ZTile<T, LZ>::EL * sizeof(cpx<T>), "prefetch staging exceeds the LDS pool");
-    // the first group prefetches the second group's first field batch
-    // (one Z*, no modulation spectrum)
-    constexpr bool PF = ZPrefetch<T, LZ>::OK && DDG == ZG_DD;
+    // the first group prefetches the second group's first field batch (one
+    // Z*, no modulation spectrum; only edges of 64 and more with register
+    // first/last passes, where it was measured: below, a step is launch-bound)
+    constexpr bool PF = ZPrefetch<T, LZ>::OK && DDG == ZG_DD && ZTile<T, LZ>::REG && LZ >= 64;
     cpx<T> pre[ZPF][ZTile<T, LZ>::PPT];
     if (PF && !c.modulation) {
         if (bx & 1) {
