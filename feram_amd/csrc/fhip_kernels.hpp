@@ -271,25 +271,28 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const FieldsT<T>& f, 
     if (a.phase == XP_MIRROR && row0 + RB <= mrow) return;
     for (int i = tid; i < LX; i += NT) tws[i] = twx[i];
 
-    double u[CPT][3], w[CPT][3], zs[CPT];
+    // the drifted u and w as stored (the state type), so the spectra are
+    // those of the stored state; FP32 mode: half the registers
+    T u[CPT][3], w[CPT][3];
+    double zs[CPT];
     bool ok[CPT];
-    size_t cell[CPT];
+    int cell[CPT];                                  // N < 2^31 for every supported supercell
     // every cell's code, displacement and momentum are loaded before any
     // drift is stored: a cell's loads cannot move above the previous cell's
     // stores to u, so loading in the drift loop waits once per cell
     const bool pre = a.phase != XP_MIRROR && (a.mode == XF_NP || a.mode == XF_LF || a.mode == XF_VS);
     unsigned char cpre[CPT];
-    double upre[CPT][3], ppre[CPT][3];
+    T upre[CPT][3], ppre[CPT][3];
 #pragma unroll
     for (int i = 0; i < CPT; ++i) {
         const int e = tid + i * NT;
         const int r = e / LX, x = e - r * LX;
         ok[i] = (CELLS % NT == 0 || e < CELLS) && (row0 + r < nrows)
                 && (a.phase == XP_ALL || ((row0 + r >= mrow) == (a.phase == XP_MIRROR)));
-        cell[i] = (size_t)(row0 + r) * LX + x;
+        cell[i] = (row0 + r) * LX + x;
         cpre[i] = 0;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) upre[i][q] = ppre[i][q] = 0.0;
+        for (int q = 0; q < 3; ++q) upre[i][q] = ppre[i][q] = T(0);
         if (pre && ok[i]) {
             const size_t cc = cell[i];
             cpre[i] = f.code[cc];
