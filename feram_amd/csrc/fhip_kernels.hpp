@@ -252,7 +252,10 @@ struct xfwd_rows {
 #define FHIP_XPAD 1
 #endif
 
-template <typename T, int LX, int RB = xfwd_rows<LX>::value>
+// GEN: the body can run the generic set-up transform (XF_GENERIC, input
+// fields a.gen); replica batches never do, and leaving it out keeps their
+// argument copy out of scratch memory.
+template <typename T, int LX, int RB = xfwd_rows<LX>::value, bool GEN = true>
 __device__ __forceinline__ void xfwd_body(const Consts& c, const FieldsT<T>& f, const Spectra<T>& sp, const XfwdArgs& a,
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* twx, int bx)
@@ -390,8 +393,8 @@ __device__ __forceinline__ void xfwd_body(const Consts& c, const FieldsT<T>& f, 
                     const int fi = f0 + h;
                     if (h == 1 && !two) break;
                     double v;
-                    if (a.mode == XF_GENERIC) {
-                        v = a.gen[fi][cell[i]];
+                    if (GEN && a.mode == XF_GENERIC) {
+                        if constexpr (GEN) v = a.gen[fi][cell[i]];
                     } else if (fi < 3) {
                         v = fi == 0 ? u[i][0] : (fi == 1 ? u[i][1] : u[i][2]);
                     } else if (fi < 9) {                                 // dipoFFT.F:20-33
@@ -2787,10 +2790,11 @@ template <typename T, int LX, int RB>
 __global__ __launch_bounds__(NT) void kb_xfwd(const Rep<T>* __restrict__ reps, int phase, int mode)
 {
     const Rep<T>& r = reps[blockIdx.z];
-    XfwdArgs a = r.xa;
+    XfwdArgs a{};                          // no generic input fields in a batch
+    a.mode = mode >= 0 ? mode : r.xa.mode; // XF_FORCE: no drift (the tuner's timing runs)
+    a.nf = r.xa.nf;
     a.phase = phase;
-    if (mode >= 0) a.mode = mode;          // XF_FORCE: no drift (the tuner's timing runs)
-    xfwd_body<T, LX, RB>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
+    xfwd_body<T, LX, RB, false>(r.c, r.f, r.sp, a, r.sc, r.twx, blockIdx.x);
 }
 
 template <typename T, int LY, int SIGN, int TB>
