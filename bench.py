@@ -403,6 +403,45 @@ def bench_heating16(args, sync_extra):
     print(json.dumps(out))
 
 
+def bench_heating48(args, sync_extra):
+    """The 48^3 acoustic heating scan: BaTiO3 bulk 48x48x48, FP64 plain mode,
+    velocity scaling with acoustic MD (acoustic_mass_amu 46.44 as the
+    reference's 26example-BaTiO3-acoustic-MD), 200 -> 400 K in 10 K steps,
+    2000 thermalising + 2000 averaging steps per temperature, each started
+    from the previous final state, seeded random initial u (init_dipo_avg
+    0.1, dev 0.02); one chain per seed (default one)."""
+    from feram_amd.params import batio3_zvr
+    from feram_amd.sweeps import temperature_chain
+    temps = np.arange(200, 401, 10)
+    seeds = tuple(range(1, args.seeds + 1))
+    prec = 1 if args.precision == "fp32" else 0
+    base = batio3_zvr(48, method="vs", kelvin=float(temps[0]), acoustic_mass_amu=46.44)
+    t0 = time.perf_counter()
+
+    def progress(i, T, u):
+        print(f"[bench] {T:.0f} K  t={time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    res = temperature_chain(temps, seeds=seeds, base=base, n_thermalize=args.scan_steps,
+                            n_average=args.avg_steps, precision=prec, progress=progress)
+    sync_extra()
+    dt = time.perf_counter() - t0
+    steps = len(temps) * (args.scan_steps + args.avg_steps)
+    rows = [{"T": T, "u_sorted": res["u_sorted_mean"][i].tolist(), "phases": res["phases"][i],
+             "strain": res["strain"][i].mean(axis=0).tolist()} for i, T in enumerate(res["temps"])]
+    out = {
+        "metric": "MD steps/s of the 48^3 acoustic heating scan (velocity scaling, acoustic MD)",
+        "value": len(seeds) * steps / dt, "unit": "steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * dt / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec else "f64", "data": "synthetic",
+        "config": {"workload": f"BaTiO3 bulk 48^3, -5 GPa, vs with acoustic MD, 200-400 K / 10 K, "
+                               f"{args.scan_steps}+{args.avg_steps} steps per temperature, chained, "
+                               f"{len(seeds)} seed(s)",
+                   "cells": 48 ** 3, "cell_updates_per_s": len(seeds) * steps * 48 ** 3 / dt},
+        "scan": rows, "wall_s": dt,
+    }
+    print(json.dumps(out))
+
+
 def bench_epit_hl(args, sync_extra):
     """The reference's epitaxial-film hysteresis example
     (11example-BaTiO3-new-hysteresis-loop/hysteresis16x16x128d0.feram):
@@ -470,11 +509,12 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the 48-replica ensemble figure of the default workload")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
-    ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16", "epit_hl"),
+    ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16", "heating48", "epit_hl"),
                     default="md128",
                     help="md128: the single-supercell NP-MD step (default); replicas: the "
                          "batched replica sweep; warmstart: the learned warm start on a heating scan; "
                          "heating16: the reference's 16^3 heating-scan example against its .avg output; "
+                         "heating48: the 48^3 acoustic (velocity-scaling) heating scan, 200-400 K; "
                          "epit_hl: the reference's 16x16x128 epit hysteresis-loop example")
     ap.add_argument("--hl-therm", type=int, default=20000)
     ap.add_argument("--hl-avg", type=int, default=3000000)
@@ -515,6 +555,15 @@ def main():
         return
     if args.workload == "epit_hl":
         bench_epit_hl(args, sync_extra)
+        return
+    if args.workload == "heating48":
+        if args.scan_steps == 4000:
+            args.scan_steps = 2000
+        if args.avg_steps == 10000:
+            args.avg_steps = 2000
+        if args.seeds == 4:
+            args.seeds = 1
+        bench_heating48(args, sync_extra)
         return
     if args.workload == "heating16":
         if args.scan_steps == 4000:
