@@ -27,7 +27,9 @@ attainable HBM rate on this box).  ``cpu_baseline`` times the CPU FP64 oracle (O
 the host threads) on a bounded sample of the same workload.  ``ensemble`` is
 the batched 48-replica sweep ({12,16,24,32}^3, T and E_z drawn per replica,
 graph-replayed steps) in each mode: replica-steps/s and the equilibrated
-replicas per hour at 5000 steps per replica.
+replicas per hour at 5000 steps per replica.  ``small_16cubed`` is the 16^3
+NP-MD case (FP64, 1000 steps in one call) beside the reference's published
+rate for that size.
 """
 from __future__ import annotations
 
@@ -203,6 +205,36 @@ def sweep_replicas(n: int, precision: int, seed: int = 20241203):
     temps = np.array([p.kelvin for p in params])
     fields = np.array([p.external_E_field[2] for p in params])
     return systems, edges, temps, fields
+
+
+# The reference's published NP-MD rate at 16^3: 40000 + 20000 steps
+# (18example-benchmark/md16x16x16.feram:32-33) in 23.22 s of simulation time
+# on one Xeon X5650 chip (18example-benchmark/md16x16x16.X5650.r1463.dat:4-5).
+REF_16_STEPS_PER_S = 60000 / 23.22
+
+
+def small_case(steps: int, warmup: int) -> dict:
+    """The 16^3 BaTiO3 NP-MD case (FP64, 300 K, u0 ~ U[-0.05, 0.05] A seeded)
+    on the device: one context, `steps` steps in one call after `warmup`."""
+    from feram_amd.params import batio3_zvr
+    from feram_amd.system import System
+    p = batio3_zvr(16, method="md", kelvin=300.0)
+    rng = np.random.default_rng(2024)
+    g = System(p)
+    g.put("dipoR", rng.uniform(-0.05, 0.05, size=(16, 16, 16, 3)))
+    g.md_prologue()
+    g.nose_poincare(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    e = g.nose_poincare(steps)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    g.close()
+    if not all(np.isfinite(v) for v in e.values()):
+        raise RuntimeError("non-finite energies after the timed 16^3 steps")
+    return {"L": 16, "steps": steps, "steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+            "reference_published_steps_per_s": REF_16_STEPS_PER_S,
+            "reference_published": "Xeon X5650, one chip (18example-benchmark/md16x16x16.X5650.r1463.dat)"}
 
 
 def ensemble_rate(n: int, precision: int, steps: int, warmup: int) -> dict:
@@ -507,7 +539,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true",
-                    help="skip the 48-replica ensemble figure of the default workload")
+                    help="skip the 48-replica ensemble and 16^3 figures of the default workload")
     ap.add_argument("--q-nose", type=float, default=0.0, help="<= 0: 1.0 * N / 64^3")
     ap.add_argument("--workload", choices=("md128", "replicas", "warmstart", "heating16", "heating48", "epit_hl"),
                     default="md128",
@@ -638,6 +670,12 @@ def main():
             print(f"[bench] ensemble {name}: {ensemble[name]['replica_steps_per_s']:.0f} replica-steps/s",
                   file=sys.stderr, flush=True)
 
+    small = None
+    if not args.no_ensemble:
+        print("[bench] 16^3 NP-MD", file=sys.stderr, flush=True)
+        small = small_case(1000, 100)
+        print(f"[bench] 16^3: {small['steps_per_s']:.0f} steps/s", file=sys.stderr, flush=True)
+
     N = p.N
     out = {
         "metric": f"NP-MD steps/s at {L}^3 ({head_name.upper()} plain mode; the other mode alongside)",
@@ -663,6 +701,7 @@ def main():
         "step_roofline": head["step_roofline"],
         "modes": modes,
         "ensemble": ensemble,
+        "small_16cubed": small,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
