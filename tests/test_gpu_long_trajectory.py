@@ -1,4 +1,5 @@
-"""1000-step trajectories of the HIP path against the FP64 oracle.
+"""1000-step trajectories of the HIP path against the FP64 oracle, and a long
+FP32-mode run against the FP64 mode.
 
 Two FP64 codes that differ only in rounding (FFT algorithm, summation order,
 FMA contraction) separate exponentially along a chaotic MD trajectory, about
@@ -49,3 +50,33 @@ def test_thousand_steps_track_the_oracle(oracle_mod, method):
         if method == "md":
             assert eg["H_Nose_Poincare"] == pytest.approx(e.H_Nose_Poincare, rel=1e-9)
     g.close()
+
+
+def test_fp32_long_run_conserves_like_fp64():
+    """4000 NP steps at 12^3 in the FP32 mode (FP32 spectra, stored state and
+    local-force accumulators) against the FP64 plain mode from the same start:
+    the two trajectories part after a few hundred steps, but the FP32 mode's
+    conserved Nose-Poincare Hamiltonian must wander no further than the FP64
+    mode's (its range over the run within 1.5x; the time step, not the
+    rounding, sets it) and the mean kinetic energy of the second half must
+    agree within 5 %."""
+    from feram_amd.system import System
+    print("[gpu] 4000-step NP run, 12^3, FP32 against FP64", flush=True)
+    p = batio3_zvr(12, method="md", kelvin=300.0)
+    u = 0.08 + 0.06 * np.random.default_rng(21).standard_normal((p.Lz, p.Ly, p.Lx, 3))
+    out = {}
+    for prec in (0, 1):
+        g = System(p, precision=prec)
+        g.put("dipoR", u)
+        g.md_prologue()
+        H, KE = [], []
+        for _ in range(40):
+            e = g.nose_poincare(100)
+            H.append(e["H_Nose_Poincare"])
+            KE.append(e["dipo_kinetic"])
+        g.close()
+        assert np.all(np.isfinite(H)) and np.all(np.isfinite(KE))
+        out[prec] = (np.ptp(H), np.mean(KE[20:]))
+    print(f"  H range fp64 {out[0][0]:.3e}, fp32 {out[1][0]:.3e}; KE fp64 {out[0][1]:.4e}, fp32 {out[1][1]:.4e}")
+    assert out[1][0] <= 1.5 * out[0][0]
+    assert out[1][1] == pytest.approx(out[0][1], rel=0.05)
