@@ -37,7 +37,7 @@ inline int xfwd_tile_rows(int n) { const int t = tile_rows(n); return t > 1 ? t 
 template <typename T>
 inline int line_tile_rows(int n) { return tile_rows(n, 16384 / (int)sizeof(cpx<T>)); }
 template <typename T>
-inline int ztile_rows(int n) { return std::min(line_tile_rows<T>(n), ZTILE_MAX); }
+inline int ztile_rows(int n) { return ztile_pencils(n, (int)sizeof(cpx<T>)); }
 
 // Dynamic LDS of a z pass (bytes): the side buffer of the kx = 0 pencils, two
 // nf-field values per point of up to (TB - 1) / Mx + 1 such pencils per tile.

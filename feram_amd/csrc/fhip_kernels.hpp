@@ -949,10 +949,30 @@ template <> struct kspace_type<float> { using type = float; };
 // that the host sizes as dynamic LDS.
 constexpr int ZTILE_MAX = 64;
 constexpr int ZSTRAIN_FB = 3;   // strain fields transformed together (DD and acoustic groups: 3)
+
+// Pencils per z tile for an edge LZ and complex points of CB bytes (host and
+// device): the y pass's 16 KB tile, at most ZTILE_MAX pencils.  Edges of 32
+// and less take instead the fewest pencils that give every thread the same
+// number of points (LZ TB a multiple of NT) in full 128-byte rows, so that a
+// small supercell's z pass spreads over more workgroups (16^3 FP64: 8
+// instead of 2); edges with a factor 3 or 5 keep the larger tile, which no
+// smaller one divides evenly among the threads.
+constexpr int ztile_pencils(int LZ, int CB)
+{
+    int tb = 1;
+    while (2 * tb * LZ <= 16384 / CB) tb *= 2;          // tile_of<LZ, 16 KB>
+    if (tb > ZTILE_MAX) tb = ZTILE_MAX;
+    if (LZ <= 32) {
+        int small = 128 / CB;                            // one 128-byte row
+        while (small < tb && (small * LZ) % NT != 0) small *= 2;
+        if (small < tb) tb = small;
+    }
+    return tb;
+}
+
 template <typename T, int LZ>
 struct ZTile {
-    static constexpr int TB = tile_of<LZ, line_tile_cap<T>()>::value < ZTILE_MAX
-                                  ? tile_of<LZ, line_tile_cap<T>()>::value : ZTILE_MAX;
+    static constexpr int TB = ztile_pencils(LZ, (int)sizeof(cpx<T>));
     static constexpr int ST = TB + 1;
     static constexpr int EL = LZ * TB;
     static constexpr int PPT = (EL + NT - 1) / NT;
