@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -986,6 +987,11 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     }
     ctx->kblocks = (ctx->N + NT - 1) / NT;
     ctx->maxblocks = std::max(std::max(ctx->xblocks, std::max(ctx->zblocks[0], ctx->zblocks[1])), ctx->kblocks);
+    {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
 
     const size_t N = ctx->N, Kh = ctx->Kh;
     // u, p, pk, F (3 each); w, p_w, dV/dw only with acoustic MD; FP32 in

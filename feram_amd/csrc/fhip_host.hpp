@@ -67,6 +67,8 @@ struct fhip_ctx {
     size_t Mh = 0;
     int xblocks = 0, xfblocks = 0, kblocks = 0, maxblocks = 0;   // xinv, xfwd, cell-wise grids
     int yblocks[2] = {0, 0}, zblocks[2] = {0, 0};   // per spectral type: [0] FP64, [1] FP32
+    int zsplit[2] = {-1, -1};         // per spectral type: the z pair runs its groups in separate workgroups (-1: not chosen yet)
+    int ncu = 0;                      // compute units of the context's device
     hipStream_t stream = nullptr;
     std::string err;
     fhip::Consts c{};

@@ -621,6 +621,7 @@ struct ZArgs {
     int maxblocks;
     int nplain;                      // ZG_PLAIN: number of slots
     int plain_sign;                  // ZG_PLAIN: -1 forward, +1 inverse
+    int split;                       // pair kernel: workgroup 2w + g runs group g of tile w alone
 };
 
 // coup2(a, i) (elastic.F:148-175)
@@ -1581,6 +1582,14 @@ __device__ __forceinline__ void zpair_body(const Consts& c, const Spectra<T>& sp
     // first/last passes, where it was measured: below, a step is launch-bound)
     constexpr bool PF = ZPrefetch<T, LZ>::OK && DDG == ZG_DD && ZTile<T, LZ>::REG && LZ >= 64;
     cpx<T> pre[ZPF][ZTile<T, LZ>::PPT];
+    if (a.split) {
+        // a grid of fewer tiles than the device has CUs: the two groups of a
+        // tile run side by side in two workgroups instead of one after the
+        // other (the partial sums are indexed by tile either way)
+        if (bx & 1) zpass_body<T, LZ, G2, 0, -1>(c, sp, a, sc, twz, bx >> 1, lds, pre);
+        else zpass_body<T, LZ, DDG, 0, -1>(c, sp, a, sc, twz, bx >> 1, lds, pre);
+        return;
+    }
     if (PF && !c.modulation) {
         if (bx & 1) {
             zpass_body<T, LZ, G2, 0, PF ? DDG : -1>(c, sp, a, sc, twz, bx, lds, pre);

@@ -7,6 +7,9 @@
 // dispatch on the edge and call the instantiation of that edge.
 #include "fhip_host.hpp"
 
+#include <algorithm>
+#include <cstdlib>
+
 using namespace fhip;
 using namespace fhip_host;
 
@@ -121,10 +124,26 @@ int zpair(fhip_ctx *ctx, const ZArgs<T> &a, size_t shm)
     Spectra<T> sp = tset<T>(ctx).sp;
     Scalars *sc = ctx->dsc;
     const cpx<T> *tw = tset<T>(ctx).twz;
-    const int nb = ctx->zblocks[tix<T>()];
     hipStream_t s = ctx->stream;
-    const ZArgs<T> za = a;
     RC(ensure_lds(ctx, (const void *)k_zpair<T, n, G2, DDG>, shm));
+    // The two groups of a tile run one after the other in one workgroup, or
+    // side by side in two where the grid then needs fewer rounds of the
+    // device's workgroup slots (counted in half-tile durations): grids of
+    // fewer tiles than slots, or just over a whole number of rounds.
+    // FHIP_ZSPLIT=0 or 1 overrides the choice.
+    int &zsplit = ctx->zsplit[tix<T>()];
+    if (zsplit < 0) {
+        int per_cu = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_zpair<T, n, G2, DDG>, NT, shm));
+        const long slots = (long)std::max(per_cu, 1) * std::max(ctx->ncu, 1);
+        const long tiles = ctx->zblocks[tix<T>()];
+        const long whole = (tiles + slots - 1) / slots, halves = (2 * tiles + slots - 1) / slots;
+        const char *zs = std::getenv("FHIP_ZSPLIT");
+        zsplit = (zs && (zs[0] == '0' || zs[0] == '1')) ? zs[0] == '1' : halves < 2 * whole;
+    }
+    ZArgs<T> za = a;
+    za.split = zsplit;
+    const int nb = ctx->zblocks[tix<T>()] * (zsplit ? 2 : 1);
     return launch(ctx, K_ZPAIR, [&] { k_zpair<T, n, G2, DDG><<<nb, NT, shm, s>>>(c, sp, za, sc, tw); });
 }
 

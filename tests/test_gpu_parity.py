@@ -506,3 +506,31 @@ def test_species_zstar_fp32(oracle_mod):
     assert _normwise(g.get("dVddi"), o.field("dVddi")) <= FP32_TOL
     _check_energies(eg, eo, tol=FP32_TOL)
     g.close()
+
+
+# The paired z kernel runs the two groups of a tile one after the other in one
+# workgroup, or side by side in two (chosen from the grid's size;
+# FHIP_ZSPLIT=0/1 forces either); both compute every point the same way.
+@pytest.mark.parametrize("L,method,acoustic,precision", [
+    (16, "vs", 46.44, 0), (64, "md", None, 0), ((12, 6, 128), "md", None, 0), (64, "md", None, 1)])
+def test_z_pair_split_matches_unsplit(monkeypatch, L, method, acoustic, precision):
+    from feram_amd.system import System
+    p = batio3_zvr(L, method=method, kelvin=300.0, acoustic_mass_amu=acoustic)
+    p.Q_Nose = 1.0
+    u = _random(p, 7)
+    out = []
+    for zs in ("0", "1"):
+        monkeypatch.setenv("FHIP_ZSPLIT", zs)
+        print(f"[gpu] z pair split={zs} L={L} {method} precision={precision}", flush=True)
+        g = System(p, precision=precision)
+        g.put("dipoR", u)
+        g.md_prologue()
+        e = (g.velocity_scaling if method == "vs" else g.nose_poincare)(5)
+        names = ("dipoR", "dipoP") + (("acouR", "acouP") if acoustic else ())
+        out.append(({n: g.get(n) for n in names}, e))
+        g.close()
+    tol = 1e-12 if precision == 0 else FP32_TOL
+    (f0, e0), (f1, e1) = out
+    for n in f0:
+        assert _normwise(f1[n], f0[n]) <= tol, n
+    _check_energies(e1, e0, tol=tol)
