@@ -11,7 +11,9 @@
 // sequences; they are captured once into a HIP graph and replayed.
 #include "fhip_host.hpp"
 
+#include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -39,6 +41,7 @@ struct BClass {
     int zsp = 0;                        // species with different Z*: split dipole-dipole z group
     int xblocks = 0, xfblocks = 0, yblocks = 0, zblocks = 0, kblocks = 0;
     size_t zshm_g = 0;                  // dynamic LDS of the paired z pass
+    int zsplit = 0;                     // the paired z pass runs a tile's groups in separate workgroups
     bool film = false;                  // a member mirrors its upper layers (film/epit)
     int mblocks = 0;                    // blocks of the mirrored-layer kick
     // tile-shape plan: rows per xfwd workgroup and lines per y tile, each the
@@ -142,7 +145,10 @@ int upload_reps(fhip_batch *b, BClass &k)
     const size_t n = k.members.size();
     k.hreps.resize(n * sizeof(Rep<T>));
     Rep<T> *h = reinterpret_cast<Rep<T> *>(k.hreps.data());
-    for (size_t i = 0; i < n; ++i) fill_rep<T>(h[i], k.members[i], b->method);
+    for (size_t i = 0; i < n; ++i) {
+        fill_rep<T>(h[i], k.members[i], b->method);
+        h[i].za.split = k.zsplit;
+    }
     if (!k.dreps) BHIP(hipMalloc(&k.dreps, k.hreps.size()));
     BHIP(hipMemcpy(k.dreps, k.hreps.data(), k.hreps.size(), hipMemcpyHostToDevice));
     return FHIP_OK;
@@ -215,7 +221,7 @@ template <typename T, int G2, int DDG>
 int l_zpair(fhip_batch *b, BClass &k, hipStream_t s, int want_acouK)
 {
     const Rep<T> *r = (const Rep<T> *)k.dreps;
-    const dim3 g(k.zblocks, 1, (unsigned)k.members.size());
+    const dim3 g(k.zblocks * (k.zsplit ? 2 : 1), 1, (unsigned)k.members.size());
     switch (k.Lz) {
 #define FHIP_CASE(n) \
     case n: fhip_stage::b_zpair<T, n, G2, DDG>(r, g, s, k.zshm_g, want_acouK); BLAUNCH("zpair");
@@ -508,6 +514,17 @@ int prepare_lds(fhip_batch *b, BClass &k)
                            : (k.acoustic ? zpair_fn<T, ZG_ACOU, ZG_DD>(k.Lz) : zpair_fn<T, ZG_STRAIN, ZG_DD>(k.Lz));
     int rc = ensure_lds(ctx, fn, k.zshm_g);
     if (rc != FHIP_OK) return bfail(b, rc, "z pass: LDS attribute");
+    // The two groups of a tile run in separate workgroups when the z tiles of
+    // every class of the batch, so split, fit in one round of this kernel's
+    // workgroup slots (the classes' z passes run at the same time on their
+    // streams); FHIP_ZSPLIT=0 or 1 overrides the choice.
+    long tiles = 0;
+    for (const BClass &o : b->classes) tiles += (long)o.zblocks * (long)o.members.size();
+    int per_cu = 0;
+    BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, k.zshm_g));
+    const long slots = (long)std::max(per_cu, 1) * std::max(k.members[0]->ncu, 1);
+    const char *zs = std::getenv("FHIP_ZSPLIT");
+    k.zsplit = (zs && (zs[0] == '0' || zs[0] == '1')) ? zs[0] == '1' : 2 * tiles <= slots;
     return FHIP_OK;
 }
 
