@@ -517,14 +517,16 @@ int prepare_lds(fhip_batch *b, BClass &k)
     // The two groups of a tile run in separate workgroups when the z tiles of
     // every class of the batch, so split, fit in one round of this kernel's
     // workgroup slots (the classes' z passes run at the same time on their
-    // streams); FHIP_ZSPLIT=0 or 1 overrides the choice.
+    // streams), and always where the paired kernel has no prefetch of its
+    // second group to give up (z edges below 64, species of different Z*);
+    // FHIP_ZSPLIT=0 or 1 overrides the choice.
     long tiles = 0;
     for (const BClass &o : b->classes) tiles += (long)o.zblocks * (long)o.members.size();
     int per_cu = 0;
     BHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, k.zshm_g));
     const long slots = (long)std::max(per_cu, 1) * std::max(k.members[0]->ncu, 1);
     const char *zs = std::getenv("FHIP_ZSPLIT");
-    k.zsplit = (zs && (zs[0] == '0' || zs[0] == '1')) ? zs[0] == '1' : 2 * tiles <= slots;
+    k.zsplit = (zs && (zs[0] == '0' || zs[0] == '1')) ? zs[0] == '1' : (2 * tiles <= slots || k.Lz < 64 || k.zsp);
     return FHIP_OK;
 }
 
