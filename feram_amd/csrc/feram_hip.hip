@@ -977,7 +977,7 @@ int fhip_create(const fhip_params *p, fhip_ctx **out)
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     fill_consts(ctx);
     ctx->nslots = nforward(ctx);
-    ctx->xblocks = (ctx->Ly * ctx->Lz + tile_rows(ctx->Lx) - 1) / tile_rows(ctx->Lx);
+    ctx->xblocks = (ctx->Ly * ctx->Lz + xinv_tile_rows(ctx->Lx) - 1) / xinv_tile_rows(ctx->Lx);
     ctx->xfblocks = (ctx->Ly * ctx->Lz + xfwd_tile_rows(ctx->Lx) - 1) / xfwd_tile_rows(ctx->Lx);
     for (int t = 0; t < 2; ++t) {
         const int ty = t ? line_tile_rows<float>(ctx->Ly) : line_tile_rows<double>(ctx->Ly);
@@ -1407,7 +1407,7 @@ int fhip_force_static(const fhip_params *p, const double *long_range, const doub
     ctx.stream = (hipStream_t)stream;
     fill_consts(&ctx);
     ctx.nslots = 9;
-    ctx.xblocks = (ctx.Ly * ctx.Lz + tile_rows(ctx.Lx) - 1) / tile_rows(ctx.Lx);
+    ctx.xblocks = (ctx.Ly * ctx.Lz + xinv_tile_rows(ctx.Lx) - 1) / xinv_tile_rows(ctx.Lx);
     ctx.xfblocks = (ctx.Ly * ctx.Lz + xfwd_tile_rows(ctx.Lx) - 1) / xfwd_tile_rows(ctx.Lx);
     ctx.yblocks[0] = (ctx.Mx * ctx.Lz + line_tile_rows<double>(ctx.Ly) - 1) / line_tile_rows<double>(ctx.Ly);
     ctx.zblocks[0] = (ctx.Mx * ctx.Ly + ztile_rows<double>(ctx.Lz) - 1) / ztile_rows<double>(ctx.Lz);

@@ -29,6 +29,9 @@ inline int tile_rows(int n, int cap = 1024)   // must agree with tile_of<n, cap>
     return p;
 }
 
+// Rows per xinv workgroup (xinv_rows_of on the device).
+inline int xinv_tile_rows(int n) { return fhip::xinv_rows_of(n); }
+
 // Rows per xfwd workgroup (xfwd_rows<n> on the device).
 inline int xfwd_tile_rows(int n) { const int t = tile_rows(n); return t > 1 ? t / 2 : 1; }
 

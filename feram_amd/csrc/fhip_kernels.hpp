@@ -238,7 +238,19 @@ struct XfwdArgs {
     const double* gen[12];        // XF_GENERIC: real input fields
 };
 
-// Rows per xfwd workgroup: half the x-transform tile of xinv.  Shorter
+// Rows per xinv workgroup: the x-transform tile (tile_of<LX>), cut for
+// power-of-two edges up to 32 to the rows that give every thread one cell, so
+// that a small supercell's x-inverse pass spreads over more workgroups (16^3:
+// 16 instead of 4).
+constexpr int xinv_rows_of(int LX)
+{
+    int rb = 1;
+    while (2 * rb * LX <= 1024) rb *= 2;          // tile_of<LX>
+    if (LX <= 32 && (LX & (LX - 1)) == 0 && rb * LX > NT) rb = NT / LX;
+    return rb;
+}
+
+// Rows per xfwd workgroup: half the x-transform tile (tile_of<LX>).  Shorter
 // drift-and-transform chains per workgroup overlap better (128^3: FP32
 // 50 -> 43 us, FP64 61 -> 55 us); xinv keeps the full tile.
 template <int LX>
@@ -1642,7 +1654,7 @@ __device__ __forceinline__ void xinv_body(const Consts& c, const FieldsT<T>& f, 
                                              const Scalars* __restrict__ sc,
                                              const cpx<T>* twx, int bx)
 {
-    constexpr int RB = tile_of<LX>::value;
+    constexpr int RB = xinv_rows_of(LX);
     constexpr int ST = RB + FHIP_XPAD;
     constexpr int CELLS = RB * LX;
     constexpr int MX = LX / 2;
@@ -1703,7 +1715,7 @@ __device__ __forceinline__ void xinv_step_body(const Consts& c, const FieldsT<T>
                                                const cpx<T>* twx, int bx)
 {
     static_assert(NF == 9 || NF == 12 || NF == 15, "step fields");
-    constexpr int RB = tile_of<LX>::value;          // rows per workgroup
+    constexpr int RB = xinv_rows_of(LX);            // rows per workgroup
     constexpr int PB = 2;                           // field pairs per LDS transform
     constexpr int NL = PB * RB;                     // lines of a full batch
     constexpr int ST = NL + FHIP_XPAD;
