@@ -239,14 +239,18 @@ struct XfwdArgs {
 };
 
 // Rows per xinv workgroup: the x-transform tile (tile_of<LX>), cut for
-// power-of-two edges up to 32 to the rows that give every thread one cell, so
-// that a small supercell's x-inverse pass spreads over more workgroups (16^3:
-// 16 instead of 4).
+// edges up to 32 to the fewest rows (a power of two) that give every thread
+// at least one cell, so that a small supercell's x-inverse pass spreads over
+// more workgroups (16^3: 16 instead of 4).
 constexpr int xinv_rows_of(int LX)
 {
     int rb = 1;
     while (2 * rb * LX <= 1024) rb *= 2;          // tile_of<LX>
-    if (LX <= 32 && (LX & (LX - 1)) == 0 && rb * LX > NT) rb = NT / LX;
+    if (LX <= 32) {
+        int small = 1;
+        while (small * LX < NT) small *= 2;
+        if (small < rb) rb = small;
+    }
     return rb;
 }
 
